@@ -1,0 +1,58 @@
+#pragma once
+
+// Arguments of the HNSW beam kernel (hnsw_beam.hip), shared with its host-side launcher.
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace vdx::dev {
+
+// Layout-identical to vecdex::Neighbor and vdx_neighbor.
+struct Neighbor {
+    float distance;
+    std::uint32_t id;
+};
+
+struct BeamArgs {
+    // the graph, in HBM
+    const float* vectors;          // n rows of ld floats (dim used, the rest zero)
+    const std::uint32_t* layer0;   // n blocks of b0 = 1 + 2M
+    const std::uint32_t* upper;    // U blocks of bu = 1 + M, FORMAT order
+    const std::uint32_t* upper_offset;  // n: first upper block of each node
+    std::uint32_t ld, dim, b0, bu;
+    std::uint32_t n, max_level, entry_point, l2;  // l2: 1 for L2, 0 for 1 - <a, b>
+    // dimension split in the distance order: nb 32-blocks, n8 8-blocks, tail elements
+    std::uint32_t nb, n8, tail, first_tail;
+    // the queries (prepared), per-query k and ef
+    const float* queries;
+    std::uint32_t qld, nq;
+    const std::uint32_t* ks;
+    const std::uint32_t* efs;
+    std::uint32_t k, ef;
+    // results
+    Neighbor* out;
+    std::uint64_t out_stride;
+    std::uint32_t* out_count;
+    // launch state
+    std::uint32_t* queue;  // next query to take; zero before the launch
+    std::uint32_t ef_cap;  // R capacity per wave (>= every max(ef, k)), a multiple of 64
+    std::uint32_t visited_log2;
+    std::uint32_t lds_per_wave;
+    unsigned long long* stats;  // 4 counters, or null
+};
+
+enum class BeamShape { D128, D100, Generic };
+
+// Bytes of LDS one wave uses for a given R capacity and visited-cache size.
+inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {
+    return 8u * static_cast<std::size_t>(ef_cap) + 8u * 64 + 4u * 64 + 4u * 64 +
+           (std::size_t{4} << visited_log2);
+}
+
+hipError_t launch_hnsw_beam(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
+                            std::size_t lds_bytes, hipStream_t stream, BeamShape shape);
+const void* hnsw_beam_kernel_ptr(BeamShape shape);
+
+}  // namespace vdx::dev

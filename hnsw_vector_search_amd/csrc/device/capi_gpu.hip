@@ -1,0 +1,582 @@
+// The GPU half of the C ABI (include/vdxgpu.h): device contexts, the HBM copies of a graph and
+// of a brute-force base, and the search entry points that replace HnswIndex::search/search_batch
+// and BruteForceIndex::search.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../host/capi_common.hpp"
+#include "../host/validate.hpp"
+#include "beam_args.hpp"
+#include "bf_args.hpp"
+#include "gpu_state.hpp"
+
+using vdx::guarded;
+using vdx::require;
+using vdx::gpu::check;
+using vdx::gpu::DeviceBuffer;
+using vdx::gpu::DeviceGuard;
+using vdx::gpu::WorkspaceLease;
+
+struct vdx_gpu_hnsw {
+    vdx_gpu_ctx* ctx = nullptr;
+    std::uint32_t metric = 0;
+    std::uint64_t dim = 0, n = 0, M = 0, max_level = 0, entry_point = 0;
+    vdx::gpu::DimSplit split{};
+    DeviceBuffer vectors, layer0, upper, upper_offset;
+    // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in
+    // turn, so up to kRingSlots calls in flight on different streams never share one.
+    DeviceBuffer ring;
+    std::atomic<std::uint32_t> next_slot{0};
+};
+
+struct vdx_gpu_bf {
+    vdx_gpu_ctx* ctx = nullptr;
+    std::uint32_t metric = 0;
+    std::uint64_t dim = 0, n = 0;
+    vdx::gpu::DimSplit split{};
+    DeviceBuffer vectors;
+};
+
+namespace {
+
+constexpr std::uint32_t kMaxGpuDim = 287;    // 8 full 32-blocks, 3 8-blocks, 7 tail elements
+constexpr std::uint32_t kMaxBeamEf = 8192;   // R capacity per wave in LDS
+constexpr std::uint32_t kMaxGpuM = 32;       // a layer-0 block fits one wave
+constexpr std::uint32_t kMaxBfK = 2048;
+constexpr std::uint32_t kRingSlots = 64;
+
+vdx::dev::BeamShape beam_shape(std::uint64_t dim) {
+    if (dim == 128) return vdx::dev::BeamShape::D128;
+    if (dim == 100) return vdx::dev::BeamShape::D100;
+    return vdx::dev::BeamShape::Generic;
+}
+
+int bf_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 2); }
+
+// Copies `rows` rows of `dim` floats into a device buffer of rows of `ld` floats (zero padded).
+void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::uint64_t dim,
+                 std::uint64_t ld, hipStream_t stream) {
+    dst.resize(std::max<std::uint64_t>(1, rows) * ld * sizeof(float));
+    if (rows == 0) {
+        return;
+    }
+    if (ld == dim) {
+        check(hipMemcpyAsync(dst.as<float>(), src, rows * dim * sizeof(float), hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync(vectors)");
+    } else {
+        check(hipMemsetAsync(dst.as<float>(), 0, rows * ld * sizeof(float), stream), "hipMemsetAsync");
+        check(hipMemcpy2DAsync(dst.as<float>(), ld * sizeof(float), src, dim * sizeof(float),
+                               dim * sizeof(float), rows, hipMemcpyHostToDevice, stream),
+              "hipMemcpy2DAsync(vectors)");
+    }
+}
+
+std::uint32_t default_visited_log2(std::uint32_t ef_cap) {
+    if (ef_cap <= 128) return 11;
+    if (ef_cap <= 512) return 12;
+    return 13;
+}
+
+struct BeamPlan {
+    std::uint32_t ef_cap, visited_log2, waves, grid;
+    std::size_t lds_per_wave, lds_bytes;
+};
+
+BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t nq,
+                   const vdx_search_opts* opts) {
+    BeamPlan p{};
+    p.ef_cap = (max_efp + 63u) / 64u * 64u;
+    p.visited_log2 = opts && opts->visited_log2 ? opts->visited_log2 : default_visited_log2(p.ef_cap);
+    require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
+    p.waves = opts && opts->waves_per_group ? opts->waves_per_group : 4;
+    require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
+    p.lds_per_wave = vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
+    p.lds_bytes = p.lds_per_wave * p.waves;
+    if (p.lds_bytes > 160 * 1024) {
+        p.waves = 1;
+        p.lds_bytes = p.lds_per_wave;
+    }
+    require(p.lds_bytes <= 160 * 1024, "ef too large for the GPU beam's LDS");
+    int per_cu = 0;
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim)), p.waves * 64,
+              p.lds_bytes),
+          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    per_cu = std::max(per_cu, 1);
+    const std::uint64_t resident = static_cast<std::uint64_t>(per_cu) * h.ctx->cu_count;
+    const std::uint64_t needed = (nq + p.waves - 1) / p.waves;
+    p.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, std::min(resident, needed)));
+    return p;
+}
+
+// Queues one beam launch over prepared device queries.
+void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, const std::uint32_t* d_ks,
+                  std::uint32_t k, const std::uint32_t* d_efs, std::uint32_t ef,
+                  std::uint32_t max_efp, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
+                  std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
+                  const vdx_search_opts* opts, hipStream_t stream) {
+    require(nq <= 0xFFFFFFFFull, "too many queries in one call");
+    const BeamPlan plan = plan_beam(h, max_efp, nq, opts);
+    vdx::dev::BeamArgs a{};
+    a.vectors = h.vectors.as<float>();
+    a.layer0 = h.layer0.as<std::uint32_t>();
+    a.upper = h.upper.as<std::uint32_t>();
+    a.upper_offset = h.upper_offset.as<std::uint32_t>();
+    a.ld = h.split.ld;
+    a.dim = static_cast<std::uint32_t>(h.dim);
+    a.b0 = static_cast<std::uint32_t>(1 + 2 * h.M);
+    a.bu = static_cast<std::uint32_t>(1 + h.M);
+    a.n = static_cast<std::uint32_t>(h.n);
+    a.max_level = static_cast<std::uint32_t>(h.max_level);
+    a.entry_point = static_cast<std::uint32_t>(h.entry_point);
+    a.l2 = h.metric == 0 ? 1u : 0u;
+    a.nb = h.split.nb;
+    a.n8 = h.split.n8;
+    a.tail = h.split.tail;
+    a.first_tail = h.split.first_tail;
+    a.queries = d_queries;
+    a.qld = static_cast<std::uint32_t>(h.dim);
+    a.nq = static_cast<std::uint32_t>(nq);
+    a.ks = d_ks;
+    a.efs = d_efs;
+    a.k = k;
+    a.ef = ef;
+    a.out = d_out;
+    a.out_stride = out_stride;
+    a.out_count = d_count;
+    a.queue = d_queue;
+    a.ef_cap = plan.ef_cap;
+    a.visited_log2 = plan.visited_log2;
+    a.lds_per_wave = static_cast<std::uint32_t>(plan.lds_per_wave);
+    a.stats = d_stats;
+    check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
+    check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
+                                     beam_shape(h.dim)),
+          "hnsw_beam launch");
+}
+
+// Validates and prepares nq host queries into pinned staging (rows of dim floats).
+void prepare_queries(const char* where, vdx::Metric metric, std::uint64_t dim, std::uint64_t size,
+                     const float* queries, std::uint64_t nq, std::uint64_t query_dim,
+                     const std::uint32_t* ks, std::uint32_t k, float* staging) {
+    const vdx::IndexShape shape{metric, static_cast<std::size_t>(dim), static_cast<std::size_t>(size)};
+    for (std::uint64_t q = 0; q < nq; ++q) {
+        const float* row = queries + q * query_dim;
+        vdx::check_query(where, shape, row, query_dim, ks ? ks[q] : k);
+        vdx::prepare_query(metric, row, dim, staging + q * dim);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vdx_gpu_open(int device, vdx_gpu_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, "vdx_gpu_open: out is null");
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) {
+            throw std::invalid_argument(vdx::fmt("vdx_gpu_open: device %d does not exist (%d visible)",
+                                                 device, count));
+        }
+        auto ctx = std::make_unique<vdx_gpu_ctx>();
+        ctx->device = device;
+        DeviceGuard guard(device);
+        hipDeviceProp_t prop{};
+        check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        ctx->cu_count = prop.multiProcessorCount;
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        *out = ctx.release();
+    });
+}
+
+void vdx_gpu_close(vdx_gpu_ctx* ctx) {
+    if (ctx != nullptr) {
+        DeviceGuard guard(ctx->device);
+        delete ctx;
+    }
+}
+
+void* vdx_gpu_stream(vdx_gpu_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw** out) {
+    return guarded([&] {
+        require(ctx != nullptr && g != nullptr && out != nullptr, "vdx_gpu_hnsw_upload: null argument");
+        require(vdx::valid_metric(g->metric), "vdx_gpu_hnsw_upload: unknown metric code");
+        require(g->dim >= 1, "vdx_gpu_hnsw_upload: dimension is 0");
+        if (g->dim > kMaxGpuDim) {
+            throw std::invalid_argument(vdx::fmt(
+                "vdx_gpu_hnsw_upload: dimension %llu is above the GPU path's limit of %u",
+                static_cast<unsigned long long>(g->dim), kMaxGpuDim));
+        }
+        if (g->M < 2 || g->M > kMaxGpuM) {
+            throw std::invalid_argument(vdx::fmt(
+                "vdx_gpu_hnsw_upload: M = %llu is outside the GPU path's range [2, %u]",
+                static_cast<unsigned long long>(g->M), kMaxGpuM));
+        }
+        require(g->n < 0x80000000ull, "vdx_gpu_hnsw_upload: more than 2^31 - 1 nodes");
+        require(g->n == 0 || (g->levels && g->layer0 && g->vectors), "vdx_gpu_hnsw_upload: null section");
+        require(g->upper_blocks == 0 || g->upper != nullptr, "vdx_gpu_hnsw_upload: null upper section");
+        require(g->n == 0 || g->entry_point < g->n, "vdx_gpu_hnsw_upload: entry point out of range");
+        // Offsets of each node's upper blocks; the levels must add up to U.
+        std::vector<std::uint32_t> offsets(std::max<std::uint64_t>(1, g->n));
+        std::uint64_t acc = 0;
+        for (std::uint64_t i = 0; i < g->n; ++i) {
+            offsets[i] = static_cast<std::uint32_t>(acc);
+            require(g->levels[i] <= g->max_level, "vdx_gpu_hnsw_upload: a level is above max_level");
+            acc += g->levels[i];
+        }
+        require(acc == g->upper_blocks, "vdx_gpu_hnsw_upload: levels do not add up to upper_blocks");
+        require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_upload: too many upper blocks");
+
+        DeviceGuard guard(ctx->device);
+        auto h = std::make_unique<vdx_gpu_hnsw>();
+        h->ctx = ctx;
+        h->metric = g->metric;
+        h->dim = g->dim;
+        h->n = g->n;
+        h->M = g->M;
+        h->max_level = g->n ? g->max_level : 0;
+        h->entry_point = g->n ? g->entry_point : 0;
+        h->split = vdx::gpu::split_dim(g->dim);
+        hipStream_t s = ctx->stream;
+        upload_rows(h->vectors, g->vectors, g->n, g->dim, h->split.ld, s);
+        const std::uint64_t l0 = g->n * (1 + 2 * g->M);
+        h->layer0.resize(std::max<std::uint64_t>(1, l0) * 4);
+        if (l0) {
+            check(hipMemcpyAsync(h->layer0.as<void>(), g->layer0, l0 * 4, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(layer0)");
+        }
+        const std::uint64_t up = g->upper_blocks * (1 + g->M);
+        h->upper.resize(std::max<std::uint64_t>(1, up) * 4);
+        if (up) {
+            check(hipMemcpyAsync(h->upper.as<void>(), g->upper, up * 4, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(upper)");
+        }
+        h->upper_offset.resize(offsets.size() * 4);
+        check(hipMemcpyAsync(h->upper_offset.as<void>(), offsets.data(), offsets.size() * 4,
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(offsets)");
+        h->ring.resize(kRingSlots * 256);
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(upload)");
+        *out = h.release();
+    });
+}
+
+void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index) {
+    if (index != nullptr) {
+        DeviceGuard guard(index->ctx->device);
+        delete index;
+    }
+}
+
+int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint64_t query_dim,
+                        const uint32_t* ks, uint32_t k, const uint32_t* efs, uint32_t ef,
+                        vdx_neighbor* out, uint64_t out_stride, uint32_t* out_count,
+                        const char* where, const vdx_search_opts* opts) {
+    return guarded([&] {
+        require(h != nullptr, "vdx_gpu_hnsw_search: index is null");
+        const char* w = where ? where : "HnswIndex::search_batch";
+        if (nq == 0) {
+            return;
+        }
+        require(queries != nullptr && out != nullptr && out_count != nullptr,
+                "vdx_gpu_hnsw_search: null buffer");
+        std::uint32_t kmax = 0, efmax = 0;
+        for (std::uint64_t q = 0; q < nq; ++q) {
+            const std::uint32_t kq = ks ? ks[q] : k;
+            const std::uint32_t eq = efs ? efs[q] : ef;
+            kmax = std::max(kmax, kq);
+            efmax = std::max(efmax, std::max(eq, kq));
+        }
+        require(out_stride >= kmax, "vdx_gpu_hnsw_search: out_stride is below k");
+        DeviceGuard guard(h->ctx->device);
+        WorkspaceLease ws(*h->ctx);
+        ws->h_queries.reserve(nq * h->dim * sizeof(float));
+        // Validation needs the dimension check first (query_dim may differ from dim).
+        if (query_dim != h->dim) {
+            vdx::check_query(w, {static_cast<vdx::Metric>(h->metric), h->dim, h->n}, queries,
+                             query_dim, ks ? ks[0] : k);
+        }
+        prepare_queries(w, static_cast<vdx::Metric>(h->metric), h->dim, h->n, queries, nq, query_dim,
+                        ks, k, ws->h_queries.as<float>());
+        if (h->n == 0) {
+            std::fill(out_count, out_count + nq, 0u);
+            return;
+        }
+        if (efmax > kMaxBeamEf) {
+            throw std::invalid_argument(vdx::fmt("%s: ef %u is above the GPU path's limit of %u", w,
+                                                 efmax, kMaxBeamEf));
+        }
+        hipStream_t s = ws->stream;
+        ws->queries.reserve(nq * h->dim * sizeof(float));
+        ws->out.reserve(nq * kmax * sizeof(vdx_neighbor));
+        ws->count.reserve(nq * sizeof(std::uint32_t));
+        ws->counters.reserve(64);
+        check(hipMemcpyAsync(ws->queries.as<void>(), ws->h_queries.as<void>(), nq * h->dim * sizeof(float),
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(queries)");
+        const std::uint32_t* d_ks = nullptr;
+        const std::uint32_t* d_efs = nullptr;
+        if (ks) {
+            ws->ks.reserve(nq * 4);
+            check(hipMemcpyAsync(ws->ks.as<void>(), ks, nq * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(ks)");
+            d_ks = ws->ks.as<std::uint32_t>();
+        }
+        if (efs) {
+            ws->efs.reserve(nq * 4);
+            check(hipMemcpyAsync(ws->efs.as<void>(), efs, nq * 4, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(efs)");
+            d_efs = ws->efs.as<std::uint32_t>();
+        }
+        unsigned long long* d_stats = nullptr;
+        if (opts && opts->stats) {
+            d_stats = reinterpret_cast<unsigned long long*>(ws->counters.as<char>() + 16);
+            check(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+        }
+        enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
+                     ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
+                     ws->counters.as<std::uint32_t>(), d_stats, opts, s);
+        ws->h_out.reserve(nq * kmax * sizeof(vdx_neighbor));
+        check(hipMemcpyAsync(ws->h_out.as<void>(), ws->out.as<void>(), nq * kmax * sizeof(vdx_neighbor),
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(out)");
+        check(hipMemcpyAsync(out_count, ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(count)");
+        unsigned long long host_stats[4] = {0, 0, 0, 0};
+        if (d_stats) {
+            check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(stats)");
+        }
+        check(hipStreamSynchronize(s), "hnsw search");
+        const vdx_neighbor* staged = ws->h_out.as<vdx_neighbor>();
+        for (std::uint64_t q = 0; q < nq; ++q) {
+            std::memcpy(out + q * out_stride, staged + q * kmax, out_count[q] * sizeof(vdx_neighbor));
+        }
+        if (d_stats) {
+            for (int i = 0; i < 4; ++i) {
+                opts->stats[i] += host_stats[i];
+            }
+        }
+    });
+}
+
+int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t nq,
+                               const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs,
+                               uint32_t ef, uint32_t max_k, vdx_neighbor* d_out,
+                               uint64_t out_stride, uint32_t* d_out_count,
+                               const vdx_search_opts* opts, void* stream) {
+    return guarded([&] {
+        require(h != nullptr, "vdx_gpu_hnsw_search_device: index is null");
+        require(max_k >= 1 && (d_ks != nullptr || k <= max_k), "vdx_gpu_hnsw_search_device: k above max_k");
+        require(out_stride >= max_k, "vdx_gpu_hnsw_search_device: out_stride is below max_k");
+        if (nq == 0) {
+            return;
+        }
+        // max(ef, k) over the batch sizes R; with per-query arrays the caller's bound is max_k
+        // and the ef array must not exceed `ef`.
+        const std::uint32_t efmax = std::max(ef, max_k);
+        require(efmax <= kMaxBeamEf, "vdx_gpu_hnsw_search_device: ef above the GPU path's limit");
+        DeviceGuard guard(h->ctx->device);
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->ctx->stream;
+        const std::uint32_t slot = h->next_slot.fetch_add(1) % kRingSlots;
+        char* base = h->ring.as<char>() + static_cast<std::size_t>(slot) * 256;
+        std::uint32_t* counter = reinterpret_cast<std::uint32_t*>(base);
+        unsigned long long* d_stats = reinterpret_cast<unsigned long long*>(base + 64);
+        if (h->n == 0) {
+            check(hipMemsetAsync(d_out_count, 0, nq * 4, s), "hipMemsetAsync(count)");
+            return;
+        }
+        const bool want_stats = opts && opts->stats;
+        if (want_stats) {
+            check(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+        }
+        enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
+                     reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
+                     want_stats ? d_stats : nullptr, opts, s);
+        if (want_stats) {
+            unsigned long long host_stats[4];
+            check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(stats)");
+            check(hipStreamSynchronize(s), "hnsw search stats");
+            for (int i = 0; i < 4; ++i) {
+                opts->stats[i] += host_stats[i];
+            }
+        }
+    });
+}
+
+// ------------------------------------------------------------------------------- brute force
+
+int vdx_gpu_bf_upload(vdx_gpu_ctx* ctx, uint32_t metric, uint64_t dim, uint64_t n,
+                      const float* vectors, vdx_gpu_bf** out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "vdx_gpu_bf_upload: null argument");
+        require(vdx::valid_metric(metric), "vdx_gpu_bf_upload: unknown metric code");
+        require(dim >= 1, "vdx_gpu_bf_upload: dimension is 0");
+        if (dim > kMaxGpuDim) {
+            throw std::invalid_argument(vdx::fmt("vdx_gpu_bf_upload: dimension %llu is above the GPU path's limit of %u",
+                                                 static_cast<unsigned long long>(dim), kMaxGpuDim));
+        }
+        require(n < 0x80000000ull, "vdx_gpu_bf_upload: more than 2^31 - 1 vectors");
+        require(n == 0 || vectors != nullptr, "vdx_gpu_bf_upload: vectors is null");
+        DeviceGuard guard(ctx->device);
+        auto b = std::make_unique<vdx_gpu_bf>();
+        b->ctx = ctx;
+        b->metric = metric;
+        b->dim = dim;
+        b->n = n;
+        b->split = vdx::gpu::split_dim(dim);
+        upload_rows(b->vectors, vectors, n, dim, b->split.ld, ctx->stream);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(upload)");
+        *out = b.release();
+    });
+}
+
+void vdx_gpu_bf_free(vdx_gpu_bf* index) {
+    if (index != nullptr) {
+        DeviceGuard guard(index->ctx->device);
+        delete index;
+    }
+}
+
+}  // extern "C"
+
+namespace vdx::gpu {
+
+// Exact scan of every row for nq prepared device queries, results to d_out (rows of out_stride).
+void bf_exact(vdx_gpu_bf& b, const float* d_queries, std::uint64_t nq, std::uint32_t k,
+              vdx::dev::Neighbor* d_out, std::uint64_t out_stride, std::uint32_t* d_count,
+              Workspace& ws, hipStream_t s) {
+    const std::uint32_t kcap = std::min<std::uint64_t>(k, std::max<std::uint64_t>(1, b.n));
+    std::uint32_t waves = kcap <= 128 ? 4 : 1;
+    std::size_t lds = sizeof(float) * 32 * b.split.ld + static_cast<std::size_t>(waves) * 8 * kcap * 8;
+    require(lds <= 160 * 1024, "BruteForceIndex::search: k too large for the GPU path");
+    const std::uint64_t qgroups = (nq + waves * 8 - 1) / (waves * 8);
+    std::uint64_t slabs = std::max<std::uint64_t>(1, (2 * static_cast<std::uint64_t>(b.ctx->cu_count) * 2 + qgroups - 1) / qgroups);
+    slabs = std::min<std::uint64_t>(slabs, 64);
+    slabs = std::min<std::uint64_t>(slabs, std::max<std::uint64_t>(1, b.n / 256));
+    ws.aux0.reserve(slabs * nq * kcap * 8);
+    ws.aux1.reserve(slabs * nq * 4);
+    vdx::dev::BfScanArgs a{};
+    a.vectors = b.vectors.as<float>();
+    a.ld = b.split.ld;
+    a.dim = static_cast<std::uint32_t>(b.dim);
+    a.n = static_cast<std::uint32_t>(b.n);
+    a.l2 = b.metric == 0 ? 1u : 0u;
+    a.nb = b.split.nb;
+    a.n8 = b.split.n8;
+    a.tail = b.split.tail;
+    a.first_tail = b.split.first_tail;
+    a.queries = d_queries;
+    a.qld = static_cast<std::uint32_t>(b.dim);
+    a.nq = static_cast<std::uint32_t>(nq);
+    a.k = k;
+    a.kcap = kcap;
+    a.partial = ws.aux0.as<std::uint64_t>();
+    a.partial_count = ws.aux1.as<std::uint32_t>();
+    a.out = d_out;
+    a.out_stride = out_stride;
+    a.out_count = d_count;
+    check(vdx::dev::launch_bf_exact_scan(a, static_cast<unsigned>(qgroups), static_cast<unsigned>(slabs),
+                                         waves, lds, s, bf_shape(b.dim)),
+          "bf_exact_scan launch");
+}
+
+}  // namespace vdx::gpu
+
+extern "C" {
+
+int vdx_gpu_bf_search_device(vdx_gpu_bf* b, const float* d_queries, uint64_t nq, uint32_t k,
+                             uint32_t precision, vdx_neighbor* d_out, uint64_t out_stride,
+                             uint32_t* d_out_count, vdx_bf_stats* stats) {
+    return guarded([&] {
+        require(b != nullptr, "vdx_gpu_bf_search_device: index is null");
+        require(k >= 1, "BruteForceIndex::search: k must be positive");
+        require(out_stride >= std::min<std::uint64_t>(k, b->n), "vdx_gpu_bf_search_device: out_stride is below k");
+        if (k > kMaxBfK && b->n > kMaxBfK) {
+            throw std::invalid_argument(vdx::fmt("BruteForceIndex::search: k %u is above the GPU path's limit of %u", k, kMaxBfK));
+        }
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+        }
+        if (nq == 0) {
+            return;
+        }
+        DeviceGuard guard(b->ctx->device);
+        WorkspaceLease ws(*b->ctx);
+        hipStream_t s = ws->stream;
+        if (b->n == 0) {
+            check(hipMemsetAsync(d_out_count, 0, nq * 4, s), "hipMemsetAsync(count)");
+            check(hipStreamSynchronize(s), "bf search");
+            return;
+        }
+        switch (precision) {
+            case VDX_PREC_EXACT:
+                vdx::gpu::bf_exact(*b, d_queries, nq, k, reinterpret_cast<vdx::dev::Neighbor*>(d_out),
+                                   out_stride, d_out_count, *ws, s);
+                break;
+            default:
+                throw std::invalid_argument("BruteForceIndex::search: unknown precision mode");
+        }
+        check(hipStreamSynchronize(s), "bf search");
+    });
+}
+
+int vdx_gpu_bf_search(vdx_gpu_bf* b, const float* queries, uint64_t nq, uint64_t query_dim,
+                      uint32_t k, uint32_t precision, vdx_neighbor* out, uint64_t out_stride,
+                      uint32_t* out_count, const char* where, vdx_bf_stats* stats) {
+    return guarded([&] {
+        require(b != nullptr, "vdx_gpu_bf_search: index is null");
+        const char* w = where ? where : "BruteForceIndex::search";
+        if (nq == 0) {
+            return;
+        }
+        require(queries != nullptr && out != nullptr && out_count != nullptr, "vdx_gpu_bf_search: null buffer");
+        DeviceGuard guard(b->ctx->device);
+        std::vector<float> prepared(nq * b->dim);
+        if (query_dim != b->dim) {
+            vdx::check_query(w, {static_cast<vdx::Metric>(b->metric), b->dim, b->n}, queries, query_dim, k);
+        }
+        prepare_queries(w, static_cast<vdx::Metric>(b->metric), b->dim, b->n, queries, nq, query_dim,
+                        nullptr, k, prepared.data());
+        const std::uint64_t keep = std::min<std::uint64_t>(k, b->n);
+        require(out_stride >= keep, "vdx_gpu_bf_search: out_stride is below k");
+        if (b->n == 0) {
+            std::fill(out_count, out_count + nq, 0u);
+            return;
+        }
+        DeviceBuffer d_q(nq * b->dim * sizeof(float));
+        DeviceBuffer d_out(nq * keep * sizeof(vdx_neighbor));
+        DeviceBuffer d_count(nq * 4);
+        check(hipMemcpy(d_q.as<void>(), prepared.data(), nq * b->dim * sizeof(float), hipMemcpyHostToDevice),
+              "hipMemcpy(queries)");
+        const int rc = vdx_gpu_bf_search_device(b, d_q.as<float>(), nq, k, precision, d_out.as<vdx_neighbor>(),
+                                                keep, d_count.as<std::uint32_t>(), stats);
+        if (rc != VDX_OK) {
+            const std::string msg = vdx_last_error();
+            if (rc == VDX_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+            if (rc == VDX_OUT_OF_MEMORY) throw std::bad_alloc();
+            if (rc == VDX_DEVICE_ERROR) throw vdx::device_error(msg);
+            throw std::runtime_error(msg);
+        }
+        std::vector<vdx_neighbor> staged(nq * keep);
+        check(hipMemcpy(staged.data(), d_out.as<void>(), staged.size() * sizeof(vdx_neighbor), hipMemcpyDeviceToHost),
+              "hipMemcpy(out)");
+        check(hipMemcpy(out_count, d_count.as<void>(), nq * 4, hipMemcpyDeviceToHost), "hipMemcpy(count)");
+        for (std::uint64_t q = 0; q < nq; ++q) {
+            std::memcpy(out + q * out_stride, staged.data() + q * keep, out_count[q] * sizeof(vdx_neighbor));
+        }
+    });
+}
+
+}  // extern "C"

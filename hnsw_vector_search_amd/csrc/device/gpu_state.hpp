@@ -1,0 +1,131 @@
+#pragma once
+
+// Host-side state behind the GPU half of the C ABI: the device context, device buffers, the
+// uploaded HNSW graph and brute-force base, and per-call workspaces.
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../host/common.hpp"
+
+namespace vdx::gpu {
+
+[[noreturn]] void throw_hip(hipError_t err, const char* what);
+
+inline void check(hipError_t err, const char* what) {
+    if (err != hipSuccess) {
+        throw_hip(err, what);
+    }
+}
+
+// An owned device allocation.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(std::size_t bytes) { resize(bytes); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
+    // Grows to at least `bytes` (contents not kept).
+    void reserve(std::size_t bytes) {
+        if (bytes > bytes_) {
+            resize(bytes);
+        }
+    }
+    void resize(std::size_t bytes);
+    void release() noexcept;
+    template <typename T>
+    T* as() const noexcept {
+        return static_cast<T*>(ptr_);
+    }
+    std::size_t bytes() const noexcept { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    std::size_t bytes_ = 0;
+};
+
+// An owned pinned host allocation (hipHostMalloc).
+class PinnedBuffer {
+public:
+    PinnedBuffer() = default;
+    ~PinnedBuffer() { release(); }
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    void reserve(std::size_t bytes);
+    void release() noexcept;
+    template <typename T>
+    T* as() const noexcept {
+        return static_cast<T*>(ptr_);
+    }
+
+private:
+    void* ptr_ = nullptr;
+    std::size_t bytes_ = 0;
+};
+
+// Device buffers and a stream for one in-flight search call; pooled per context so that
+// concurrent calls never share one (the GPU analogue of vecdex's ScratchPool,
+// src/search_scratch.hpp:61-105).
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DeviceBuffer queries, ks, efs, out, count, counters, aux0, aux1, aux2, aux3;
+    PinnedBuffer h_queries, h_out, h_count;
+    ~Workspace();
+};
+
+}  // namespace vdx::gpu
+
+struct vdx_gpu_ctx {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    std::mutex pool_mutex;
+    std::vector<std::unique_ptr<vdx::gpu::Workspace>> free_workspaces;
+    ~vdx_gpu_ctx();
+};
+
+namespace vdx::gpu {
+
+// RAII loan of a workspace from the context's pool.
+class WorkspaceLease {
+public:
+    explicit WorkspaceLease(vdx_gpu_ctx& ctx);
+    ~WorkspaceLease();
+    WorkspaceLease(const WorkspaceLease&) = delete;
+    WorkspaceLease& operator=(const WorkspaceLease&) = delete;
+    Workspace& operator*() const noexcept { return *ws_; }
+    Workspace* operator->() const noexcept { return ws_.get(); }
+
+private:
+    vdx_gpu_ctx& ctx_;
+    std::unique_ptr<Workspace> ws_;
+};
+
+// Makes `device` current for the calling thread while alive.
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+
+private:
+    int previous_ = 0;
+};
+
+// The split of a dimension in the distance summation order.
+struct DimSplit {
+    std::uint32_t nb, n8, tail, first_tail, ld;
+};
+DimSplit split_dim(std::uint64_t dim);
+
+}  // namespace vdx::gpu

@@ -1,0 +1,152 @@
+// The host half of the C ABI: the restated graph build, FORMAT v1 and validation.
+
+#include <cstring>
+#include <memory>
+
+#include "capi_common.hpp"
+#include "cpu_distance.hpp"
+#include "graph.hpp"
+#include "validate.hpp"
+
+namespace vdx {
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& message) { g_last_error = message; }
+
+}  // namespace vdx
+
+struct vdx_host_graph {
+    std::unique_ptr<vdx::HnswGraph> graph;
+};
+
+using vdx::guarded;
+using vdx::require;
+
+extern "C" {
+
+const char* vdx_last_error(void) { return vdx::g_last_error.c_str(); }
+
+const char* vdx_version(void) { return "0.1.0"; }
+
+int vdx_host_graph_create(uint64_t dim, uint32_t metric, uint64_t M, uint64_t ef_construction,
+                          uint64_t seed, vdx_host_graph** out) {
+    return guarded([&] {
+        require(out != nullptr, "vdx_host_graph_create: out is null");
+        require(vdx::valid_metric(metric), "vdx_host_graph_create: unknown metric code");
+        auto g = std::make_unique<vdx_host_graph>();
+        g->graph = std::make_unique<vdx::HnswGraph>(
+            dim, static_cast<vdx::Metric>(metric), vdx::GraphParams{M, ef_construction, seed});
+        *out = g.release();
+    });
+}
+
+int vdx_host_graph_add(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t given_dim,
+                       uint64_t num_threads) {
+    return guarded([&] {
+        require(g != nullptr, "vdx_host_graph_add: graph is null");
+        require(rows == 0 || data != nullptr, "vdx_host_graph_add: data is null");
+        if (given_dim != g->graph->dim()) {
+            throw std::invalid_argument(vdx::fmt("HnswIndex::add: got dimension %llu, index has %zu",
+                                                 static_cast<unsigned long long>(given_dim),
+                                                 g->graph->dim()));
+        }
+        g->graph->add(data, rows, num_threads);
+    });
+}
+
+int vdx_host_graph_save(const vdx_host_graph* g, const char* path) {
+    return guarded([&] {
+        require(g != nullptr && path != nullptr, "vdx_host_graph_save: null argument");
+        g->graph->save(path);
+    });
+}
+
+int vdx_host_graph_load(const char* path, vdx_host_graph** out) {
+    return guarded([&] {
+        require(out != nullptr && path != nullptr, "vdx_host_graph_load: null argument");
+        auto g = std::make_unique<vdx_host_graph>();
+        g->graph = vdx::HnswGraph::load(path);
+        *out = g.release();
+    });
+}
+
+int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out) {
+    return guarded([&] {
+        require(g != nullptr && out != nullptr, "vdx_host_graph_view: null argument");
+        const vdx::HnswGraph& h = *g->graph;
+        std::memset(out, 0, sizeof(*out));
+        out->metric = static_cast<uint32_t>(h.metric());
+        out->dim = h.dim();
+        out->n = h.size();
+        out->M = h.params().M;
+        out->max_level = h.size() == 0 ? 0 : h.max_level();
+        out->entry_point = h.size() == 0 ? 0 : h.entry_point();
+        out->upper_blocks = h.upper_blocks();
+        out->levels = h.levels().data();
+        out->layer0 = h.layer0().data();
+        out->upper = h.upper().data();
+        out->vectors = h.vectors().data();
+    });
+}
+
+int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
+                          uint64_t* version) {
+    return guarded([&] {
+        require(g != nullptr, "vdx_host_graph_params: graph is null");
+        if (ef_construction) *ef_construction = g->graph->params().ef_construction;
+        if (seed) *seed = g->graph->params().seed;
+        if (version) *version = g->graph->version();
+    });
+}
+
+uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g) {
+    return g == nullptr ? 0 : g->graph->memory_bytes();
+}
+
+int vdx_host_graph_inspect(const vdx_host_graph* g, vdx_graph_report* out, char* first_violation,
+                           uint64_t buflen) {
+    return guarded([&] {
+        require(g != nullptr && out != nullptr, "vdx_host_graph_inspect: null argument");
+        const vdx::GraphReport r = g->graph->inspect();
+        out->reachable = r.reachable;
+        out->max_level = r.max_level;
+        out->violation_count = r.violations.size();
+        out->mean_degree0 = r.mean_degree0;
+        if (first_violation != nullptr && buflen > 0) {
+            const std::string& msg = r.violations.empty() ? std::string() : r.violations.front();
+            const std::size_t n = std::min<std::size_t>(msg.size(), buflen - 1);
+            std::memcpy(first_violation, msg.data(), n);
+            first_violation[n] = '\0';
+        }
+    });
+}
+
+void vdx_host_graph_free(vdx_host_graph* g) { delete g; }
+
+int vdx_host_check_new_vectors(const char* where, uint32_t metric, uint64_t dim,
+                               uint64_t index_size, const float* data, uint64_t rows,
+                               uint64_t given_dim) {
+    return guarded([&] {
+        require(vdx::valid_metric(metric), "vdx_host_check_new_vectors: unknown metric code");
+        require(rows == 0 || data != nullptr, "vdx_host_check_new_vectors: data is null");
+        vdx::check_new_vectors(where ? where : "add", {static_cast<vdx::Metric>(metric), dim, index_size},
+                               given_dim, data, rows);
+    });
+}
+
+int vdx_host_normalize_rows(float* data, uint64_t rows, uint64_t dim) {
+    return guarded([&] {
+        require(rows == 0 || data != nullptr, "vdx_host_normalize_rows: data is null");
+        for (uint64_t r = 0; r < rows; ++r) {
+            vdx::normalize(data + r * dim, dim);
+        }
+    });
+}
+
+float vdx_host_distance(uint32_t metric, const float* a, const float* b, uint64_t dim) {
+    return vdx::cpu::metric_distance(metric, a, b, dim);
+}
+
+}  // extern "C"

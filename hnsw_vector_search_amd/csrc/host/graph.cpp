@@ -1,0 +1,467 @@
+// The CPU graph build, restated from vecdex's HnswIndex (src/hnsw.cpp:131-409, 416-482 for the
+// build's own layer search) so that the same vectors, parameters and seed give the same graph.
+// The single-threaded build is pinned by vecdex's golden fingerprints
+// (tests/hnsw_parallel_test.cpp:146-169), checked in tests/test_graph_build.py.
+
+#include "graph.hpp"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+
+#include "cpu_distance.hpp"
+#include "validate.hpp"
+
+namespace vdx {
+
+// Epoch-stamped visited marks: O(1) reset per search (src/search_scratch.hpp:21-47).
+class VisitedMarks {
+public:
+    void reset(std::size_t n) {
+        if (marks_.size() < n) {
+            marks_.resize(n, 0);
+        }
+        if (++epoch_ == 0) {
+            std::fill(marks_.begin(), marks_.end(), 0U);
+            epoch_ = 1;
+        }
+    }
+    bool insert(Id id) noexcept {
+        if (marks_[id] == epoch_) {
+            return false;
+        }
+        marks_[id] = epoch_;
+        return true;
+    }
+
+private:
+    std::vector<std::uint32_t> marks_;
+    std::uint32_t epoch_ = 0;
+};
+
+struct HnswGraph::Scratch {
+    VisitedMarks visited;
+    std::vector<Neighbor> candidates;  // min-heap under `farther`
+    std::vector<Neighbor> results;     // max-heap under `closer`
+    std::vector<Id> fresh;
+    std::vector<Id> link_copy;
+};
+
+// Hands each concurrent insert its own scratch and keeps it for the next (src/search_scratch.hpp:61-105).
+class HnswGraph::ScratchPool {
+public:
+    std::unique_ptr<Scratch> take() {
+        const std::lock_guard<std::mutex> lock(mutex_);
+        if (free_.empty()) {
+            return std::make_unique<Scratch>();
+        }
+        std::unique_ptr<Scratch> s = std::move(free_.back());
+        free_.pop_back();
+        return s;
+    }
+    void give(std::unique_ptr<Scratch> s) {
+        const std::lock_guard<std::mutex> lock(mutex_);
+        free_.push_back(std::move(s));
+    }
+
+private:
+    std::mutex mutex_;
+    std::vector<std::unique_ptr<Scratch>> free_;
+};
+
+struct HnswGraph::BuildLocks {
+    explicit BuildLocks(std::size_t nodes) : links(new std::mutex[nodes]) {}
+    std::unique_ptr<std::mutex[]> links;
+    std::mutex entry;
+};
+
+namespace {
+
+constexpr bool farther(const Neighbor& a, const Neighbor& b) noexcept { return closer(b, a); }
+
+GraphParams validated(const GraphParams& p) {
+    if (p.M < 2) {
+        throw std::invalid_argument("HnswIndex: M must be at least 2");
+    }
+    if (p.M > GraphParams::kMaxM) {
+        throw std::invalid_argument(fmt("HnswIndex: M must be at most %zu", GraphParams::kMaxM));
+    }
+    if (p.ef_construction == 0) {
+        throw std::invalid_argument("HnswIndex: ef_construction must be positive");
+    }
+    return p;
+}
+
+// RAII loan of a scratch from the pool.
+class Lease {
+public:
+    explicit Lease(HnswGraph::ScratchPool& pool) : pool_(pool), scratch_(pool.take()) {}
+    ~Lease() { pool_.give(std::move(scratch_)); }
+    Lease(const Lease&) = delete;
+    Lease& operator=(const Lease&) = delete;
+    HnswGraph::Scratch& operator*() const noexcept { return *scratch_; }
+
+private:
+    HnswGraph::ScratchPool& pool_;
+    std::unique_ptr<HnswGraph::Scratch> scratch_;
+};
+
+}  // namespace
+
+HnswGraph::HnswGraph(std::size_t dim, Metric metric, GraphParams params)
+    : dim_(dim),
+      metric_(metric),
+      params_(validated(params)),
+      level_multiplier_(1.0 / std::log(static_cast<double>(params_.M))),
+      rng_(params_.seed),
+      scratch_(std::make_unique<ScratchPool>()) {
+    if (dim == 0) {
+        throw std::invalid_argument("Matrix: dim must be positive");
+    }
+}
+
+HnswGraph::~HnswGraph() = default;
+
+const Id* HnswGraph::block(Id node, std::size_t layer) const noexcept {
+    if (layer == 0) {
+        return layer0_.data() + static_cast<std::size_t>(node) * (1 + 2 * params_.M);
+    }
+    return upper_.data() +
+           (static_cast<std::size_t>(upper_offset_[node]) + (layer - 1)) * (1 + params_.M);
+}
+
+Id* HnswGraph::block(Id node, std::size_t layer) noexcept {
+    return const_cast<Id*>(static_cast<const HnswGraph&>(*this).block(node, layer));
+}
+
+float HnswGraph::distance(const float* a, const float* b) const noexcept {
+    return cpu::metric_distance(static_cast<unsigned>(metric_), a, b, dim_);
+}
+
+std::size_t HnswGraph::memory_bytes() const noexcept {
+    return vectors_.size() * sizeof(float) + levels_.size() + layer0_.size() * sizeof(Id) +
+           upper_.size() * sizeof(Id) + upper_offset_.size() * sizeof(std::uint32_t);
+}
+
+// level = floor(-ln(U) * mL), U in (0, 1] from the top 53 bits of mt19937_64
+// (src/hnsw.cpp:219-228): platform-independent levels for a given seed.
+std::size_t HnswGraph::random_level() {
+    const double u = 1.0 - static_cast<double>(rng_() >> 11U) * 0x1.0p-53;
+    return static_cast<std::size_t>(-std::log(u) * level_multiplier_);
+}
+
+void HnswGraph::store(const float* vector) {
+    vectors_.insert(vectors_.end(), vector, vector + dim_);
+    if (metric_ == Metric::Cosine) {
+        normalize(vectors_.data() + (size() * dim_), dim_);
+    }
+    const std::size_t level = random_level();
+    const std::size_t first_block = upper_.size() / (1 + params_.M);
+    if (first_block > UINT32_MAX - level) {
+        throw std::length_error("HnswIndex::add: too many upper-layer blocks for 32-bit offsets");
+    }
+    levels_.push_back(static_cast<std::uint8_t>(level));
+    upper_offset_.push_back(static_cast<std::uint32_t>(first_block));
+    layer0_.resize(layer0_.size() + 1 + 2 * params_.M, Id{0});
+    upper_.resize(upper_.size() + level * (1 + params_.M), Id{0});
+}
+
+void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads) {
+    check_new_vectors("HnswIndex::add", IndexShape{metric_, dim_, size()}, dim_, data, rows);
+    ++version_;
+    const std::size_t first = size();
+    const std::size_t total = first + rows;
+    vectors_.reserve(total * dim_);
+    levels_.reserve(total);
+    upper_offset_.reserve(total);
+    layer0_.reserve(total * (1 + 2 * params_.M));
+    const std::size_t threads = std::min(resolve_threads(num_threads), rows);
+    if (threads <= 1) {
+        for (std::size_t r = 0; r < rows; ++r) {
+            store(data + r * dim_);
+            insert<Locking::None>(static_cast<Id>(size() - 1));
+        }
+        return;
+    }
+    // Every row is stored before any is linked, so no container grows under a thread's feet;
+    // the levels are drawn in serial order.
+    build_locks_ = std::make_unique<BuildLocks>(total);
+    for (std::size_t r = 0; r < rows; ++r) {
+        store(data + r * dim_);
+    }
+    std::size_t next = first;
+    if (next == 0) {
+        insert<Locking::None>(0);
+        next = 1;
+    }
+    parallel_for(total - next, threads,
+                 [this, next](std::size_t i) { insert<Locking::PerNode>(static_cast<Id>(next + i)); });
+    build_locks_.reset();
+}
+
+template <HnswGraph::Locking L>
+const Id* HnswGraph::read_links(Id node, std::size_t layer, std::vector<Id>& copy,
+                                std::size_t& count) const {
+    if constexpr (L == Locking::PerNode) {
+        const std::lock_guard<std::mutex> lock(build_locks_->links[node]);
+        const Id* b = block(node, layer);
+        copy.assign(b + 1, b + 1 + b[0]);
+        count = copy.size();
+        return copy.data();
+    } else {
+        const Id* b = block(node, layer);
+        count = b[0];
+        return b + 1;
+    }
+}
+
+template <HnswGraph::Locking L>
+Neighbor HnswGraph::greedy_closest(const float* query, Neighbor entry, std::size_t layer,
+                                   Scratch& scratch) const {
+    for (bool moved = true; moved;) {
+        moved = false;
+        std::size_t count = 0;
+        const Id* ids = read_links<L>(entry.id, layer, scratch.link_copy, count);
+        for (std::size_t i = 0; i < count; ++i) {
+            const float d = distance(query, vector_of(ids[i]));
+            if (d < entry.distance) {
+                entry = {d, ids[i]};
+                moved = true;
+            }
+        }
+    }
+    return entry;
+}
+
+template <HnswGraph::Locking L>
+void HnswGraph::search_layer(const float* query, Neighbor entry, std::size_t ef, std::size_t layer,
+                             Scratch& scratch) const {
+    VisitedMarks& visited = scratch.visited;
+    std::vector<Neighbor>& candidates = scratch.candidates;
+    std::vector<Neighbor>& results = scratch.results;
+    std::vector<Id>& fresh = scratch.fresh;
+    visited.reset(size());
+    visited.insert(entry.id);
+    candidates.assign(1, entry);
+    results.assign(1, entry);
+    while (!candidates.empty()) {
+        std::pop_heap(candidates.begin(), candidates.end(), farther);
+        const Neighbor current = candidates.back();
+        candidates.pop_back();
+        if (results.size() >= ef && closer(results.front(), current)) {
+            break;
+        }
+        fresh.clear();
+        std::size_t count = 0;
+        const Id* ids = read_links<L>(current.id, layer, scratch.link_copy, count);
+        for (std::size_t i = 0; i < count; ++i) {
+            if (visited.insert(ids[i])) {
+                fresh.push_back(ids[i]);
+                __builtin_prefetch(vector_of(ids[i]));
+            }
+        }
+        for (const Id id : fresh) {
+            const Neighbor next{distance(query, vector_of(id)), id};
+            if (results.size() < ef || closer(next, results.front())) {
+                candidates.push_back(next);
+                std::push_heap(candidates.begin(), candidates.end(), farther);
+                results.push_back(next);
+                std::push_heap(results.begin(), results.end(), closer);
+                if (results.size() > ef) {
+                    std::pop_heap(results.begin(), results.end(), closer);
+                    results.pop_back();
+                }
+            }
+        }
+    }
+}
+
+template <HnswGraph::Locking L>
+void HnswGraph::insert(Id node) {
+    const std::size_t level = levels_[node];
+    if (node == 0) {
+        entry_point_ = node;
+        max_level_ = level;
+        return;
+    }
+    // A node that raises the top layer holds the entry lock until it is linked.
+    std::unique_lock<std::mutex> entry_lock;
+    if constexpr (L == Locking::PerNode) {
+        entry_lock = std::unique_lock<std::mutex>(build_locks_->entry);
+    }
+    const Id entry_point = entry_point_;
+    const std::size_t max_level = max_level_;
+    if (entry_lock.owns_lock() && level <= max_level) {
+        entry_lock.unlock();
+    }
+    Lease lease(*scratch_);
+    Scratch& scratch = *lease;
+    const float* query = vector_of(node);
+
+    Neighbor entry{distance(query, vector_of(entry_point)), entry_point};
+    for (std::size_t layer = max_level; layer > level; --layer) {
+        entry = greedy_closest<L>(query, entry, layer, scratch);
+    }
+    const std::size_t ef = std::max(params_.ef_construction, params_.M);
+    const std::size_t top = std::min(level, max_level);
+    std::vector<std::vector<Neighbor>> chosen(top + 1);
+    for (std::size_t i = 0; i <= top; ++i) {
+        const std::size_t layer = top - i;
+        search_layer<L>(query, entry, ef, layer, scratch);
+        entry = *std::min_element(scratch.results.begin(), scratch.results.end(), closer);
+        chosen[layer] = select_neighbors(scratch.results, params_.M);
+    }
+    {
+        std::unique_lock<std::mutex> lock;
+        if constexpr (L == Locking::PerNode) {
+            lock = std::unique_lock<std::mutex>(build_locks_->links[node]);
+        }
+        for (std::size_t layer = 0; layer <= top; ++layer) {
+            Id* b = block(node, layer);
+            b[0] = static_cast<Id>(chosen[layer].size());
+            for (std::size_t i = 0; i < chosen[layer].size(); ++i) {
+                b[1 + i] = chosen[layer][i].id;
+            }
+        }
+    }
+    for (std::size_t layer = 0; layer <= top; ++layer) {
+        link_back<L>(node, layer, chosen[layer]);
+    }
+    if (level > max_level) {
+        max_level_ = level;
+        entry_point_ = node;
+    }
+}
+
+template <HnswGraph::Locking L>
+void HnswGraph::link_back(Id node, std::size_t layer, const std::vector<Neighbor>& chosen) {
+    for (const Neighbor& neighbor : chosen) {
+        std::unique_lock<std::mutex> lock;
+        if constexpr (L == Locking::PerNode) {
+            lock = std::unique_lock<std::mutex>(build_locks_->links[neighbor.id]);
+        }
+        Id* b = block(neighbor.id, layer);
+        const std::size_t count = b[0];
+        if (count < max_links(layer)) {
+            b[1 + count] = node;
+            b[0] = static_cast<Id>(count + 1);
+            continue;
+        }
+        std::vector<Neighbor> candidates;
+        candidates.reserve(count + 1);
+        const float* base = vector_of(neighbor.id);
+        for (std::size_t i = 0; i < count; ++i) {
+            candidates.push_back({distance(base, vector_of(b[1 + i])), b[1 + i]});
+        }
+        candidates.push_back({neighbor.distance, node});
+        const std::vector<Neighbor> kept = select_neighbors(std::move(candidates), max_links(layer));
+        b[0] = static_cast<Id>(kept.size());
+        for (std::size_t i = 0; i < kept.size(); ++i) {
+            b[1 + i] = kept[i].id;
+        }
+    }
+}
+
+// The neighbor-selection heuristic (Malkov & Yashunin, Algorithm 4): walking the candidates
+// from closest to farthest, keep one only if it is closer to the base than to every kept one.
+std::vector<Neighbor> HnswGraph::select_neighbors(std::vector<Neighbor> candidates,
+                                                  std::size_t m) const {
+    std::sort(candidates.begin(), candidates.end(), closer);
+    if (candidates.size() <= m) {
+        return candidates;
+    }
+    std::vector<Neighbor> kept;
+    kept.reserve(m);
+    for (const Neighbor& c : candidates) {
+        if (kept.size() == m) {
+            break;
+        }
+        const float* v = vector_of(c.id);
+        bool diverse = true;
+        for (const Neighbor& other : kept) {
+            if (distance(v, vector_of(other.id)) < c.distance) {
+                diverse = false;
+                break;
+            }
+        }
+        if (diverse) {
+            kept.push_back(c);
+        }
+    }
+    return kept;
+}
+
+GraphReport HnswGraph::inspect() const {
+    GraphReport report;
+    report.max_level = max_level_;
+    const std::size_t n = size();
+    if (n == 0) {
+        return report;
+    }
+    std::size_t highest = 0;
+    std::size_t layer0_links = 0;
+    std::vector<Id> sorted;
+    for (Id node = 0; node < n; ++node) {
+        const std::size_t level = levels_[node];
+        highest = std::max(highest, level);
+        for (std::size_t layer = 0; layer <= level; ++layer) {
+            const Id* b = block(node, layer);
+            const std::size_t count = b[0];
+            if (layer == 0) {
+                layer0_links += count;
+            }
+            if (count > max_links(layer)) {
+                report.violations.push_back(fmt("node %u has %zu links on layer %zu (max %zu)",
+                                                node, count, layer, max_links(layer)));
+                continue;
+            }
+            sorted.assign(b + 1, b + 1 + count);
+            std::sort(sorted.begin(), sorted.end());
+            if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+                report.violations.push_back(
+                    fmt("node %u has a duplicate link on layer %zu", node, layer));
+            }
+            for (std::size_t i = 0; i < count; ++i) {
+                const Id id = b[1 + i];
+                if (id == node) {
+                    report.violations.push_back(
+                        fmt("node %u links to itself on layer %zu", node, layer));
+                } else if (id >= n) {
+                    report.violations.push_back(fmt("node %u links to missing node %u", node, id));
+                } else if (levels_[id] < layer) {
+                    report.violations.push_back(fmt(
+                        "node %u links to node %u on layer %zu, above its level", node, id, layer));
+                }
+            }
+        }
+    }
+    if (highest != max_level_) {
+        report.violations.push_back(
+            fmt("max level is %zu but the tallest node reaches %zu", max_level_, highest));
+    }
+    if (levels_[entry_point_] != max_level_) {
+        report.violations.push_back(fmt("entry point %u is not on the top layer", entry_point_));
+    }
+    std::vector<bool> seen(n, false);
+    std::vector<Id> frontier{entry_point_};
+    seen[entry_point_] = true;
+    for (std::size_t i = 0; i < frontier.size(); ++i) {
+        const Id* b = block(frontier[i], 0);
+        const std::size_t count = std::min<std::size_t>(b[0], max_links(0));
+        for (std::size_t j = 0; j < count; ++j) {
+            const Id id = b[1 + j];
+            if (id < n && !seen[id]) {
+                seen[id] = true;
+                frontier.push_back(id);
+            }
+        }
+    }
+    report.reachable = frontier.size();
+    report.mean_degree0 = static_cast<double>(layer0_links) / static_cast<double>(n);
+    return report;
+}
+
+template void HnswGraph::insert<HnswGraph::Locking::None>(Id);
+template void HnswGraph::insert<HnswGraph::Locking::PerNode>(Id);
+
+}  // namespace vdx

@@ -1,0 +1,210 @@
+/* libvdxgpu: the C ABI of the MI355X-native vecdex query path.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross it and no exception escapes it.
+ * Every function returns a vdx_status (0 = success); on failure vdx_last_error() holds the
+ * message of the C++ exception vecdex would have thrown, and the status names its type:
+ *   VDX_INVALID_ARGUMENT  std::invalid_argument  (Python ValueError)
+ *   VDX_RUNTIME_ERROR     std::runtime_error     (Python RuntimeError)
+ *   VDX_LENGTH_ERROR      std::length_error
+ *   VDX_OUT_OF_MEMORY     std::bad_alloc, or hipErrorOutOfMemory on the device
+ *   VDX_DEVICE_ERROR      any other HIP failure
+ * A rejected call changes nothing (vecdex include/vecdex/hnsw.hpp:60-62).
+ *
+ * Each entry point names the vecdex interface it replaces or serves (file:line in vecdex).
+ */
+#ifndef VDXGPU_H
+#define VDXGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define VDX_API __attribute__((visibility("default")))
+#else
+#define VDX_API
+#endif
+
+enum vdx_status {
+    VDX_OK = 0,
+    VDX_INVALID_ARGUMENT = 1,
+    VDX_RUNTIME_ERROR = 2,
+    VDX_LENGTH_ERROR = 3,
+    VDX_OUT_OF_MEMORY = 4,
+    VDX_DEVICE_ERROR = 5
+};
+
+/* Layout-identical to vecdex::Neighbor {float distance; Id id} (include/vecdex/neighbor.hpp:13-18). */
+typedef struct vdx_neighbor {
+    float distance;
+    uint32_t id;
+} vdx_neighbor;
+
+/* A graph in FORMAT v1 section layout (docs/FORMAT.md:13-96): what HnswIndex holds in
+ * levels_, layer0_, upper_ (concatenated in id order) and vectors_
+ * (include/vecdex/hnsw.hpp:170-178). The caller keeps ownership; upload copies it to HBM. */
+typedef struct vdx_graph_view {
+    uint32_t metric; /* FORMAT code: 0 L2, 1 inner product, 2 cosine */
+    uint32_t reserved;
+    uint64_t dim;
+    uint64_t n;
+    uint64_t M;
+    uint64_t max_level;
+    uint64_t entry_point;
+    uint64_t upper_blocks;   /* U: the sum of all levels */
+    const uint8_t* levels;   /* n */
+    const uint32_t* layer0;  /* n * (1 + 2M) */
+    const uint32_t* upper;   /* U * (1 + M) */
+    const float* vectors;    /* n * dim; unit length under cosine */
+} vdx_graph_view;
+
+/* The message of the calling thread's last failure. */
+VDX_API const char* vdx_last_error(void);
+VDX_API const char* vdx_version(void);
+
+/* ================================================================================= host side
+ * A restatement of vecdex's graph build and file format, so an index can be built or loaded
+ * on the host without vecdex itself. Not accelerated (graph build stays on the CPU). */
+
+typedef struct vdx_host_graph vdx_host_graph;
+
+/* HnswIndex::HnswIndex(dim, metric, params) (include/vecdex/hnsw.hpp:67). */
+VDX_API int vdx_host_graph_create(uint64_t dim, uint32_t metric, uint64_t M, uint64_t ef_construction,
+                          uint64_t seed, vdx_host_graph** out);
+/* HnswIndex::add(MatrixView, num_threads) (include/vecdex/hnsw.hpp:87; src/hnsw.cpp:259-291). */
+VDX_API int vdx_host_graph_add(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t given_dim,
+                       uint64_t num_threads);
+/* HnswIndex::save / load (include/vecdex/hnsw.hpp:115,123; src/hnsw_io.cpp:341-467). */
+VDX_API int vdx_host_graph_save(const vdx_host_graph* g, const char* path);
+VDX_API int vdx_host_graph_load(const char* path, vdx_host_graph** out);
+/* Zero-copy view of the graph's sections, valid until the next add or free. */
+VDX_API int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out);
+VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
+                          uint64_t* version);
+VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
+
+/* HnswIndex::inspect (include/vecdex/hnsw.hpp:102; src/hnsw.cpp:527-594). */
+typedef struct vdx_graph_report {
+    uint64_t reachable;
+    uint64_t max_level;
+    uint64_t violation_count;
+    double mean_degree0;
+} vdx_graph_report;
+VDX_API int vdx_host_graph_inspect(const vdx_host_graph* g, vdx_graph_report* out, char* first_violation,
+                           uint64_t buflen);
+VDX_API void vdx_host_graph_free(vdx_host_graph* g);
+
+/* detail::check_new_vectors (src/vector_ops.cpp:36-58), with `where` as the message prefix. */
+VDX_API int vdx_host_check_new_vectors(const char* where, uint32_t metric, uint64_t dim,
+                               uint64_t index_size, const float* data, uint64_t rows,
+                               uint64_t given_dim);
+/* detail::normalize on each row (src/vector_ops.cpp:29-34), as BruteForceIndex::store does
+ * under cosine (src/brute_force.cpp:58-66). */
+VDX_API int vdx_host_normalize_rows(float* data, uint64_t rows, uint64_t dim);
+/* The library's distance (the arithmetic every kernel reproduces). */
+VDX_API float vdx_host_distance(uint32_t metric, const float* a, const float* b, uint64_t dim);
+
+/* ================================================================================= GPU side */
+
+typedef struct vdx_gpu_ctx vdx_gpu_ctx;
+
+/* Opens HIP device `device`; each context owns one stream. */
+VDX_API int vdx_gpu_open(int device, vdx_gpu_ctx** out);
+VDX_API void vdx_gpu_close(vdx_gpu_ctx* ctx);
+/* The context's stream (a hipStream_t), for callers that enqueue around the device entry points. */
+VDX_API void* vdx_gpu_stream(vdx_gpu_ctx* ctx);
+
+/* ---- HNSW: replaces HnswIndex::search / search_batch (include/vecdex/hnsw.hpp:93-99;
+ *      src/hnsw.cpp:484-522) on a graph uploaded once. */
+
+typedef struct vdx_gpu_hnsw vdx_gpu_hnsw;
+
+VDX_API int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* graph, vdx_gpu_hnsw** out);
+VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
+
+/* Knobs of one search launch. Zero-initialise for defaults (autotuned). */
+typedef struct vdx_search_opts {
+    uint32_t visited_log2;     /* LDS visited-cache slots = 2^visited_log2; 0: autotune */
+    uint32_t waves_per_group;  /* waves per workgroup; 0: autotune */
+    uint32_t termination;      /* 0: fixed ef (exact vecdex semantics); 1: learned stop */
+    uint32_t reserved;
+    const void* mlp;           /* learned stop model (vdx_mlp_weights), termination == 1 */
+    float mlp_threshold;       /* stop when the model's score exceeds this */
+    float reserved_f;
+    uint64_t* stats;           /* optional host array of VDX_STAT_COUNT counters, summed */
+} vdx_search_opts;
+
+enum vdx_stat {
+    VDX_STAT_EXPANSIONS = 0,   /* layer-0 link blocks read */
+    VDX_STAT_DISTANCES = 1,    /* distances computed (layer 0 and above) */
+    VDX_STAT_UPPER_BLOCKS = 2, /* upper-layer link blocks read */
+    VDX_STAT_RECOMPUTED = 3,   /* distances of already-visited nodes the lossy visited cache let through */
+    VDX_STAT_COUNT = 8
+};
+
+/* search_batch over `nq` host queries of `query_dim` floats each (row-major, contiguous).
+ * k and ef: per-query arrays `ks`/`efs` (may be NULL), else the uniform `k`/`ef`; ef below k acts
+ * as k. Each query is validated like detail::check_query with `where` as the message prefix
+ * ("HnswIndex::search_batch" if NULL) and normalized under cosine before upload.
+ * Results: out[q * out_stride + i] for i < out_count[q] <= k, closest first, ties by smaller id
+ * — element q equals HnswIndex::search(row q, k, ef). */
+VDX_API int vdx_gpu_hnsw_search(vdx_gpu_hnsw* index, const float* queries, uint64_t nq,
+                        uint64_t query_dim, const uint32_t* ks, uint32_t k, const uint32_t* efs,
+                        uint32_t ef, vdx_neighbor* out, uint64_t out_stride, uint32_t* out_count,
+                        const char* where, const vdx_search_opts* opts);
+
+/* The same on device memory, asynchronous on `stream` (the context's stream if NULL):
+ * d_queries are already prepared (validated; unit length under cosine), nq x dim floats.
+ * d_ks / d_efs may be NULL. max_k bounds every k (it sizes nothing; it is checked). */
+VDX_API int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* index, const float* d_queries, uint64_t nq,
+                               const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs,
+                               uint32_t ef, uint32_t max_k, vdx_neighbor* d_out,
+                               uint64_t out_stride, uint32_t* d_out_count,
+                               const vdx_search_opts* opts, void* stream);
+
+/* ---- Brute force: replaces BruteForceIndex::search (include/vecdex/brute_force.hpp:36;
+ *      src/brute_force.cpp:68-84), batched over queries. */
+
+typedef struct vdx_gpu_bf vdx_gpu_bf;
+
+enum vdx_precision {
+    VDX_PREC_FP32 = 0, /* FP32 MFMA shortlist, exact re-rank */
+    VDX_PREC_BF16 = 1, /* BF16 MFMA shortlist (FP32 accumulate), exact re-rank */
+    VDX_PREC_FP8 = 2,  /* FP8-e4m3 MFMA with per-vector scales, exact re-rank */
+    VDX_PREC_EXACT = 3 /* no GEMM: every distance in the exact order (slow; for checks) */
+};
+
+/* Vectors as BruteForceIndex stores them (unit length under cosine). */
+VDX_API int vdx_gpu_bf_upload(vdx_gpu_ctx* ctx, uint32_t metric, uint64_t dim, uint64_t n,
+                      const float* vectors, vdx_gpu_bf** out);
+VDX_API void vdx_gpu_bf_free(vdx_gpu_bf* index);
+
+typedef struct vdx_bf_stats {
+    uint64_t shortlist;        /* candidates kept per query in the first pass */
+    uint64_t widened;          /* queries whose shortlist failed the bound and were redone */
+    uint64_t exact_fallback;   /* queries finished by an exact scan */
+    uint64_t fp64_ties;        /* candidate pairs re-ranked in float64 (fp64_ties mode) */
+    double max_abs_error;      /* max |GEMM distance - exact distance| over re-ranked candidates */
+} vdx_bf_stats;
+
+/* The min(k, n) closest stored vectors of each query, ascending by (distance, id); the
+ * distances are the exact ones (bit-equal to BruteForceIndex::search). Queries validated and
+ * prepared as in vdx_gpu_hnsw_search ("BruteForceIndex::search" if where is NULL). */
+VDX_API int vdx_gpu_bf_search(vdx_gpu_bf* index, const float* queries, uint64_t nq, uint64_t query_dim,
+                      uint32_t k, uint32_t precision, vdx_neighbor* out, uint64_t out_stride,
+                      uint32_t* out_count, const char* where, vdx_bf_stats* stats);
+
+/* Device-memory variant: prepared d_queries, results to d_out (rows of out_stride), on the
+ * context's stream; returns after the results are complete. */
+VDX_API int vdx_gpu_bf_search_device(vdx_gpu_bf* index, const float* d_queries, uint64_t nq, uint32_t k,
+                             uint32_t precision, vdx_neighbor* d_out, uint64_t out_stride,
+                             uint32_t* d_out_count, vdx_bf_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VDXGPU_H */
