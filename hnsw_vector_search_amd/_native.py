@@ -76,6 +76,13 @@ _SIGS = {
     "vdx_gpu_open": (c_int, [c_int, c_ptr]),
     "vdx_gpu_close": (None, [c_ptr]),
     "vdx_gpu_stream": (c_ptr, [c_ptr]),
+    "vdx_gpu_alloc": (c_int, [c_ptr, c_u64, c_ptr]),
+    "vdx_gpu_free": (None, [c_ptr, c_ptr]),
+    "vdx_gpu_copy_to_device": (c_int, [c_ptr, c_ptr, c_ptr, c_u64]),
+    "vdx_gpu_copy_to_host": (c_int, [c_ptr, c_ptr, c_ptr, c_u64]),
+    "vdx_gpu_synchronize": (c_int, [c_ptr]),
+    "vdx_gpu_event_record": (c_int, [c_ptr, c_u32]),
+    "vdx_gpu_event_elapsed": (c_int, [c_ptr, c_u32, c_u32, ctypes.POINTER(c_float)]),
     "vdx_gpu_hnsw_upload": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),
     "vdx_gpu_hnsw_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_ptr, c_u32, c_ptr, c_u32,
@@ -138,3 +145,45 @@ class Context:
     @property
     def stream(self) -> int:
         return lib.vdx_gpu_stream(self.handle) or 0
+
+
+class DeviceArray:
+    """An HBM buffer owned through the library (one HIP runtime per process), shaped like a
+    numpy array for copies in and out."""
+
+    def __init__(self, shape, dtype, ctx: "Context | None" = None):
+        self.ctx = ctx or Context.get(0)
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = c_ptr()
+        check(lib.vdx_gpu_alloc(self.ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, ctx: "Context | None" = None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, ctx)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib.vdx_gpu_copy_to_device(self.ctx.handle, self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib.vdx_gpu_copy_to_host(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.vdx_gpu_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+def elapsed_ms(ctx: Context, start_slot: int, stop_slot: int) -> float:
+    ms = c_float()
+    check(lib.vdx_gpu_event_elapsed(ctx.handle, start_slot, stop_slot, ctypes.byref(ms)))
+    return ms.value

@@ -35,6 +35,7 @@ struct BeamArgs {
     Neighbor* out;
     std::uint64_t out_stride;
     std::uint32_t* out_count;
+    std::uint32_t k_cap;  // per-query k is clamped to this (<= out_stride, <= ef_cap)
     // launch state
     std::uint32_t* queue;  // next query to take; zero before the launch
     std::uint32_t ef_cap;  // R capacity per wave (>= every max(ef, k)), a multiple of 64

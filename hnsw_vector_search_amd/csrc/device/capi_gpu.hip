@@ -99,12 +99,12 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     p.waves = opts && opts->waves_per_group ? opts->waves_per_group : 4;
     require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
     p.lds_per_wave = vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
-    p.lds_bytes = p.lds_per_wave * p.waves;
-    if (p.lds_bytes > 160 * 1024) {
-        p.waves = 1;
-        p.lds_bytes = p.lds_per_wave;
+    const std::size_t limit = h.ctx->lds_per_block;
+    while (p.waves > 1 && p.lds_per_wave * p.waves > limit) {
+        --p.waves;
     }
-    require(p.lds_bytes <= 160 * 1024, "ef too large for the GPU beam's LDS");
+    p.lds_bytes = p.lds_per_wave * p.waves;
+    require(p.lds_bytes <= limit, "ef too large for the GPU beam's LDS");
     int per_cu = 0;
     check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
               &per_cu, vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim)), p.waves * 64,
@@ -152,6 +152,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     a.out = d_out;
     a.out_stride = out_stride;
     a.out_count = d_count;
+    a.k_cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(out_stride, plan.ef_cap));
     a.queue = d_queue;
     a.ef_cap = plan.ef_cap;
     a.visited_log2 = plan.visited_log2;
@@ -194,6 +195,7 @@ int vdx_gpu_open(int device, vdx_gpu_ctx** out) {
         hipDeviceProp_t prop{};
         check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         ctx->cu_count = prop.multiProcessorCount;
+        ctx->lds_per_block = prop.sharedMemPerBlock;
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = ctx.release();
     });
@@ -207,6 +209,72 @@ void vdx_gpu_close(vdx_gpu_ctx* ctx) {
 }
 
 void* vdx_gpu_stream(vdx_gpu_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int vdx_gpu_alloc(vdx_gpu_ctx* ctx, uint64_t bytes, void** out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "vdx_gpu_alloc: null argument");
+        DeviceGuard guard(ctx->device);
+        *out = nullptr;
+        check(hipMalloc(out, std::max<uint64_t>(bytes, 1)), "hipMalloc");
+    });
+}
+
+void vdx_gpu_free(vdx_gpu_ctx* ctx, void* ptr) {
+    if (ctx != nullptr && ptr != nullptr) {
+        DeviceGuard guard(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ptr);
+    }
+}
+
+int vdx_gpu_copy_to_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr, "vdx_gpu_copy_to_device: null context");
+        DeviceGuard guard(ctx->device);
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int vdx_gpu_copy_to_host(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr, "vdx_gpu_copy_to_host: null context");
+        DeviceGuard guard(ctx->device);
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int vdx_gpu_synchronize(vdx_gpu_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, "vdx_gpu_synchronize: null context");
+        DeviceGuard guard(ctx->device);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int vdx_gpu_event_record(vdx_gpu_ctx* ctx, uint32_t slot) {
+    return guarded([&] {
+        require(ctx != nullptr && slot < 16, "vdx_gpu_event_record: bad argument");
+        DeviceGuard guard(ctx->device);
+        if (ctx->events[slot] == nullptr) {
+            check(hipEventCreate(&ctx->events[slot]), "hipEventCreate");
+        }
+        check(hipEventRecord(ctx->events[slot], ctx->stream), "hipEventRecord");
+    });
+}
+
+int vdx_gpu_event_elapsed(vdx_gpu_ctx* ctx, uint32_t from, uint32_t to, float* ms) {
+    return guarded([&] {
+        require(ctx != nullptr && from < 16 && to < 16 && ms != nullptr,
+                "vdx_gpu_event_elapsed: bad argument");
+        require(ctx->events[from] != nullptr && ctx->events[to] != nullptr,
+                "vdx_gpu_event_elapsed: event not recorded");
+        DeviceGuard guard(ctx->device);
+        check(hipEventSynchronize(ctx->events[to]), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, ctx->events[from], ctx->events[to]), "hipEventElapsedTime");
+    });
+}
 
 int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw** out) {
     return guarded([&] {
@@ -237,6 +305,24 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         }
         require(acc == g->upper_blocks, "vdx_gpu_hnsw_upload: levels do not add up to upper_blocks");
         require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_upload: too many upper blocks");
+        require(g->n == 0 || g->levels[g->entry_point] == g->max_level,
+                "vdx_gpu_hnsw_upload: the entry point is not on the top layer");
+        // Every block the kernel may follow: counts within capacity, ids naming nodes. The kernel
+        // relies on both to keep its gathers in bounds.
+        for (std::uint64_t i = 0; i < g->n; ++i) {
+            const std::uint32_t* b = g->layer0 + i * (1 + 2 * g->M);
+            require(b[0] <= 2 * g->M, "vdx_gpu_hnsw_upload: a layer-0 block exceeds 2M links");
+            for (std::uint32_t j = 0; j < b[0]; ++j) {
+                require(b[1 + j] < g->n, "vdx_gpu_hnsw_upload: a layer-0 link names a missing node");
+            }
+        }
+        for (std::uint64_t blk = 0; blk < g->upper_blocks; ++blk) {
+            const std::uint32_t* b = g->upper + blk * (1 + g->M);
+            require(b[0] <= g->M, "vdx_gpu_hnsw_upload: an upper block exceeds M links");
+            for (std::uint32_t j = 0; j < b[0]; ++j) {
+                require(b[1 + j] < g->n, "vdx_gpu_hnsw_upload: an upper link names a missing node");
+            }
+        }
 
         DeviceGuard guard(ctx->device);
         auto h = std::make_unique<vdx_gpu_hnsw>();
@@ -458,9 +544,16 @@ void bf_exact(vdx_gpu_bf& b, const float* d_queries, std::uint64_t nq, std::uint
               vdx::dev::Neighbor* d_out, std::uint64_t out_stride, std::uint32_t* d_count,
               Workspace& ws, hipStream_t s) {
     const std::uint32_t kcap = std::min<std::uint64_t>(k, std::max<std::uint64_t>(1, b.n));
-    std::uint32_t waves = kcap <= 128 ? 4 : 1;
-    std::size_t lds = sizeof(float) * 32 * b.split.ld + static_cast<std::size_t>(waves) * 8 * kcap * 8;
-    require(lds <= 160 * 1024, "BruteForceIndex::search: k too large for the GPU path");
+    const std::size_t limit = b.ctx->lds_per_block;
+    const auto lds_for = [&](std::uint32_t w) {
+        return sizeof(float) * 32 * b.split.ld + static_cast<std::size_t>(w) * 8 * kcap * 8;
+    };
+    std::uint32_t waves = 4;
+    while (waves > 1 && lds_for(waves) > limit) {
+        --waves;
+    }
+    const std::size_t lds = lds_for(waves);
+    require(lds <= limit, "BruteForceIndex::search: k too large for the GPU path");
     const std::uint64_t qgroups = (nq + waves * 8 - 1) / (waves * 8);
     std::uint64_t slabs = std::max<std::uint64_t>(1, (2 * static_cast<std::uint64_t>(b.ctx->cu_count) * 2 + qgroups - 1) / qgroups);
     slabs = std::min<std::uint64_t>(slabs, 64);

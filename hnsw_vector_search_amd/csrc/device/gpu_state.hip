@@ -101,6 +101,11 @@ DimSplit split_dim(std::uint64_t dim) {
 
 vdx_gpu_ctx::~vdx_gpu_ctx() {
     free_workspaces.clear();
+    for (hipEvent_t& e : events) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+        }
+    }
     if (stream != nullptr) {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);

@@ -89,7 +89,9 @@ struct Workspace {
 struct vdx_gpu_ctx {
     int device = 0;
     int cu_count = 0;
+    std::size_t lds_per_block = 65536;
     hipStream_t stream = nullptr;
+    hipEvent_t events[16] = {};
     std::mutex pool_mutex;
     std::vector<std::unique_ptr<vdx::gpu::Workspace>> free_workspaces;
     ~vdx_gpu_ctx();

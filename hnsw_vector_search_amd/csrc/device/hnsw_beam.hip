@@ -175,9 +175,11 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         if (q >= a.nq) {
             break;
         }
-        const std::uint32_t k = uniform(a.ks ? a.ks[q] : a.k);
+        // The host sized R and the output rows; clamping keeps a caller's larger per-query
+        // value from reaching past them.
+        const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
         const std::uint32_t ef = uniform(a.efs ? a.efs[q] : a.ef);
-        const std::uint32_t efp = max(ef, k);
+        const std::uint32_t efp = min(max(ef, k), a.ef_cap);
         Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
         if (a.n == 0) {
             if (lane == 0) {

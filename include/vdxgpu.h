@@ -117,6 +117,18 @@ VDX_API void vdx_gpu_close(vdx_gpu_ctx* ctx);
 /* The context's stream (a hipStream_t), for callers that enqueue around the device entry points. */
 VDX_API void* vdx_gpu_stream(vdx_gpu_ctx* ctx);
 
+/* Device memory and timing plumbing on the context's stream, so callers that keep inputs
+ * resident in HBM need no second HIP runtime in their process. Copies are synchronous. */
+VDX_API int vdx_gpu_alloc(vdx_gpu_ctx* ctx, uint64_t bytes, void** out);
+VDX_API void vdx_gpu_free(vdx_gpu_ctx* ctx, void* ptr);
+VDX_API int vdx_gpu_copy_to_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+VDX_API int vdx_gpu_copy_to_host(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+VDX_API int vdx_gpu_synchronize(vdx_gpu_ctx* ctx);
+/* Records event `slot` (0..15) on the context's stream; elapsed milliseconds between two
+ * recorded events once both completed. */
+VDX_API int vdx_gpu_event_record(vdx_gpu_ctx* ctx, uint32_t slot);
+VDX_API int vdx_gpu_event_elapsed(vdx_gpu_ctx* ctx, uint32_t from, uint32_t to, float* ms);
+
 /* ---- HNSW: replaces HnswIndex::search / search_batch (include/vecdex/hnsw.hpp:93-99;
  *      src/hnsw.cpp:484-522) on a graph uploaded once. */
 

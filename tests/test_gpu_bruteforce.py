@@ -1,0 +1,84 @@
+"""GPU brute force against the oracle's restatement of BruteForceIndex::search (bit-exact) and
+against float64 (ids equal up to ties)."""
+
+import numpy as np
+import pytest
+
+import hnsw_vector_search_amd as vdx
+from hnsw_vector_search_amd import datasets
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def line_index(points):
+    index = vdx.BruteForceIndex(1)
+    for p in points:
+        index.add([p])
+    return index
+
+
+def pairs(result):
+    return [(n.distance, n.id) for n in result]
+
+
+@pytest.mark.parametrize("precision", ["exact"])
+def test_hand_cases(precision):
+    # tests/brute_force_test.cpp:46-73, 158-175
+    idx = line_index([10, 0, 7, 3, -2])
+    assert pairs(idx.search([2.0], 3, precision)) == [(1, 3), (4, 1), (16, 4)]
+    assert pairs(line_index([5, 1]).search([0.0], 10, precision)) == [(1, 1), (25, 0)]
+    ties = line_index([4, -4, 4, 9, -4])
+    assert pairs(ties.search([0.0], 4, precision)) == [(16, 0), (16, 1), (16, 2), (16, 4)]
+    assert pairs(ties.search([0.0], 2, precision)) == [(16, 0), (16, 1)]
+    ip = vdx.BruteForceIndex(2, vdx.Metric.InnerProduct)
+    for v in ([1, 0], [0, 2], [3, 3]):
+        ip.add(v)
+    assert pairs(ip.search([1.0, 1.0], 3, precision)) == [(-5, 2), (-1, 1), (0, 0)]
+    cos = vdx.BruteForceIndex(2, vdx.Metric.Cosine)
+    for v in ([10, 0], [0, 0.5], [-3, 0]):
+        cos.add(v)
+    assert pairs(cos.search([2.0, 0.0], 3, precision)) == [(0, 0), (1, 1), (2, 2)]
+    assert vdx.BruteForceIndex(3).search([1.0, 2.0, 3.0], 5, precision) == []
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+@pytest.mark.parametrize("dim", [1, 5, 24, 100, 128, 130, 287])
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_exact_mode_is_bit_equal_to_cpu(metric, dim, k):
+    print(f"case: bf exact metric {metric} dim {dim} k {k}", flush=True)
+    rng = np.random.default_rng(dim * 10 + k)
+    base = rng.uniform(-1, 1, (3000, dim)).astype(np.float32)
+    queries = rng.uniform(-1, 1, (37, dim)).astype(np.float32)
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(base)
+    got, cnt = index.search_arrays(queries, k, "exact")
+    prepared = o.normalize(queries) if metric == 2 else queries
+    ref = o.bruteforce_f32(index.vectors(), prepared, k, metric)
+    assert (cnt == min(k, 3000)).all()
+    assert got.tobytes() == ref.tobytes()
+
+
+def test_exact_mode_matches_float64_ids_on_integer_data():
+    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 10_000, 100)
+    index = vdx.BruteForceIndex(128)
+    index.add(base)
+    got, _ = index.search_arrays(queries, 10, "exact")
+    ids, dist = o.bruteforce_f64(base, queries, 10)
+    np.testing.assert_array_equal(got["id"], ids)  # integer data: exact distances, no ties lost
+    np.testing.assert_array_equal(got["distance"], dist.astype(np.float32))
+
+
+def test_full_ranking_when_k_is_n():
+    # tests/brute_force_test.cpp:111-151
+    rng = np.random.default_rng(42)
+    base = rng.uniform(-1, 1, (500, 16)).astype(np.float32)
+    index = vdx.BruteForceIndex(16)
+    index.add(base)
+    q = rng.uniform(-1, 1, (20, 16)).astype(np.float32)
+    got, _ = index.search_arrays(q, 500, "exact")
+    for r in range(20):
+        assert sorted(got[r]["id"].tolist()) == list(range(500))
+        for k in (1, 10, 100):
+            sub, _ = index.search_arrays(q[r:r + 1], k, "exact")
+            assert sub[0].tobytes() == got[r, :k].tobytes()
