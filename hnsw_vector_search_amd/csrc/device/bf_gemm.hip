@@ -1,0 +1,557 @@
+// K2/K3: brute force as an MFMA GEMM with a fused threshold filter, then an exact re-rank.
+//
+// BruteForceIndex::search (vecdex src/brute_force.cpp:20-84) returns the k smallest exact
+// distances. Here, per batch of queries:
+//
+//  1. prepare: base rows and queries are quantized once into the MFMA operand format (FP32,
+//     BF16, or FP8-e4m3 with a per-vector scale), with ||v||^2, the scale, and the exact
+//     quantization error ||v~ - v|| and norm ||v~|| of every vector (bf_prepare_rows).
+//  2. threshold: distances from every query to a strided sample of the base (the same GEMM,
+//     dense output) give, by radix select, a per-query threshold tau whose expected number of
+//     survivors over the whole base is a small multiple of the shortlist size.
+//  3. filter GEMM: Q x X^T on MFMA, dist = |q|^2 + |x|^2 - 2 q.x (L2) or 1 - q.x; each
+//     distance <= tau is appended (key = (distance, id)) to its query's candidate buffer.
+//  4. shortlist: per query, radix select of the S smallest keys, and the smallest distance
+//     left out (L).
+//  5. re-rank: exact-order distances of the S candidates (the library's arithmetic, bit-equal
+//     to vecdex), sorted by (distance, id); the first k are the result.
+//  6. certification: every vector left out has GEMM distance >= L, so its exact distance is
+//     >= L - E, where E bounds |GEMM distance - exact distance| for this query over all rows
+//     (Cauchy-Schwarz on the measured quantization errors, plus FP32 accumulation and rounding
+//     terms). If L - E > (k-th exact distance) the result equals the exact scan; otherwise the
+//     query is redone by the exact scan (bf_exact.hip). Every mode therefore returns exactly
+//     the exact result.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "bf_gemm_args.hpp"
+#include "device_common.hpp"
+#include "gather_distances.hpp"
+
+namespace vdx::dev {
+
+namespace {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, kWave);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, off, kWave));
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned char to_fp8(float x) {
+    // OCP e4m3 on gfx950, round to nearest even, saturating to +-448.
+    const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false);
+    return static_cast<unsigned char>(packed & 0xFF);
+}
+
+__device__ __forceinline__ float from_fp8(unsigned char b) {
+    return __builtin_amdgcn_cvt_f32_fp8(static_cast<int>(b), 0);
+}
+
+}  // namespace
+
+// One wave per row: quantizes `rows` rows of `dim` floats (row stride ld_src) into `dst`
+// (rows of kp elements, zero padded), and writes ||v||^2, the scale (FP8; 1 otherwise), the
+// quantization error ||v~ - v|| and ||v~||, where v~ is the dequantized vector.
+__global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
+    const int lane = lane_id();
+    const std::uint32_t row = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (row >= a.rows) {
+        return;
+    }
+    const float* src = a.src + static_cast<std::size_t>(row) * a.ld_src;
+    float maxabs = 0.f;
+    for (std::uint32_t e = lane; e < a.dim; e += kWave) {
+        maxabs = fmaxf(maxabs, fabsf(src[e]));
+    }
+    maxabs = wave_max(maxabs);
+    const float scale = (a.prec == kPrecFP8 && maxabs > 0.f) ? maxabs / 448.0f : 1.0f;
+    const float inv = 1.0f / scale;
+    float n2 = 0.f, err2 = 0.f, qn2 = 0.f;
+    for (std::uint32_t e = lane; e < a.kp; e += kWave) {
+        const float x = e < a.dim ? src[e] : 0.f;
+        float xq;
+        if (a.prec == kPrecF32) {
+            xq = x;
+            static_cast<float*>(a.dst)[static_cast<std::size_t>(row) * a.kp + e] = x;
+        } else if (a.prec == kPrecBF16) {
+            const __bf16 b = static_cast<__bf16>(x);
+            xq = static_cast<float>(b);
+            static_cast<__bf16*>(a.dst)[static_cast<std::size_t>(row) * a.kp + e] = b;
+        } else {
+            const unsigned char b = to_fp8(x * inv);
+            xq = from_fp8(b) * scale;
+            static_cast<unsigned char*>(a.dst)[static_cast<std::size_t>(row) * a.kp + e] = b;
+        }
+        n2 += x * x;
+        err2 += (xq - x) * (xq - x);
+        qn2 += xq * xq;
+    }
+    n2 = wave_sum(n2);
+    err2 = wave_sum(err2);
+    qn2 = wave_sum(qn2);
+    if (lane == 0) {
+        a.norm2[row] = n2;
+        a.scale[row] = scale;
+        a.qerr[row] = sqrtf(err2);
+        a.qnorm[row] = sqrtf(qn2);
+    }
+}
+
+// The GEMM. A workgroup of 4 waves takes 128 queries (32 per wave, one MFMA row tile each,
+// its operand fragments held in registers) and one slab of base rows, 64 rows (two MFMA column
+// tiles) per step; the 4 waves read the same base rows, which the L1/L2 serve.
+//
+// MFMA 32x32 output layout: lane l holds column n = l % 32 and rows
+// m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) of its 16 accumulators i. Operand fragments: lane l
+// holds row/column l % 32 and the K slice [kstep * K + (l / 32) * K / 2, ... + K / 2).
+template <int PREC>
+__global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 31;
+    const int half = lane >> 5;
+    const std::uint32_t q0 = blockIdx.x * 128 + wave * 32;  // this wave's first query
+    const std::uint32_t kp = a.kp;
+
+    // A fragments (queries), per K step.
+    constexpr int KS = PREC == kPrecF32 ? 32 : 8;  // float2 steps pairs (F32) or 16-wide steps
+    typedef typename std::conditional<PREC == kPrecF32, float2,
+                                      typename std::conditional<PREC == kPrecBF16, v8bf, long>::type>::type Frag;
+    Frag af[KS];
+    const std::uint32_t ks = a.ks;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (static_cast<std::uint32_t>(s) < ks) {
+            const std::size_t qrow = static_cast<std::size_t>(q0 + col) * kp;
+            if constexpr (PREC == kPrecF32) {
+                af[s] = *reinterpret_cast<const float2*>(static_cast<const float*>(a.qq) + qrow + 4 * s + 2 * half);
+            } else if constexpr (PREC == kPrecBF16) {
+                af[s] = *reinterpret_cast<const v8bf*>(static_cast<const __bf16*>(a.qq) + qrow + 16 * s + 8 * half);
+            } else {
+                af[s] = *reinterpret_cast<const long*>(static_cast<const unsigned char*>(a.qq) + qrow + 16 * s + 8 * half);
+            }
+        }
+    }
+    // Per-lane query constants for the 16 rows this lane's accumulators cover.
+    float qn[16], qs[16], tau[16];
+    std::uint32_t qidx[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const std::uint32_t q = q0 + 8 * (i / 4) + 4 * half + (i % 4);
+        qidx[i] = q;
+        const bool ok = q < a.nq;
+        qn[i] = ok ? a.qn[q] : 0.f;
+        qs[i] = ok && a.qs ? a.qs[q] : 1.f;
+        tau[i] = ok && a.tau ? a.tau[q] : __int_as_float(0x7f800000);
+    }
+
+    const std::uint32_t tiles = (a.rows + 63) / 64;
+    const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
+    const std::uint32_t t0 = blockIdx.y * per_slab;
+    const std::uint32_t t1 = min(tiles, t0 + per_slab);
+    for (std::uint32_t t = t0; t < t1; ++t) {
+        v16f acc[2];
+        acc[0] = v16f{};
+        acc[1] = v16f{};
+        std::uint32_t xrow[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const std::uint32_t j = t * 64 + h * 32 + col;  // index among the covered rows
+            // base row (strided in the sample); past the last covered row, load the last one
+            // again (masked in the epilogue) so every load stays inside the operand buffer
+            xrow[h] = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            if (static_cast<std::uint32_t>(s) < ks) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const std::size_t xr = static_cast<std::size_t>(xrow[h]) * kp;
+                    if constexpr (PREC == kPrecF32) {
+                        const float2 b = *reinterpret_cast<const float2*>(static_cast<const float*>(a.xq) + xr + 4 * s + 2 * half);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[s].x, b.x, acc[h], 0, 0, 0);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[s].y, b.y, acc[h], 0, 0, 0);
+                    } else if constexpr (PREC == kPrecBF16) {
+                        const v8bf b = *reinterpret_cast<const v8bf*>(static_cast<const __bf16*>(a.xq) + xr + 16 * s + 8 * half);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[s], b, acc[h], 0, 0, 0);
+                    } else {
+                        const long b = *reinterpret_cast<const long*>(static_cast<const unsigned char*>(a.xq) + xr + 16 * s + 8 * half);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(af[s], b, acc[h], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // Epilogue: distances, then the dense store (sample pass) or the threshold filter.
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const std::uint32_t j = t * 64 + h * 32 + col;
+            const bool row_ok = j < a.rows;
+            const std::uint32_t r = xrow[h];
+            const float xn = row_ok ? a.xn[r] : 0.f;
+            const float xs = row_ok && a.xs ? a.xs[r] : 1.f;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                float dot = acc[h][i];
+                if constexpr (PREC == kPrecFP8) {
+                    dot = dot * qs[i] * xs;
+                }
+                const float d = a.l2 ? (qn[i] + xn) - 2.0f * dot : 1.0f - dot;
+                const std::uint32_t q = qidx[i];
+                if (row_ok && q < a.nq) {
+                    if (a.mode == 0) {
+                        a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;
+                    } else if (d <= tau[i]) {
+                        const std::uint32_t slot = atomicAdd(a.cnt + q, 1u);
+                        if (slot < a.cap) {
+                            a.buf[static_cast<std::size_t>(q) * a.cap + slot] = make_key(d, r);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Per query (one workgroup): the `rank`-th smallest (1-based) of `count` u32 sortable
+// distances (dense sample row), by an MSB-first radix select, 8 bits per pass.
+__global__ __launch_bounds__(256) void bf_threshold_kernel(ThresholdArgs a) {
+    __shared__ std::uint32_t hist[256];
+    __shared__ std::uint32_t sel_digit, sel_want;
+    const std::uint32_t q = blockIdx.x;
+    const float* row = a.dense + static_cast<std::size_t>(q) * a.dense_ld;
+    std::uint32_t prefix = 0, mask = 0, want = a.rank;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        for (std::uint32_t i = threadIdx.x; i < a.count; i += blockDim.x) {
+            const std::uint32_t v = sortable(row[i]);
+            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            std::uint32_t cum = 0, b = 0;
+            for (; b < 255; ++b) {
+                if (cum + hist[b] >= want) break;
+                cum += hist[b];
+            }
+            sel_digit = b;
+            sel_want = want - cum;
+        }
+        __syncthreads();
+        prefix |= sel_digit << shift;
+        mask |= 255u << shift;
+        want = sel_want;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.tau[q] = unsortable(prefix);
+}
+
+// Per query (one workgroup): from the candidate buffer keep the S smallest keys (exactly S,
+// keys are unique) in the shortlist, and record the smallest GEMM distance left out: the
+// (S+1)-th key's when the buffer held more than S, else tau (everything left out was > tau).
+__global__ __launch_bounds__(256) void bf_shortlist_kernel(ShortlistArgs a) {
+    __shared__ std::uint32_t hist[256];
+    __shared__ std::uint32_t sel_digit, sel_want, taken;
+    const std::uint32_t q = blockIdx.x;
+    const std::uint32_t total = a.cnt[q];
+    const std::uint64_t* buf = a.buf + static_cast<std::size_t>(q) * a.cap;
+    std::uint32_t* list = a.list + static_cast<std::size_t>(q) * a.S;
+    if (total > a.cap) {  // overflowed: the exact scan redoes this query
+        if (threadIdx.x == 0) {
+            a.list_count[q] = 0;
+            a.left[q] = -__int_as_float(0x7f800000);
+            a.strict[q] = 1;
+        }
+        return;
+    }
+    if (total <= a.S) {
+        for (std::uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            list[i] = key_id(buf[i]);
+            a.approx[static_cast<std::size_t>(q) * a.S + i] = key_distance(buf[i]);
+        }
+        if (threadIdx.x == 0) {
+            a.list_count[q] = total;
+            a.left[q] = a.tau ? a.tau[q] : __int_as_float(0x7f800000);
+            a.strict[q] = 0;  // left out means GEMM distance > tau
+        }
+        return;
+    }
+    std::uint64_t prefix = 0, mask = 0;
+    std::uint32_t want = a.S;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        for (std::uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const std::uint64_t v = buf[i];
+            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            std::uint32_t cum = 0, b = 0;
+            for (; b < 255; ++b) {
+                if (cum + hist[b] >= want) break;
+                cum += hist[b];
+            }
+            sel_digit = b;
+            sel_want = want - cum;
+        }
+        __syncthreads();
+        prefix |= static_cast<std::uint64_t>(sel_digit) << shift;
+        mask |= 255ull << shift;
+        want = sel_want;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        taken = 0;
+    }
+    __syncthreads();
+    float my_left = __int_as_float(0x7f800000);
+    for (std::uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const std::uint64_t v = buf[i];
+        if (v <= prefix) {
+            const std::uint32_t at = atomicAdd(&taken, 1u);
+            if (at < a.S) {  // exactly S keys qualify (keys are unique); the test keeps writes in bounds
+                list[at] = key_id(v);
+                a.approx[static_cast<std::size_t>(q) * a.S + at] = key_distance(v);
+            }
+        } else {
+            my_left = fminf(my_left, key_distance(v));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) my_left = fminf(my_left, __shfl_xor(my_left, off, kWave));
+    __shared__ float wave_left[4];
+    if ((threadIdx.x & 63) == 0) wave_left[threadIdx.x >> 6] = my_left;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = wave_left[0];
+        for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) m = fminf(m, wave_left[w]);
+        a.list_count[q] = a.S;
+        a.left[q] = m;
+        a.strict[q] = 1;  // left out means GEMM distance >= left
+    }
+}
+
+// One wave per query: exact-order distances of the shortlist, sorted by (distance, id); the
+// first k are the result. Then the certification test of the file comment.
+template <int NB, int N8, int TAIL, int R>
+__global__ __launch_bounds__(256) void bf_rerank_kernel(RerankArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int m = lane & 7;
+    const std::uint32_t q = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (q >= a.nq) {
+        return;
+    }
+    unsigned char* base = lds + static_cast<std::size_t>(wave) * a.lds_per_wave;
+    std::uint64_t* keys = reinterpret_cast<std::uint64_t*>(base);
+    std::uint32_t* fid = reinterpret_cast<std::uint32_t*>(keys + a.sort_cap);
+    float* fd = reinterpret_cast<float*>(fid + 8 * R);
+    const int nb = static_cast<int>(a.nb), n8 = static_cast<int>(a.n8);
+    const int tail = static_cast<int>(a.tail), first_tail = static_cast<int>(a.first_tail);
+    const RowSource src{a.vectors, a.ld, a.l2};
+
+    const float* qp = a.queries + static_cast<std::size_t>(q) * a.qld;
+    float4 qb[NB > 0 ? NB : 1];
+    float4 q8[N8 > 0 ? N8 : 1];
+    float qt[8];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        qb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < nb) {
+            const float* p = qp + 32 * i + 4 * m;
+            qb[i] = make_float4(p[0], p[1], p[2], p[3]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < N8; ++t) {
+        q8[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t < n8 && m < 2) {
+            const float* p = qp + 32 * nb + 8 * t + 4 * m;
+            q8[t] = make_float4(p[0], p[1], p[2], p[3]);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        qt[e] = (e < TAIL && e < tail) ? qp[first_tail + e] : 0.f;
+    }
+
+    const std::uint32_t cnt = a.list_count[q];
+    const std::uint32_t* list = a.list + static_cast<std::size_t>(q) * a.S;
+    const float* approx = a.approx + static_cast<std::size_t>(q) * a.S;
+    float max_err = 0.f;
+    for (std::uint32_t c0 = 0; c0 < cnt; c0 += 8 * R) {
+        const std::uint32_t fc = min(8u * R, cnt - c0);
+        if (static_cast<std::uint32_t>(lane) < fc) {
+            fid[lane] = list[c0 + lane];
+        }
+        wave_lds_fence();
+        distances<NB, N8, TAIL, R>(src, fid, fc, qb, q8, qt, nb, n8, tail, first_tail, fd);
+        wave_lds_fence();
+        if (static_cast<std::uint32_t>(lane) < fc) {
+            keys[c0 + lane] = make_key(fd[lane], fid[lane]);
+            max_err = fmaxf(max_err, fabsf(fd[lane] - approx[c0 + lane]));
+        }
+        wave_lds_fence();
+    }
+    for (std::uint32_t i = cnt + lane; i < a.sort_cap; i += kWave) {
+        keys[i] = ~0ull;
+    }
+    wave_lds_fence();
+    // Bitonic sort of sort_cap (a power of two) keys.
+    for (std::uint32_t size = 2; size <= a.sort_cap; size <<= 1) {
+        for (std::uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (std::uint32_t i = lane; i < a.sort_cap; i += kWave) {
+                const std::uint32_t j = i ^ stride;
+                if (j > i) {
+                    const std::uint64_t x = keys[i], y = keys[j];
+                    const bool ascending = (i & size) == 0;
+                    if ((x > y) == ascending) {
+                        keys[i] = y;
+                        keys[j] = x;
+                    }
+                }
+            }
+            wave_lds_fence();
+        }
+    }
+    const std::uint32_t want = min(a.k, a.n);
+    const std::uint32_t got = min(want, cnt);
+    Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
+    for (std::uint32_t i = lane; i < got; i += kWave) {
+        out[i] = Neighbor{key_distance(keys[i]), key_id(keys[i])};
+    }
+    // Certification (file comment, step 6).
+    for (int off = 32; off > 0; off >>= 1) max_err = fmaxf(max_err, __shfl_xor(max_err, off, kWave));
+    if (lane == 0) {
+        a.out_count[q] = got;
+        atomicMax(reinterpret_cast<unsigned int*>(a.max_err), __float_as_uint(max_err));
+        bool ok = got == want;
+        if (ok) {
+            const float left = a.left[q];
+            if (left != __int_as_float(0x7f800000)) {
+                const float dk = key_distance(keys[want - 1]);
+                const float qnorm = a.q_qnorm[q];   // |q~|
+                const float qerr = a.q_qerr[q];     // |q~ - q|
+                const float qlen = sqrtf(a.q_norm2[q]);
+                const float e_dot = qnorm * a.x_max_qerr + qerr * a.x_max_norm +
+                                    a.gamma_k * qnorm * a.x_max_qnorm;
+                float lower;
+                if (a.l2) {
+                    const float e = 2.0f * e_dot + (a.gamma_d + 2.4e-7f) *
+                                    (a.q_norm2[q] + a.x_max_norm2 + 2.0f * qnorm * a.x_max_qnorm);
+                    lower = left * (1.0f - a.gamma_d) - e * (1.0f + a.gamma_d);
+                } else {
+                    const float e = e_dot + a.gamma_d * qlen * a.x_max_norm +
+                                    2.4e-7f * (1.0f + fabsf(left) + e_dot);
+                    lower = left - e;
+                }
+                lower = lower * (lower >= 0.f ? 0.99999f : 1.00001f) - 1e-30f;  // margin
+                ok = a.strict[q] ? lower > dk : lower >= dk;
+            }
+        }
+        a.fail[q] = ok ? 0 : 1;
+    }
+}
+
+// Copies the rows listed in idx (count of them) from src to consecutive rows of dst.
+__global__ void bf_gather_rows_kernel(const float* src, const std::uint32_t* idx, std::uint32_t count,
+                                      std::uint32_t dim, float* dst) {
+    const std::uint32_t r = blockIdx.x;
+    if (r >= count) return;
+    const float* from = src + static_cast<std::size_t>(idx[r]) * dim;
+    for (std::uint32_t e = threadIdx.x; e < dim; e += blockDim.x) {
+        dst[static_cast<std::size_t>(r) * dim + e] = from[e];
+    }
+}
+
+// Writes result row r of `part` (rows of `width`) back to row idx[r] of out (rows of stride).
+__global__ void bf_scatter_results_kernel(const Neighbor* part, const std::uint32_t* part_count,
+                                          const std::uint32_t* idx, std::uint32_t count,
+                                          std::uint32_t width, Neighbor* out,
+                                          std::uint64_t out_stride, std::uint32_t* out_count) {
+    const std::uint32_t r = blockIdx.x;
+    if (r >= count) return;
+    const std::uint32_t q = idx[r];
+    const std::uint32_t c = part_count[r];
+    for (std::uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+        out[static_cast<std::size_t>(q) * out_stride + i] = part[static_cast<std::size_t>(r) * width + i];
+    }
+    if (threadIdx.x == 0) out_count[q] = c;
+}
+
+hipError_t launch_bf_gather_rows(const float* src, const std::uint32_t* idx, std::uint32_t count,
+                                 std::uint32_t dim, float* dst, hipStream_t stream) {
+    hipLaunchKernelGGL(bf_gather_rows_kernel, dim3(count), dim3(128), 0, stream, src, idx, count, dim, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_scatter_results(const Neighbor* part, const std::uint32_t* part_count,
+                                     const std::uint32_t* idx, std::uint32_t count,
+                                     std::uint32_t width, Neighbor* out, std::uint64_t out_stride,
+                                     std::uint32_t* out_count, hipStream_t stream) {
+    hipLaunchKernelGGL(bf_scatter_results_kernel, dim3(count), dim3(128), 0, stream, part, part_count,
+                       idx, count, width, out, out_stride, out_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
+    const unsigned blocks = (args.rows + 3) / 4;
+    hipLaunchKernelGGL(bf_prepare_rows, dim3(blocks), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec,
+                          hipStream_t stream) {
+    const dim3 grid(qtiles, slabs);
+    if (prec == kPrecF32) {
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecF32>, grid, dim3(256), 0, stream, args);
+    } else if (prec == kPrecBF16) {
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecBF16>, grid, dim3(256), 0, stream, args);
+    } else {
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecFP8>, grid, dim3(256), 0, stream, args);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_threshold(const ThresholdArgs& args, unsigned nq, hipStream_t stream) {
+    hipLaunchKernelGGL(bf_threshold_kernel, dim3(nq), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_shortlist(const ShortlistArgs& args, unsigned nq, hipStream_t stream) {
+    hipLaunchKernelGGL(bf_shortlist_kernel, dim3(nq), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_rerank(const RerankArgs& args, unsigned nq, std::size_t lds_per_wave,
+                            hipStream_t stream, int shape) {
+    const unsigned waves = 4;
+    const dim3 grid((nq + waves - 1) / waves);
+    const std::size_t lds = lds_per_wave * waves;
+    if (shape == 0) {
+        hipLaunchKernelGGL((bf_rerank_kernel<4, 0, 0, 4>), grid, dim3(waves * 64), lds, stream, args);
+    } else if (shape == 1) {
+        hipLaunchKernelGGL((bf_rerank_kernel<3, 0, 4, 4>), grid, dim3(waves * 64), lds, stream, args);
+    } else {
+        hipLaunchKernelGGL((bf_rerank_kernel<8, 3, 7, 1>), grid, dim3(waves * 64), lds, stream, args);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace vdx::dev

@@ -15,6 +15,8 @@
 #include "../host/validate.hpp"
 #include "beam_args.hpp"
 #include "bf_args.hpp"
+#include "bf_host.hpp"
+#include "gpu_index.hpp"
 #include "gpu_state.hpp"
 
 using vdx::guarded;
@@ -23,26 +25,6 @@ using vdx::gpu::check;
 using vdx::gpu::DeviceBuffer;
 using vdx::gpu::DeviceGuard;
 using vdx::gpu::WorkspaceLease;
-
-struct vdx_gpu_hnsw {
-    vdx_gpu_ctx* ctx = nullptr;
-    std::uint32_t metric = 0;
-    std::uint64_t dim = 0, n = 0, M = 0, max_level = 0, entry_point = 0;
-    vdx::gpu::DimSplit split{};
-    DeviceBuffer vectors, layer0, upper, upper_offset;
-    // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in
-    // turn, so up to kRingSlots calls in flight on different streams never share one.
-    DeviceBuffer ring;
-    std::atomic<std::uint32_t> next_slot{0};
-};
-
-struct vdx_gpu_bf {
-    vdx_gpu_ctx* ctx = nullptr;
-    std::uint32_t metric = 0;
-    std::uint64_t dim = 0, n = 0;
-    vdx::gpu::DimSplit split{};
-    DeviceBuffer vectors;
-};
 
 namespace {
 
@@ -617,6 +599,13 @@ int vdx_gpu_bf_search_device(vdx_gpu_bf* b, const float* d_queries, uint64_t nq,
             case VDX_PREC_EXACT:
                 vdx::gpu::bf_exact(*b, d_queries, nq, k, reinterpret_cast<vdx::dev::Neighbor*>(d_out),
                                    out_stride, d_out_count, *ws, s);
+                break;
+            case VDX_PREC_FP32:
+            case VDX_PREC_BF16:
+            case VDX_PREC_FP8:
+                vdx::gpu::bf_mfma(*b, static_cast<int>(precision), d_queries, nq, k,
+                                  reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride,
+                                  d_out_count, *ws, s, stats);
                 break;
             default:
                 throw std::invalid_argument("BruteForceIndex::search: unknown precision mode");

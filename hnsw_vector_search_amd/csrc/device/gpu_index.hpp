@@ -1,0 +1,44 @@
+#pragma once
+
+// The HBM copies behind the C ABI's opaque handles.
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+
+#include "gpu_state.hpp"
+
+struct vdx_gpu_hnsw {
+    vdx_gpu_ctx* ctx = nullptr;
+    std::uint32_t metric = 0;
+    std::uint64_t dim = 0, n = 0, M = 0, max_level = 0, entry_point = 0;
+    vdx::gpu::DimSplit split{};
+    vdx::gpu::DeviceBuffer vectors, layer0, upper, upper_offset;
+    // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in
+    // turn, so up to 64 calls in flight on different streams never share one.
+    vdx::gpu::DeviceBuffer ring;
+    std::atomic<std::uint32_t> next_slot{0};
+};
+
+namespace vdx::gpu {
+
+// A base quantized for one MFMA precision, made on first use and kept.
+struct BfOperands {
+    bool ready = false;
+    std::uint32_t kp = 0;      // operand row length (elements)
+    std::uint64_t rows = 0;    // padded row count (multiple of 64)
+    DeviceBuffer xq, norm2, scale, qerr, qnorm;
+    float max_qerr = 0.f, max_norm = 0.f, max_norm2 = 0.f, max_qnorm = 0.f;
+};
+
+}  // namespace vdx::gpu
+
+struct vdx_gpu_bf {
+    vdx_gpu_ctx* ctx = nullptr;
+    std::uint32_t metric = 0;
+    std::uint64_t dim = 0, n = 0;
+    vdx::gpu::DimSplit split{};
+    vdx::gpu::DeviceBuffer vectors;
+    std::mutex prep_mutex;
+    vdx::gpu::BfOperands prep[3];
+};

@@ -80,6 +80,7 @@ private:
 struct Workspace {
     hipStream_t stream = nullptr;
     DeviceBuffer queries, ks, efs, out, count, counters, aux0, aux1, aux2, aux3;
+    DeviceBuffer extra[20];  // the MFMA brute force's per-call buffers
     PinnedBuffer h_queries, h_out, h_count;
     ~Workspace();
 };

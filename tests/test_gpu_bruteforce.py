@@ -82,3 +82,56 @@ def test_full_ranking_when_k_is_n():
         for k in (1, 10, 100):
             sub, _ = index.search_arrays(q[r:r + 1], k, "exact")
             assert sub[0].tobytes() == got[r, :k].tobytes()
+
+
+# ---- MFMA shortlist modes: every mode returns exactly the exact scan's result -----------------
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
+@pytest.mark.parametrize("metric", [0, 1, 2])
+@pytest.mark.parametrize("dim", [5, 24, 100, 128])
+def test_mfma_modes_equal_exact_small_base(precision, metric, dim):
+    """n = 3000: every row reaches the shortlist step (no sampling)."""
+    print(f"case: bf {precision} metric {metric} dim {dim} n 3000", flush=True)
+    rng = np.random.default_rng(dim + 7 * metric)
+    base = rng.uniform(-1, 1, (3000, dim)).astype(np.float32)
+    queries = rng.uniform(-1, 1, (40, dim)).astype(np.float32)
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(base)
+    for k in (1, 10, 100):
+        exact, _ = index.search_arrays(queries, k, "exact")
+        got, cnt = index.search_arrays(queries, k, precision)
+        assert got.tobytes() == exact.tobytes(), (precision, metric, dim, k)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
+@pytest.mark.parametrize("metric", [0, 2])
+def test_mfma_modes_equal_exact_with_sampled_threshold(precision, metric):
+    """n = 60,000: the threshold pass, the filter and the certification all run."""
+    print(f"case: bf {precision} metric {metric} n 60000", flush=True)
+    if metric == 0:
+        base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 60_000, 300)
+        dim = 128
+    else:
+        gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
+        base, queries = gen.sample(60_000, 1), gen.sample(300, 2)
+        dim = 100
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(base)
+    for k in (1, 10, 100):
+        exact, _ = index.search_arrays(queries, k, "exact")
+        got, cnt = index.search_arrays(queries, k, precision)
+        stats = index.last_stats
+        print(f"  k {k}: shortlist {stats.shortlist}, exact fallbacks {stats.exact_fallback}, "
+              f"max |gemm - exact| {stats.max_abs_error:.3g}", flush=True)
+        assert got.tobytes() == exact.tobytes(), (precision, metric, k)
+        assert stats.shortlist >= k
+
+
+def test_bf16_is_exact_on_small_integers():
+    """Integer data in [0, 255] is exact in BF16, so the GEMM distances are exact."""
+    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 20_000, 100)
+    index = vdx.BruteForceIndex(128)
+    index.add(base)
+    got, _ = index.search_arrays(queries, 10, "bf16")
+    assert index.last_stats.max_abs_error == 0.0
+    assert index.last_stats.exact_fallback == 0
