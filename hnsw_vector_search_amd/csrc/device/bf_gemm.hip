@@ -82,6 +82,7 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
     const float scale = (a.prec == kPrecFP8 && maxabs > 0.f) ? maxabs / 448.0f : 1.0f;
     const float inv = 1.0f / scale;
     float n2 = 0.f, err2 = 0.f, qn2 = 0.f;
+    bool integral = true;
     for (std::uint32_t e = lane; e < a.kp; e += kWave) {
         const float x = e < a.dim ? src[e] : 0.f;
         float xq;
@@ -100,7 +101,9 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
         n2 += x * x;
         err2 += (xq - x) * (xq - x);
         qn2 += xq * xq;
+        integral = integral && xq == x && rintf(x) == x;
     }
+    integral = ballot(!integral) == 0;
     n2 = wave_sum(n2);
     err2 = wave_sum(err2);
     qn2 = wave_sum(qn2);
@@ -109,6 +112,7 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
         a.scale[row] = scale;
         a.qerr[row] = sqrtf(err2);
         a.qnorm[row] = sqrtf(qn2);
+        a.integral[row] = integral ? 1u : 0u;
     }
 }
 
@@ -442,7 +446,13 @@ __global__ __launch_bounds__(256) void bf_rerank_kernel(RerankArgs a) {
         a.out_count[q] = got;
         atomicMax(reinterpret_cast<unsigned int*>(a.max_err), __float_as_uint(max_err));
         bool ok = got == want;
-        if (ok) {
+        // Exact GEMM: integral operands, lossless quantization, and every partial sum an
+        // integer below 2^24: partial sums of products are bounded by |q||x| (Cauchy-Schwarz),
+        // and |q|^2 + |x|^2 and |q|^2 + |x|^2 - 2 q.x by (|q| + |x|)^2. Then every GEMM distance
+        // equals the exact one and the S smallest keys contain the k smallest: certified.
+        const float reach = a.q_qnorm[q] + a.x_max_qnorm;
+        const bool exact_gemm = a.x_integral && a.q_integral[q] && reach * reach < 16777216.0f;
+        if (ok && !exact_gemm) {
             const float left = a.left[q];
             if (left != __int_as_float(0x7f800000)) {
                 const float dk = key_distance(keys[want - 1]);

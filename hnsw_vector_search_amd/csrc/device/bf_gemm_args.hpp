@@ -22,6 +22,7 @@ struct PrepArgs {
     float* scale;   // FP8 per-vector scale (1 otherwise)
     float* qerr;    // ||v~ - v||
     float* qnorm;   // ||v~||
+    std::uint32_t* integral;  // 1 if every element is an integer and v~ == v exactly
 };
 
 struct GemmArgs {
@@ -76,6 +77,8 @@ struct RerankArgs {
     const float* q_norm2;  // ||q||^2
     const float* q_qerr;   // ||q~ - q||
     const float* q_qnorm;  // ||q~||
+    const std::uint32_t* q_integral;
+    std::uint32_t x_integral;  // every base row integral
     float x_max_qerr, x_max_norm, x_max_norm2, x_max_qnorm, gamma_k, gamma_d;
     Neighbor* out;
     std::uint64_t out_stride;

@@ -32,7 +32,7 @@ int rerank_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 
 
 void prepare(const float* src, std::uint32_t ld_src, std::uint64_t rows, std::uint64_t dim,
              std::uint32_t kp, int prec, void* dst, float* n2, float* sc, float* qe, float* qn,
-             hipStream_t s) {
+             std::uint32_t* integral, hipStream_t s) {
     if (rows == 0) return;
     dev::PrepArgs p{};
     p.src = src;
@@ -46,6 +46,7 @@ void prepare(const float* src, std::uint32_t ld_src, std::uint64_t rows, std::ui
     p.scale = sc;
     p.qerr = qe;
     p.qnorm = qn;
+    p.integral = integral;
     check(dev::launch_bf_prepare(p, s), "bf_prepare launch");
 }
 
@@ -60,18 +61,24 @@ BfOperands& base_operands(vdx_gpu_bf& b, int prec, hipStream_t s) {
     op.scale.resize(op.rows * 4);
     op.qerr.resize(op.rows * 4);
     op.qnorm.resize(op.rows * 4);
+    op.integral.resize(op.rows * 4);
     check(hipMemsetAsync(op.xq.as<void>(), 0, op.xq.bytes(), s), "hipMemsetAsync");
-    for (DeviceBuffer* d : {&op.norm2, &op.scale, &op.qerr, &op.qnorm}) {
+    for (DeviceBuffer* d : {&op.norm2, &op.scale, &op.qerr, &op.qnorm, &op.integral}) {
         check(hipMemsetAsync(d->as<void>(), 0, d->bytes(), s), "hipMemsetAsync");
     }
     prepare(b.vectors.as<float>(), b.split.ld, b.n, b.dim, op.kp, prec, op.xq.as<void>(),
-            op.norm2.as<float>(), op.scale.as<float>(), op.qerr.as<float>(), op.qnorm.as<float>(), s);
+            op.norm2.as<float>(), op.scale.as<float>(), op.qerr.as<float>(), op.qnorm.as<float>(),
+            op.integral.as<std::uint32_t>(), s);
     std::vector<float> n2(b.n), qe(b.n), qn(b.n);
+    std::vector<std::uint32_t> integral(b.n);
+    check(hipMemcpyAsync(integral.data(), op.integral.as<void>(), b.n * 4, hipMemcpyDeviceToHost, s), "copy");
     check(hipMemcpyAsync(n2.data(), op.norm2.as<void>(), b.n * 4, hipMemcpyDeviceToHost, s), "copy");
     check(hipMemcpyAsync(qe.data(), op.qerr.as<void>(), b.n * 4, hipMemcpyDeviceToHost, s), "copy");
     check(hipMemcpyAsync(qn.data(), op.qnorm.as<void>(), b.n * 4, hipMemcpyDeviceToHost, s), "copy");
     check(hipStreamSynchronize(s), "bf prepare");
+    op.all_integral = true;
     for (std::uint64_t i = 0; i < b.n; ++i) {
+        op.all_integral = op.all_integral && integral[i] != 0;
         op.max_norm2 = std::max(op.max_norm2, n2[i]);
         op.max_qerr = std::max(op.max_qerr, qe[i]);
         op.max_qnorm = std::max(op.max_qnorm, qn[i]);
@@ -155,15 +162,17 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         DeviceBuffer& strict = ws.extra[13];
         DeviceBuffer& fail = ws.extra[14];
         DeviceBuffer& errbuf = ws.extra[15];
+        DeviceBuffer& qint = ws.aux2;
         qq.reserve(cq_pad * kp * operand_bytes(prec));
-        for (DeviceBuffer* d : {&qn2, &qsc, &qqe, &qqn, &tau, &cnt, &list_count, &left, &strict, &fail}) {
+        for (DeviceBuffer* d : {&qn2, &qsc, &qqe, &qqn, &tau, &cnt, &list_count, &left, &strict, &fail, &qint}) {
             d->reserve(cq_pad * 4);
         }
         errbuf.reserve(256);
         check(hipMemsetAsync(qq.as<void>(), 0, cq_pad * kp * operand_bytes(prec), s), "hipMemsetAsync");
         check(hipMemsetAsync(errbuf.as<void>(), 0, 4, s), "hipMemsetAsync");
         prepare(cq_src, static_cast<std::uint32_t>(b.dim), cq, b.dim, kp, prec, qq.as<void>(),
-                qn2.as<float>(), qsc.as<float>(), qqe.as<float>(), qqn.as<float>(), s);
+                qn2.as<float>(), qsc.as<float>(), qqe.as<float>(), qqn.as<float>(),
+                qint.as<std::uint32_t>(), s);
 
         dev::GemmArgs g{};
         g.xq = op.xq.as<void>();
@@ -243,6 +252,8 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         r.q_norm2 = qn2.as<float>();
         r.q_qerr = qqe.as<float>();
         r.q_qnorm = qqn.as<float>();
+        r.q_integral = qint.as<std::uint32_t>();
+        r.x_integral = op.all_integral ? 1u : 0u;
         r.x_max_qerr = op.max_qerr;
         r.x_max_norm = op.max_norm;
         r.x_max_norm2 = op.max_norm2;
