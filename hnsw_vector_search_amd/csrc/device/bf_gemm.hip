@@ -116,120 +116,196 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
     }
 }
 
-// The GEMM. A workgroup of 4 waves takes 128 queries (32 per wave, one MFMA row tile each,
-// its operand fragments held in registers) and one slab of base rows, 64 rows (two MFMA column
-// tiles) per step; the 4 waves read the same base rows, which the L1/L2 serve.
+// The GEMM. A workgroup of 4 waves takes 128 queries (32 per wave, one MFMA column tile held
+// in registers for the whole launch) and one slab of base rows. Base tiles of NT rows (64; 32
+// for FP32) are the MFMA row operand and go through LDS, double buffered: the 256 threads issue
+// the 16-byte global loads of tile t+1 before the MFMAs of tile t and store them to the other
+// buffer afterwards, so one barrier per tile suffices and load latency hides behind compute.
+// LDS rows are padded by 16 bytes so the 32 lanes of a fragment read hit distinct banks; the
+// tile's |x|^2 and scales sit next to it in LDS.
 //
-// MFMA 32x32 output layout: lane l holds column n = l % 32 and rows
-// m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) of its 16 accumulators i. Operand fragments: lane l
-// holds row/column l % 32 and the K slice [kstep * K + (l / 32) * K / 2, ... + K / 2).
+// MFMA 32x32 output layout: lane l holds column n = l % 32 (its query) and rows
+// m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) (base rows) of its 16 accumulators i. Operand
+// fragments: lane l holds row/column l % 32 and the K slice
+// [kstep * K + (l / 32) * K / 2, ... + K / 2).
 template <int PREC>
 __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
+    constexpr int EB = PREC == kPrecF32 ? 4 : (PREC == kPrecBF16 ? 2 : 1);  // bytes per element
+    constexpr int NT = PREC == kPrecF32 ? 32 : 64;                           // rows per tile
+    constexpr int NH = NT / 32;                                              // MFMA row tiles
+    constexpr int KS = PREC == kPrecF32 ? 32 : 8;  // K steps (F32: float2 pairs of K=2 steps)
+    extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int col = lane & 31;
     const int half = lane >> 5;
-    const std::uint32_t q0 = blockIdx.x * 128 + wave * 32;  // this wave's first query
+    const std::uint32_t q = blockIdx.x * 128 + wave * 32 + col;  // this lane's query
     const std::uint32_t kp = a.kp;
+    const std::uint32_t row_bytes = kp * EB;
+    const std::uint32_t rs = row_bytes + 16;  // padded LDS row stride
+    // buffer b: rows at lds + b * NT * rs, |x|^2 at vals + b * NT, scales at vals + (2 + b) * NT
+    float* const vals = reinterpret_cast<float*>(lds + 2 * NT * rs);
 
-    // A fragments (queries), per K step.
-    constexpr int KS = PREC == kPrecF32 ? 32 : 8;  // float2 steps pairs (F32) or 16-wide steps
     typedef typename std::conditional<PREC == kPrecF32, float2,
                                       typename std::conditional<PREC == kPrecBF16, v8bf, long>::type>::type Frag;
-    Frag af[KS];
+    Frag bq[KS];
     const std::uint32_t ks = a.ks;
+    const unsigned char* qbase = static_cast<const unsigned char*>(a.qq) + static_cast<std::size_t>(q) * row_bytes;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         if (static_cast<std::uint32_t>(s) < ks) {
-            const std::size_t qrow = static_cast<std::size_t>(q0 + col) * kp;
             if constexpr (PREC == kPrecF32) {
-                af[s] = *reinterpret_cast<const float2*>(static_cast<const float*>(a.qq) + qrow + 4 * s + 2 * half);
+                bq[s] = *reinterpret_cast<const float2*>(qbase + (4 * s + 2 * half) * 4);
             } else if constexpr (PREC == kPrecBF16) {
-                af[s] = *reinterpret_cast<const v8bf*>(static_cast<const __bf16*>(a.qq) + qrow + 16 * s + 8 * half);
+                bq[s] = *reinterpret_cast<const v8bf*>(qbase + (16 * s + 8 * half) * 2);
             } else {
-                af[s] = *reinterpret_cast<const long*>(static_cast<const unsigned char*>(a.qq) + qrow + 16 * s + 8 * half);
+                bq[s] = *reinterpret_cast<const long*>(qbase + 16 * s + 8 * half);
             }
         }
     }
-    // Per-lane query constants for the 16 rows this lane's accumulators cover.
-    float qn[16], qs[16], tau[16];
-    std::uint32_t qidx[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const std::uint32_t q = q0 + 8 * (i / 4) + 4 * half + (i % 4);
-        qidx[i] = q;
-        const bool ok = q < a.nq;
-        qn[i] = ok ? a.qn[q] : 0.f;
-        qs[i] = ok && a.qs ? a.qs[q] : 1.f;
-        tau[i] = ok && a.tau ? a.tau[q] : __int_as_float(0x7f800000);
-    }
+    const bool qok = q < a.nq;
+    const float qn = qok ? a.qn[q] : 0.f;
+    const float qs = qok && a.qs ? a.qs[q] : 1.f;
+    // Padding lanes (q >= nq) get -inf, so no distance passes their filter: their candidate
+    // rows would lie past the end of the buffer.
+    const float tau = !qok ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[q] : __int_as_float(0x7f800000));
 
-    const std::uint32_t tiles = (a.rows + 63) / 64;
+    const std::uint32_t tiles = (a.rows + NT - 1) / NT;
     const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
     const std::uint32_t t0 = blockIdx.y * per_slab;
     const std::uint32_t t1 = min(tiles, t0 + per_slab);
+    if (t0 >= t1) {
+        return;
+    }
+    // Cooperative tile loads: 16-byte pieces, piece p is row p / chunks, chunk p % chunks.
+    const std::uint32_t chunks = row_bytes / 16;
+    const std::uint32_t pieces = NT * chunks;
+    // NT * row_bytes <= 16 KB: at most 4 pieces per thread
+    uint4 s0 = make_uint4(0, 0, 0, 0), s1 = s0, s2 = s0, s3 = s0;
+    float stage_xn = 0.f, stage_xs = 1.f;
+#define VDX_FETCH_PIECE(I, DST)                                                                   \
+    {                                                                                             \
+        const std::uint32_t p = threadIdx.x + 256u * (I);                                         \
+        if (p < pieces) {                                                                         \
+            const std::uint32_t j = t_ * NT + p / chunks;                                         \
+            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
+            DST = *reinterpret_cast<const uint4*>(static_cast<const unsigned char*>(a.xq) +       \
+                                                  static_cast<std::size_t>(xr) * row_bytes +      \
+                                                  (p % chunks) * 16);                             \
+        }                                                                                         \
+    }
+#define VDX_STORE_PIECE(I, SRC)                                                                   \
+    {                                                                                             \
+        const std::uint32_t p = threadIdx.x + 256u * (I);                                         \
+        if (p < pieces) {                                                                         \
+            *reinterpret_cast<uint4*>(lds + b_ * NT * rs + (p / chunks) * rs + (p % chunks) * 16) = SRC; \
+        }                                                                                         \
+    }
+    const auto fetch = [&](std::uint32_t t_) {
+        VDX_FETCH_PIECE(0, s0)
+        VDX_FETCH_PIECE(1, s1)
+        VDX_FETCH_PIECE(2, s2)
+        VDX_FETCH_PIECE(3, s3)
+        if (threadIdx.x < NT) {
+            const std::uint32_t j = t_ * NT + threadIdx.x;
+            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+            stage_xn = a.xn[xr];
+            stage_xs = a.xs ? a.xs[xr] : 1.f;
+        }
+    };
+    const auto store = [&](int b_) {
+        VDX_STORE_PIECE(0, s0)
+        VDX_STORE_PIECE(1, s1)
+        VDX_STORE_PIECE(2, s2)
+        VDX_STORE_PIECE(3, s3)
+        if (threadIdx.x < NT) {
+            vals[b_ * NT + threadIdx.x] = stage_xn;
+            vals[(2 + b_) * NT + threadIdx.x] = stage_xs;
+        }
+    };
+    fetch(t0);
+    store(0);
+    __syncthreads();
     for (std::uint32_t t = t0; t < t1; ++t) {
-        v16f acc[2];
-        acc[0] = v16f{};
-        acc[1] = v16f{};
-        std::uint32_t xrow[2];
+        const int cb = static_cast<int>((t - t0) & 1u);
+        const bool more = t + 1 < t1;
+        if (more) {
+            fetch(t + 1);
+        }
+        v16f acc[NH];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const std::uint32_t j = t * 64 + h * 32 + col;  // index among the covered rows
-            // base row (strided in the sample); past the last covered row, load the last one
-            // again (masked in the epilogue) so every load stays inside the operand buffer
-            xrow[h] = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+        for (int h = 0; h < NH; ++h) {
+            acc[h] = v16f{};
         }
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             if (static_cast<std::uint32_t>(s) < ks) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const std::size_t xr = static_cast<std::size_t>(xrow[h]) * kp;
+                for (int h = 0; h < NH; ++h) {
+                    const unsigned char* rowp = lds + cb * NT * rs + (h * 32 + col) * rs;
                     if constexpr (PREC == kPrecF32) {
-                        const float2 b = *reinterpret_cast<const float2*>(static_cast<const float*>(a.xq) + xr + 4 * s + 2 * half);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[s].x, b.x, acc[h], 0, 0, 0);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[s].y, b.y, acc[h], 0, 0, 0);
+                        const float2 x = *reinterpret_cast<const float2*>(rowp + (4 * s + 2 * half) * 4);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, bq[s].x, acc[h], 0, 0, 0);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, bq[s].y, acc[h], 0, 0, 0);
                     } else if constexpr (PREC == kPrecBF16) {
-                        const v8bf b = *reinterpret_cast<const v8bf*>(static_cast<const __bf16*>(a.xq) + xr + 16 * s + 8 * half);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[s], b, acc[h], 0, 0, 0);
+                        const v8bf x = *reinterpret_cast<const v8bf*>(rowp + (16 * s + 8 * half) * 2);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
                     } else {
-                        const long b = *reinterpret_cast<const long*>(static_cast<const unsigned char*>(a.xq) + xr + 16 * s + 8 * half);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(af[s], b, acc[h], 0, 0, 0);
+                        const long x = *reinterpret_cast<const long*>(rowp + 16 * s + 8 * half);
+                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(x, bq[s], acc[h], 0, 0, 0);
                     }
                 }
             }
         }
-        // Epilogue: distances, then the dense store (sample pass) or the threshold filter.
+        // Epilogue: rows 8 * g + 4 * half + (0..3) of each row tile, four at a time.
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const std::uint32_t j = t * 64 + h * 32 + col;
-            const bool row_ok = j < a.rows;
-            const std::uint32_t r = xrow[h];
-            const float xn = row_ok ? a.xn[r] : 0.f;
-            const float xs = row_ok && a.xs ? a.xs[r] : 1.f;
+        for (int h = 0; h < NH; ++h) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                float dot = acc[h][i];
+            for (int g = 0; g < 4; ++g) {
+                const int r0 = h * 32 + 8 * g + 4 * half;  // first of four consecutive tile rows
+                const float4 xn4 = *reinterpret_cast<const float4*>(vals + cb * NT + r0);
+                float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);
                 if constexpr (PREC == kPrecFP8) {
-                    dot = dot * qs[i] * xs;
+                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + cb) * NT + r0);
                 }
-                const float d = a.l2 ? (qn[i] + xn) - 2.0f * dot : 1.0f - dot;
-                const std::uint32_t q = qidx[i];
-                if (row_ok && q < a.nq) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w));
+                    const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w));
+                    float dot = acc[h][4 * g + e];
+                    if constexpr (PREC == kPrecFP8) {
+                        dot = dot * qs * xsv;
+                    }
+                    const std::uint32_t j = t * NT + r0 + e;
                     if (a.mode == 0) {
-                        a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;
-                    } else if (d <= tau[i]) {
-                        const std::uint32_t slot = atomicAdd(a.cnt + q, 1u);
-                        if (slot < a.cap) {
-                            a.buf[static_cast<std::size_t>(q) * a.cap + slot] = make_key(d, r);
+                        if (qok && j < a.rows) {
+                            const float d = a.l2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;
+                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;
+                        }
+                    } else {
+                        // The filter tests exactly the distance it stores (2 dot is exact, so the
+                        // fma equals (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM
+                        // distance exceeds tau, which the certification relies on.
+                        const float d = a.l2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;
+                        if (qok && d <= tau && j < a.rows) {
+                            const std::uint32_t slot = atomicAdd(a.cnt + q, 1u);
+                            if (slot < a.cap) {
+                                a.buf[static_cast<std::size_t>(q) * a.cap + slot] = make_key(d, j * a.row_stride);
+                            }
                         }
                     }
                 }
             }
         }
+        if (more) {
+            store(cb ^ 1);
+        }
+        __syncthreads();
     }
 }
+
+#undef VDX_FETCH_PIECE
+#undef VDX_STORE_PIECE
 
 // Per query (one workgroup): the `rank`-th smallest (1-based) of `count` u32 sortable
 // distances (dense sample row), by an MSB-first radix select, 8 bits per pass.
@@ -446,12 +522,16 @@ __global__ __launch_bounds__(256) void bf_rerank_kernel(RerankArgs a) {
         a.out_count[q] = got;
         atomicMax(reinterpret_cast<unsigned int*>(a.max_err), __float_as_uint(max_err));
         bool ok = got == want;
-        // Exact GEMM: integral operands, lossless quantization, and every partial sum an
-        // integer below 2^24: partial sums of products are bounded by |q||x| (Cauchy-Schwarz),
-        // and |q|^2 + |x|^2 and |q|^2 + |x|^2 - 2 q.x by (|q| + |x|)^2. Then every GEMM distance
-        // equals the exact one and the S smallest keys contain the k smallest: certified.
-        const float reach = a.q_qnorm[q] + a.x_max_qnorm;
-        const bool exact_gemm = a.x_integral && a.q_integral[q] && reach * reach < 16777216.0f;
+        // Exact GEMM: integral operands and lossless quantization; every partial sum of
+        // products is an integer of magnitude <= |q||x| < 2^24 (Cauchy-Schwarz), |q|^2 + |x|^2 <
+        // 2^24, and the distance itself is then rounded once: exact when below 2^24, and >= 2^24
+        // (so beyond a k-th distance below 2^24) otherwise. The S smallest GEMM keys then
+        // contain the k smallest exact ones: certified without a margin.
+        const float two24 = 16777216.0f;
+        const bool exact_gemm = a.x_integral && a.q_integral[q] && got == want && want > 0 &&
+                                a.q_qnorm[q] * a.x_max_qnorm < two24 - 2.0f &&
+                                (!a.l2 || a.q_norm2[q] + a.x_max_norm2 < two24) &&
+                                fabsf(key_distance(keys[want - 1])) < two24;
         if (ok && !exact_gemm) {
             const float left = a.left[q];
             if (left != __int_as_float(0x7f800000)) {
@@ -526,15 +606,22 @@ hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
     return hipGetLastError();
 }
 
+std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp) {
+    const std::size_t eb = prec == kPrecF32 ? 4 : (prec == kPrecBF16 ? 2 : 1);
+    const std::size_t nt = prec == kPrecF32 ? 32 : 64;
+    return 2 * nt * (kp * eb + 16) + 4 * nt * sizeof(float);
+}
+
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec,
                           hipStream_t stream) {
     const dim3 grid(qtiles, slabs);
+    const std::size_t lds = bf_gemm_lds_bytes(prec, args.kp);
     if (prec == kPrecF32) {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecF32>, grid, dim3(256), 0, stream, args);
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecF32>, grid, dim3(256), lds, stream, args);
     } else if (prec == kPrecBF16) {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecBF16>, grid, dim3(256), 0, stream, args);
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecBF16>, grid, dim3(256), lds, stream, args);
     } else {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecFP8>, grid, dim3(256), 0, stream, args);
+        hipLaunchKernelGGL(bf_gemm_kernel<kPrecFP8>, grid, dim3(256), lds, stream, args);
     }
     return hipGetLastError();
 }

@@ -134,9 +134,11 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
     if (sample) {
         m = static_cast<std::uint32_t>(std::min<std::uint64_t>(8192, std::max<std::uint64_t>(512, b.n / 8)));
         stride = static_cast<std::uint32_t>(b.n / m);
-        rank = static_cast<std::uint32_t>(std::min<double>(m, std::ceil(2.0 * S * m / b.n) + 4));
+        // rank >= 16 keeps the survivor count's spread small (Gamma(rank): cv = 1/sqrt(rank));
+        // the buffer holds 4x the expected survivors, so overflows (redone exactly) are rare.
+        rank = static_cast<std::uint32_t>(std::min<double>(m, std::max(16.0, std::ceil(2.0 * S * m / b.n))));
         const double expect = static_cast<double>(rank) * b.n / m;
-        cap = static_cast<std::uint32_t>(std::clamp<double>(std::ceil(3.0 * expect / 256) * 256, 1024, 16384));
+        cap = static_cast<std::uint32_t>(std::clamp<double>(std::ceil(4.0 * expect / 256) * 256, 1024, 16384));
         cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(cap, op.rows));
     }
 
