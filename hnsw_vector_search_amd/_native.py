@@ -30,7 +30,8 @@ VDX_OK, VDX_INVALID_ARGUMENT, VDX_RUNTIME_ERROR = 0, 1, 2
 VDX_LENGTH_ERROR, VDX_OUT_OF_MEMORY, VDX_DEVICE_ERROR = 3, 4, 5
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "exact": 3}
-STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates")
+STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates", "early_stops")
+MLP_PARAMS = 5345
 
 
 class GraphView(ctypes.Structure):
@@ -47,7 +48,7 @@ class GraphReport(ctypes.Structure):
 
 class SearchOpts(ctypes.Structure):
     _fields_ = [("visited_log2", c_u32), ("waves_per_group", c_u32), ("termination", c_u32),
-                ("reserved", c_u32), ("mlp", c_ptr), ("mlp_threshold", c_float),
+                ("min_steps", c_u32), ("mlp", c_ptr), ("mlp_threshold", c_float),
                 ("reserved_f", c_float), ("stats", c_ptr)]
 
 
@@ -90,6 +91,13 @@ _SIGS = {
     "vdx_gpu_hnsw_search_device": (c_int, [c_ptr, c_ptr, c_u64, c_ptr, c_u32, c_ptr, c_u32,
                                            c_u32, c_ptr, c_u64, c_ptr,
                                            ctypes.POINTER(SearchOpts), c_ptr]),
+    "vdx_gpu_mlp_create": (c_int, [c_ptr, c_ptr, c_u64, c_ptr]),
+    "vdx_gpu_mlp_params": (c_int, [c_ptr, c_ptr, c_u64]),
+    "vdx_gpu_mlp_free": (None, [c_ptr]),
+    "vdx_gpu_mlp_train": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u32, c_u32, c_float, c_u64, c_ptr]),
+    "vdx_gpu_mlp_infer": (c_int, [c_ptr, c_ptr, c_u64, c_ptr]),
+    "vdx_gpu_hnsw_trace_device": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr, c_u32, c_ptr,
+                                          c_ptr, c_ptr, c_ptr]),
     "vdx_gpu_bf_upload": (c_int, [c_ptr, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
     "vdx_gpu_bf_free": (None, [c_ptr]),
     "vdx_gpu_bf_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u32, c_u32, c_ptr, c_u64,

@@ -41,19 +41,36 @@ struct BeamArgs {
     std::uint32_t ef_cap;  // R capacity per wave (>= every max(ef, k)), a multiple of 64
     std::uint32_t visited_log2;
     std::uint32_t lds_per_wave;
-    unsigned long long* stats;  // 4 counters, or null
+    unsigned long long* stats;  // 5 counters, or null
+    // trace mode: per step features and top-10 hits against `truth` (10 ids per query)
+    const std::uint32_t* truth;
+    float* trace_feat;           // [nq][trace_cap][16]
+    std::uint8_t* trace_hits;    // [nq][trace_cap]
+    std::uint32_t* trace_steps;  // [nq] expansions taken
+    std::uint32_t* trace_final;  // [nq] top-10 hits of the result
+    std::uint32_t trace_cap;
+    std::uint32_t M2;            // 2M, for feature scaling
+    // learned mode: the stop model (device copy, staged in LDS) and its decision threshold
+    const void* mlp;
+    float stop_logit;
+    std::uint32_t min_steps;
 };
+
+enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
 
 enum class BeamShape { D128, D100, Generic };
 
 // Bytes of LDS one wave uses for a given R capacity and visited-cache size.
 inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {
-    return 8u * static_cast<std::size_t>(ef_cap) + 8u * 64 + 4u * 64 + 4u * 64 +
+    return 8u * static_cast<std::size_t>(ef_cap) + 8u * 64 + 4u * 64 + 4u * 64 + 4u * 16 +
            (std::size_t{4} << visited_log2);
 }
 
+// LDS a workgroup needs besides its waves' regions (the stop model in learned mode).
+std::size_t beam_lds_shared(int mode);
+
 hipError_t launch_hnsw_beam(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
-                            std::size_t lds_bytes, hipStream_t stream, BeamShape shape);
-const void* hnsw_beam_kernel_ptr(BeamShape shape);
+                            std::size_t lds_bytes, hipStream_t stream, BeamShape shape, int mode);
+const void* hnsw_beam_kernel_ptr(BeamShape shape, int mode);
 
 }  // namespace vdx::dev

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -73,7 +74,7 @@ struct BeamPlan {
 };
 
 BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t nq,
-                   const vdx_search_opts* opts) {
+                   const vdx_search_opts* opts, int mode) {
     BeamPlan p{};
     p.ef_cap = (max_efp + 63u) / 64u * 64u;
     p.visited_log2 = opts && opts->visited_log2 ? opts->visited_log2 : default_visited_log2(p.ef_cap);
@@ -82,14 +83,15 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
     p.lds_per_wave = vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
     const std::size_t limit = h.ctx->lds_per_block;
-    while (p.waves > 1 && p.lds_per_wave * p.waves > limit) {
+    const std::size_t shared = vdx::dev::beam_lds_shared(mode);
+    while (p.waves > 1 && p.lds_per_wave * p.waves + shared > limit) {
         --p.waves;
     }
-    p.lds_bytes = p.lds_per_wave * p.waves;
+    p.lds_bytes = p.lds_per_wave * p.waves + shared;
     require(p.lds_bytes <= limit, "ef too large for the GPU beam's LDS");
     int per_cu = 0;
     check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &per_cu, vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim)), p.waves * 64,
+              &per_cu, vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim), mode), p.waves * 64,
               p.lds_bytes),
           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     per_cu = std::max(per_cu, 1);
@@ -104,9 +106,21 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t k, const std::uint32_t* d_efs, std::uint32_t ef,
                   std::uint32_t max_efp, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
-                  const vdx_search_opts* opts, hipStream_t stream) {
+                  const vdx_search_opts* opts, hipStream_t stream,
+                  const vdx::dev::BeamArgs* trace = nullptr) {
     require(nq <= 0xFFFFFFFFull, "too many queries in one call");
-    const BeamPlan plan = plan_beam(h, max_efp, nq, opts);
+    int mode = vdx::dev::kBeamPlain;
+    if (trace != nullptr) {
+        mode = vdx::dev::kBeamTrace;
+    } else if (opts != nullptr && opts->termination == 1) {
+        mode = vdx::dev::kBeamLearned;
+        require(opts->mlp != nullptr, "learned termination needs a model (opts.mlp)");
+        require(opts->mlp_threshold > 0.f && opts->mlp_threshold < 1.f,
+                "learned termination needs a threshold in (0, 1)");
+    } else if (opts != nullptr && opts->termination > 1) {
+        throw std::invalid_argument("unknown termination mode");
+    }
+    const BeamPlan plan = plan_beam(h, max_efp, nq, opts, mode);
     vdx::dev::BeamArgs a{};
     a.vectors = h.vectors.as<float>();
     a.layer0 = h.layer0.as<std::uint32_t>();
@@ -140,9 +154,25 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     a.visited_log2 = plan.visited_log2;
     a.lds_per_wave = static_cast<std::uint32_t>(plan.lds_per_wave);
     a.stats = d_stats;
+    a.M2 = static_cast<std::uint32_t>(2 * h.M);
+    if (trace != nullptr) {
+        a.truth = trace->truth;
+        a.trace_feat = trace->trace_feat;
+        a.trace_hits = trace->trace_hits;
+        a.trace_steps = trace->trace_steps;
+        a.trace_final = trace->trace_final;
+        a.trace_cap = trace->trace_cap;
+    }
+    if (mode == vdx::dev::kBeamLearned) {
+        const auto* model = static_cast<const vdx_gpu_mlp*>(opts->mlp);
+        a.mlp = model->weights.as<void>();
+        const float t = opts->mlp_threshold;
+        a.stop_logit = std::log(t / (1.0f - t));
+        a.min_steps = opts->min_steps;
+    }
     check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
     check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                     beam_shape(h.dim)),
+                                     beam_shape(h.dim), mode),
           "hnsw_beam launch");
 }
 
@@ -409,7 +439,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         unsigned long long* d_stats = nullptr;
         if (opts && opts->stats) {
             d_stats = reinterpret_cast<unsigned long long*>(ws->counters.as<char>() + 16);
-            check(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+            check(hipMemsetAsync(d_stats, 0, 5 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
         }
         enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
@@ -420,7 +450,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
               "hipMemcpyAsync(out)");
         check(hipMemcpyAsync(out_count, ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(count)");
-        unsigned long long host_stats[4] = {0, 0, 0, 0};
+        unsigned long long host_stats[5] = {0, 0, 0, 0, 0};
         if (d_stats) {
             check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(stats)");
@@ -431,7 +461,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             std::memcpy(out + q * out_stride, staged + q * kmax, out_count[q] * sizeof(vdx_neighbor));
         }
         if (d_stats) {
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < 5; ++i) {
                 opts->stats[i] += host_stats[i];
             }
         }
@@ -466,20 +496,52 @@ int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t
         }
         const bool want_stats = opts && opts->stats;
         if (want_stats) {
-            check(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+            check(hipMemsetAsync(d_stats, 0, 5 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
         }
         enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
                      reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
                      want_stats ? d_stats : nullptr, opts, s);
         if (want_stats) {
-            unsigned long long host_stats[4];
+            unsigned long long host_stats[5];
             check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(stats)");
             check(hipStreamSynchronize(s), "hnsw search stats");
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < 5; ++i) {
                 opts->stats[i] += host_stats[i];
             }
         }
+    });
+}
+
+int vdx_gpu_hnsw_trace_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t nq, uint32_t k,
+                              uint32_t ef, const uint32_t* d_truth10, uint32_t max_steps,
+                              float* d_features, uint8_t* d_hits, uint32_t* d_steps,
+                              uint32_t* d_final) {
+    return guarded([&] {
+        require(h != nullptr, "vdx_gpu_hnsw_trace_device: index is null");
+        require(k >= 1 && max_steps >= 1, "vdx_gpu_hnsw_trace_device: k and max_steps must be positive");
+        require(d_truth10 && d_features && d_hits && d_steps && d_final,
+                "vdx_gpu_hnsw_trace_device: null buffer");
+        require(h->n > 0, "vdx_gpu_hnsw_trace_device: empty index");
+        if (nq == 0) return;
+        const std::uint32_t efmax = std::max(ef, k);
+        require(efmax <= kMaxBeamEf, "vdx_gpu_hnsw_trace_device: ef above the GPU path's limit");
+        DeviceGuard guard(h->ctx->device);
+        WorkspaceLease ws(*h->ctx);
+        hipStream_t s = ws->stream;
+        ws->out.reserve(nq * k * sizeof(vdx_neighbor));
+        ws->count.reserve(nq * 4);
+        ws->counters.reserve(64);
+        vdx::dev::BeamArgs t{};
+        t.truth = d_truth10;
+        t.trace_feat = d_features;
+        t.trace_hits = d_hits;
+        t.trace_steps = d_steps;
+        t.trace_final = d_final;
+        t.trace_cap = max_steps;
+        enqueue_beam(*h, d_queries, nq, nullptr, k, nullptr, ef, efmax, ws->out.as<vdx::dev::Neighbor>(), k,
+                     ws->count.as<std::uint32_t>(), ws->counters.as<std::uint32_t>(), nullptr, nullptr, s, &t);
+        check(hipStreamSynchronize(s), "hnsw trace");
     });
 }
 

@@ -43,3 +43,9 @@ struct vdx_gpu_bf {
     std::mutex prep_mutex;
     vdx::gpu::BfOperands prep[3];
 };
+
+struct vdx_gpu_mlp {
+    vdx_gpu_ctx* ctx = nullptr;
+    vdx::gpu::DeviceBuffer master, m, v, grad, weights, loss, perm;
+    std::uint64_t step = 0;
+};

@@ -27,6 +27,7 @@
 #include "beam_args.hpp"
 #include "device_common.hpp"
 #include "gather_distances.hpp"
+#include "mlp_stop.hpp"
 
 namespace vdx::dev {
 
@@ -59,7 +60,10 @@ __device__ __forceinline__ std::uint32_t uniform(std::uint32_t v) {
 // (every entry below hint is expanded). F is LDS scratch for 64 keys.
 __device__ __forceinline__ void merge(std::uint64_t* R, std::uint32_t& size, std::uint32_t efp,
                                       std::uint32_t& hint, std::uint64_t f, std::uint32_t cnt,
-                                      std::uint64_t* F, unsigned long long& dups) {
+                                      std::uint64_t* F, unsigned long long& dups,
+                                      std::uint32_t& inserted, std::uint32_t& first_pos) {
+    inserted = 0;
+    first_pos = 0xFFFFFFFFu;
     const std::uint32_t lane = static_cast<std::uint32_t>(lane_id());
     bool keep = lane < cnt;
     if (keep && size == efp) {
@@ -107,6 +111,8 @@ __device__ __forceinline__ void merge(std::uint64_t* R, std::uint32_t& size, std
     }
     const std::uint32_t lo_move = wave_min_u32(keep ? lb : 0xFFFFFFFFu);
     const std::uint32_t min_pos = wave_min_u32(keep ? pos : 0xFFFFFFFFu);
+    inserted = nins;
+    first_pos = min_pos;
     wave_lds_fence();
     // Shift the members of R at or after lo_move right by the number of inserted keys below
     // them, top chunk first, so no member is overwritten before it has been read.
@@ -144,7 +150,7 @@ __device__ __forceinline__ void merge(std::uint64_t* R, std::uint32_t& size, std
 
 }  // namespace
 
-template <int NB, int N8, int TAIL, int R>
+template <int NB, int N8, int TAIL, int R, int MODE>
 __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -155,7 +161,19 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
     std::uint64_t* F = reinterpret_cast<std::uint64_t*>(base + 8u * a.ef_cap);
     std::uint32_t* fid = reinterpret_cast<std::uint32_t*>(F + 64);
     float* fd = reinterpret_cast<float*>(fid + 64);
-    std::uint32_t* vis = reinterpret_cast<std::uint32_t*>(fd + 64);
+    float* feat = fd + 64;
+    std::uint32_t* vis = reinterpret_cast<std::uint32_t*>(feat + 16);
+    MlpWeights* model = nullptr;
+    if constexpr (MODE == kBeamLearned) {
+        // The stop model, staged once per workgroup after the waves' regions.
+        model = reinterpret_cast<MlpWeights*>(lds + static_cast<std::size_t>(blockDim.x >> 6) * a.lds_per_wave);
+        const std::uint32_t* src_words = static_cast<const std::uint32_t*>(a.mlp);
+        std::uint32_t* dst_words = reinterpret_cast<std::uint32_t*>(model);
+        for (std::uint32_t i = threadIdx.x; i < sizeof(MlpWeights) / 4; i += blockDim.x) {
+            dst_words[i] = src_words[i];
+        }
+        __syncthreads();
+    }
     const std::uint32_t vslots = 1u << a.visited_log2;
     const std::uint32_t vshift = 32u - a.visited_log2;
     const int nb = static_cast<int>(a.nb);
@@ -164,7 +182,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
     const int first_tail = static_cast<int>(a.first_tail);
     const RowSource src{a.vectors, a.ld, a.l2};
 
-    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0;
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0;
 
     for (;;) {
         std::uint32_t q = 0;
@@ -268,6 +286,15 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         wave_lds_fence();
         std::uint32_t size = 1;
         std::uint32_t hint = 0;
+        // search-state features (trace and learned modes)
+        const float d_scale = fmaxf(fabsf(cur_d), 1e-6f);
+        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0;
+        std::uint32_t truth_id = 0xFFFFFFFFu;
+        if constexpr (MODE == kBeamTrace) {
+            if (lane < 10) {
+                truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
+            }
+        }
         for (;;) {
             // The closest unexpanded member of R.
             std::int32_t p = -1;
@@ -321,7 +348,86 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
             if (static_cast<std::uint32_t>(lane) < nf) {
                 f = make_key(fd[lane], fid[lane]);
             }
-            merge(Rl, size, efp, hint, f, nf, F, n_dup);
+            std::uint32_t inserted = 0, first_pos = 0;
+            merge(Rl, size, efp, hint, f, nf, F, n_dup, inserted, first_pos);
+            if constexpr (MODE != kBeamPlain) {
+                ++steps;
+                const std::uint32_t kk = min(k, size);
+                const bool improved = inserted > 0 && first_pos < kk;
+                improved_hist = ((improved_hist << 1) | (improved ? 1u : 0u)) & 0xFFu;
+                since_improve = improved ? 0u : since_improve + 1u;
+                // Lane i computes feature i (all inputs are wave-uniform).
+                if (lane < kFeatures) {
+                    const float d_best = key_distance(Rl[0]);
+                    const float d_k = key_distance(Rl[kk - 1]);
+                    const float d_worst = key_distance(Rl[size - 1]);
+                    const float d_cur = key_distance(kp);
+                    const float sk = fmaxf(fabsf(d_k), 1e-6f);
+                    float v;
+                    switch (lane) {
+                        case 0: v = d_best / d_scale; break;
+                        case 1: v = d_k / d_scale; break;
+                        case 2: v = d_cur / sk; break;
+                        case 3: v = d_best / sk; break;
+                        case 4: v = static_cast<float>(steps) / efp; break;
+                        case 5: v = __log2f(1.0f + steps) * 0.125f; break;
+                        case 6: v = static_cast<float>(size) / efp; break;
+                        case 7: v = __popc(improved_hist) * 0.125f; break;
+                        case 8: v = static_cast<float>(p) / efp; break;
+                        case 9: v = static_cast<float>(inserted) / a.M2; break;
+                        case 10: v = static_cast<float>(nf) / a.M2; break;
+                        case 11: v = min(since_improve, 64u) * 0.0625f; break;
+                        case 12: v = efp * (1.0f / 512.0f); break;
+                        case 13: v = k * 0.01f; break;
+                        case 14: v = d_worst / sk; break;
+                        default: v = a.l2 ? 1.0f : 0.0f; break;
+                    }
+                    feat[lane] = v;
+                }
+                wave_lds_fence();
+                if constexpr (MODE == kBeamTrace) {
+                    // top-10 hits of the current result against the ground truth
+                    const std::uint32_t kt = min(10u, size);
+                    const std::uint32_t rid = static_cast<std::uint32_t>(lane) < kt ? key_id(Rl[lane]) : 0xFFFFFFFEu;
+                    bool found = false;
+                    for (int j = 0; j < 10; ++j) {
+                        found = found || (static_cast<std::uint32_t>(__shfl(static_cast<int>(truth_id), j, kWave)) == rid);
+                    }
+                    const std::uint32_t hits = static_cast<std::uint32_t>(__popcll(ballot(found && static_cast<std::uint32_t>(lane) < kt)));
+                    if (steps <= a.trace_cap) {
+                        const std::size_t row = static_cast<std::size_t>(q) * a.trace_cap + (steps - 1);
+                        if (lane < kFeatures) {
+                            a.trace_feat[row * kFeatures + lane] = feat[lane];
+                        }
+                        if (lane == 0) {
+                            a.trace_hits[row] = static_cast<std::uint8_t>(hits);
+                        }
+                    }
+                    if (lane == 0) {
+                        a.trace_final[q] = hits;  // the last step's value is the result's
+                    }
+                }
+                if constexpr (MODE == kBeamLearned) {
+                    if (steps >= a.min_steps && size >= k) {
+                        float x[kFeatures];
+#pragma unroll
+                        for (int i = 0; i < kFeatures; ++i) {
+                            x[i] = mlp_input(*model, i, feat[i]);
+                        }
+                        float h1, h2;
+                        const float z = mlp_forward(*model, x, lane, h1, h2);
+                        if (z > a.stop_logit) {
+                            ++n_stop;
+                            break;  // stop: the model judges further expansions useless
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (MODE == kBeamTrace) {
+            if (lane == 0) {
+                a.trace_steps[q] = steps;
+            }
         }
 
         const std::uint32_t count = min(k, size);
@@ -339,38 +445,39 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         atomicAdd(a.stats + 1, n_dist);
         atomicAdd(a.stats + 2, n_up);
         atomicAdd(a.stats + 3, n_dup);
+        atomicAdd(a.stats + 4, n_stop);
     }
+}
+
+std::size_t beam_lds_shared(int mode) { return mode == kBeamLearned ? sizeof(MlpWeights) : 0; }
+
+namespace {
+
+template <int MODE>
+const void* kernel_for(BeamShape shape) {
+    switch (shape) {
+        case BeamShape::D128:
+            return reinterpret_cast<const void*>(&hnsw_beam_kernel<4, 0, 0, 4, MODE>);
+        case BeamShape::D100:
+            return reinterpret_cast<const void*>(&hnsw_beam_kernel<3, 0, 4, 4, MODE>);
+        default:
+            return reinterpret_cast<const void*>(&hnsw_beam_kernel<8, 3, 7, 1, MODE>);
+    }
+}
+
+}  // namespace
+
+const void* hnsw_beam_kernel_ptr(BeamShape shape, int mode) {
+    return mode == kBeamTrace ? kernel_for<kBeamTrace>(shape)
+                              : (mode == kBeamLearned ? kernel_for<kBeamLearned>(shape)
+                                                      : kernel_for<kBeamPlain>(shape));
 }
 
 hipError_t launch_hnsw_beam(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
-                            std::size_t lds_bytes, hipStream_t stream, BeamShape shape) {
-    const dim3 block(waves_per_group * kWave);
-    switch (shape) {
-        case BeamShape::D128:
-            hipLaunchKernelGGL((hnsw_beam_kernel<4, 0, 0, 4>), dim3(grid), block, lds_bytes, stream,
-                               args);
-            break;
-        case BeamShape::D100:
-            hipLaunchKernelGGL((hnsw_beam_kernel<3, 0, 4, 4>), dim3(grid), block, lds_bytes, stream,
-                               args);
-            break;
-        default:
-            hipLaunchKernelGGL((hnsw_beam_kernel<8, 3, 7, 1>), dim3(grid), block, lds_bytes, stream,
-                               args);
-            break;
-    }
-    return hipGetLastError();
-}
-
-const void* hnsw_beam_kernel_ptr(BeamShape shape) {
-    switch (shape) {
-        case BeamShape::D128:
-            return reinterpret_cast<const void*>(&hnsw_beam_kernel<4, 0, 0, 4>);
-        case BeamShape::D100:
-            return reinterpret_cast<const void*>(&hnsw_beam_kernel<3, 0, 4, 4>);
-        default:
-            return reinterpret_cast<const void*>(&hnsw_beam_kernel<8, 3, 7, 1>);
-    }
+                            std::size_t lds_bytes, hipStream_t stream, BeamShape shape, int mode) {
+    void* params[] = {const_cast<BeamArgs*>(&args)};
+    return hipLaunchKernel(hnsw_beam_kernel_ptr(shape, mode), dim3(grid), dim3(waves_per_group * kWave),
+                           params, lds_bytes, stream);
 }
 
 }  // namespace vdx::dev

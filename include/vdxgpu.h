@@ -142,9 +142,9 @@ typedef struct vdx_search_opts {
     uint32_t visited_log2;     /* LDS visited-cache slots = 2^visited_log2; 0: autotune */
     uint32_t waves_per_group;  /* waves per workgroup; 0: autotune */
     uint32_t termination;      /* 0: fixed ef (exact vecdex semantics); 1: learned stop */
-    uint32_t reserved;
-    const void* mlp;           /* learned stop model (vdx_mlp_weights), termination == 1 */
-    float mlp_threshold;       /* stop when the model's score exceeds this */
+    uint32_t min_steps;        /* learned stop: expansions before the model is consulted */
+    const void* mlp;           /* learned stop model (a vdx_gpu_mlp*), termination == 1 */
+    float mlp_threshold;       /* stop when the model's probability exceeds this, in (0, 1) */
     float reserved_f;
     uint64_t* stats;           /* optional host array of VDX_STAT_COUNT counters, summed */
 } vdx_search_opts;
@@ -153,7 +153,8 @@ enum vdx_stat {
     VDX_STAT_EXPANSIONS = 0,   /* layer-0 link blocks read */
     VDX_STAT_DISTANCES = 1,    /* distances computed (layer 0 and above) */
     VDX_STAT_UPPER_BLOCKS = 2, /* upper-layer link blocks read */
-    VDX_STAT_RECOMPUTED = 3,   /* distances of already-visited nodes the lossy visited cache let through */
+    VDX_STAT_DUPLICATES = 3,   /* recomputed members of the result list the lossy visited cache let through */
+    VDX_STAT_EARLY_STOPS = 4,  /* queries the learned model stopped before the fixed-ef stop */
     VDX_STAT_COUNT = 8
 };
 
@@ -176,6 +177,38 @@ VDX_API int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* index, const float* d_queri
                                uint32_t ef, uint32_t max_k, vdx_neighbor* d_out,
                                uint64_t out_stride, uint32_t* d_out_count,
                                const vdx_search_opts* opts, void* stream);
+
+/* ---- Learned adaptive termination (no vecdex counterpart; an addition of this path).
+ * A 16 -> 64 -> 64 -> 1 MLP (ReLU, sigmoid), BF16 with FP32 accumulation, reads 16 search-state
+ * features after every layer-0 expansion and stops the beam when its probability exceeds a
+ * threshold. It can only stop earlier than the fixed-ef search it runs inside. */
+
+#define VDX_MLP_PARAMS 5345
+#define VDX_TRACE_FEATURES 16
+
+typedef struct vdx_gpu_mlp vdx_gpu_mlp;
+
+/* params: VDX_MLP_PARAMS floats: mean[16], inv_std[16] (feature standardisation), W1[16][64],
+ * b1[64], W2[64][64], b2[64], w3[64], b3. */
+VDX_API int vdx_gpu_mlp_create(vdx_gpu_ctx* ctx, const float* params, uint64_t count, vdx_gpu_mlp** out);
+VDX_API int vdx_gpu_mlp_params(vdx_gpu_mlp* mlp, float* params, uint64_t count);
+VDX_API void vdx_gpu_mlp_free(vdx_gpu_mlp* mlp);
+/* Mini-batch Adam (beta 0.9, 0.999) on binary cross-entropy over n device rows of 16 features
+ * and labels (0/1). epoch_loss (optional, host, `epochs` floats) gets each epoch's mean loss. */
+VDX_API int vdx_gpu_mlp_train(vdx_gpu_mlp* mlp, const float* d_features, const float* d_labels,
+                              uint64_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                              float* epoch_loss);
+/* Logits for n device feature rows into d_logits. */
+VDX_API int vdx_gpu_mlp_infer(vdx_gpu_mlp* mlp, const float* d_features, uint64_t n, float* d_logits);
+
+/* Fixed-ef search that records, after each layer-0 expansion of query q (up to max_steps), the
+ * 16 features into d_features[q][step][16] and the current top-10 hits against d_truth10[q][10]
+ * into d_hits[q][step]; d_steps[q] = expansions taken, d_final[q] = hits of the result.
+ * Synchronous. */
+VDX_API int vdx_gpu_hnsw_trace_device(vdx_gpu_hnsw* index, const float* d_queries, uint64_t nq,
+                                      uint32_t k, uint32_t ef, const uint32_t* d_truth10,
+                                      uint32_t max_steps, float* d_features, uint8_t* d_hits,
+                                      uint32_t* d_steps, uint32_t* d_final);
 
 /* ---- Brute force: replaces BruteForceIndex::search (include/vecdex/brute_force.hpp:36;
  *      src/brute_force.cpp:68-84), batched over queries. */

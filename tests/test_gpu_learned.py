@@ -1,0 +1,136 @@
+"""Learned adaptive termination: traces, the BF16 MLP, training, and the learned stop inside
+the beam (no vecdex counterpart; judged by recall against brute force and by equivalence with
+fixed-ef search at the degenerate thresholds)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import hnsw_vector_search_amd as vdx
+from hnsw_vector_search_amd import _native as nat
+from hnsw_vector_search_amd import datasets, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest, ties to even), returned as float32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return r.astype(np.uint32).view(np.float32)
+
+
+def numpy_logits(params, features):
+    """The model's forward pass restated in numpy: BF16 inputs, weights and activations, float64
+    sums (the GPU sums in FP32, so the comparison allows FP32 rounding)."""
+    o = 0
+    def take(n):
+        nonlocal o
+        v = params[o:o + n]
+        o += n
+        return v
+    mean, inv = take(16), take(16)
+    w1, b1 = bf16(take(1024)).reshape(16, 64), take(64)
+    w2, b2 = bf16(take(4096)).reshape(64, 64), take(64)
+    w3, b3 = bf16(take(64)), take(1)[0]
+    x = bf16((features - mean) * inv).astype(np.float64)
+    h1 = bf16(np.maximum(x @ w1 + b1, 0)).astype(np.float64)
+    h2 = bf16(np.maximum(h1 @ w2 + b2, 0)).astype(np.float64)
+    return h2 @ w3 + b3
+
+
+@pytest.fixture(scope="module")
+def c0():
+    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 10_000, 2000)
+    index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=200, seed=0)
+    index.add(base, num_threads=8)
+    bf = vdx.BruteForceIndex(128)
+    bf.add(base)
+    truth, _ = bf.search_arrays(queries, 10, "bf16")
+    return index, queries, truth["id"].astype(np.uint32)
+
+
+def test_inference_matches_numpy_bf16_forward():
+    rng = np.random.default_rng(0)
+    feats = rng.standard_normal((500, 16)).astype(np.float32)
+    params = learned.init_params(feats.mean(0), feats.std(0), seed=3)
+    params[learned.FEATURES * 2 + 1024: learned.FEATURES * 2 + 1024 + 64] = rng.standard_normal(64) * 0.1
+    model = learned.StopModel(params)
+    got = model.logits(feats)
+    ref = numpy_logits(params, feats)
+    np.testing.assert_allclose(got, ref, rtol=2e-3, atol=2e-3)
+
+
+def test_trace_agrees_with_plain_search(c0):
+    index, queries, truth = c0
+    q = queries[:500]
+    f, labels, steps, final = learned.collect_traces(index, q, truth[:500], k=10, ef=64, max_steps=256)
+    # plain search: same expansions, same result
+    ctx = nat.Context.get(0)
+    dq = nat.DeviceArray.from_numpy(q, ctx)
+    out = nat.DeviceArray((500, 10), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(500, np.uint32, ctx)
+    stats = (ctypes.c_uint64 * 8)()
+    opts = nat.SearchOpts()
+    opts.stats = ctypes.cast(stats, ctypes.c_void_p)
+    learned.search_device(index, dq, 500, 10, 64, out, cnt, opts)
+    assert stats[0] == steps.sum()
+    ids = out.numpy()["id"]
+    hits = (truth[:500, :, None] == ids[:, None, :]).any(axis=2).sum(axis=1)
+    np.testing.assert_array_equal(hits, final)
+    assert f.shape == (np.minimum(steps, 256).sum(), 16) and np.isfinite(f).all()
+    assert set(np.unique(labels)) <= {0.0, 1.0}
+    assert 0.05 < labels.mean() < 0.95
+
+
+def test_degenerate_models_bracket_fixed_ef(c0):
+    index, queries, truth = c0
+    nq = 500
+    ctx = nat.Context.get(0)
+    dq = nat.DeviceArray.from_numpy(queries[:nq], ctx)
+    out = nat.DeviceArray((nq, 10), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(nq, np.uint32, ctx)
+    learned.search_device(index, dq, nq, 10, 64, out, cnt)
+    fixed = out.numpy()
+    params = learned.init_params(np.zeros(16), np.ones(16))
+    params[-1] = -100.0  # b3: never stop
+    never = learned.StopModel(params)
+    learned.search_device(index, dq, nq, 10, 64, out, cnt, learned.learned_opts(never, 0.5))
+    assert out.numpy().tobytes() == fixed.tobytes()
+    params[-65:-1] = 0.0
+    params[-1] = 100.0   # always stop once allowed
+    always = learned.StopModel(params)
+    stats = (ctypes.c_uint64 * 8)()
+    learned.search_device(index, dq, nq, 10, 64, out, cnt, learned.learned_opts(always, 0.5, 12, stats))
+    assert stats[4] == nq and stats[0] == 12 * nq
+
+
+def test_training_and_calibrated_learned_search(c0):
+    index, queries, truth = c0
+    train_q, eval_q = queries[:1500], queries[1500:]
+    f, labels, steps, _ = learned.collect_traces(index, train_q, truth[:1500], k=10, ef=128)
+    model, losses = learned.train_stop_model(f, labels, epochs=6, batch=4096, lr=1e-3)
+    print("epoch losses", losses)
+    assert losses[-1] < losses[0]
+    logits = model.logits(f)
+    acc = ((logits > 0) == (labels > 0.5)).mean()
+    assert acc > max(labels.mean(), 1 - labels.mean())
+    thr, rec, work, table = learned.calibrate(index, model, train_q, truth[:1500], 10, 128, 0.95)
+    print("calibration", table)
+    ctx = nat.Context.get(0)
+    nq = len(eval_q)
+    dq = nat.DeviceArray.from_numpy(eval_q, ctx)
+    out = nat.DeviceArray((nq, 10), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(nq, np.uint32, ctx)
+    stats = (ctypes.c_uint64 * 8)()
+    learned.search_device(index, dq, nq, 10, 128, out, cnt, learned.learned_opts(model, thr, 1, stats))
+    r = learned.recall_of(out.numpy()["id"], truth[1500:])
+    fixed_stats = (ctypes.c_uint64 * 8)()
+    opts = nat.SearchOpts()
+    opts.stats = ctypes.cast(fixed_stats, ctypes.c_void_p)
+    learned.search_device(index, dq, nq, 10, 128, out, cnt, opts)
+    print(f"held-out: threshold {thr}, recall {r:.4f}, expansions {stats[0] / nq:.1f} "
+          f"(fixed ef=128: {fixed_stats[0] / nq:.1f})")
+    assert r >= 0.93
+    assert stats[0] < fixed_stats[0]
