@@ -1,24 +1,32 @@
-"""Benchmark: QPS at recall@10 = 0.95 on the 1,000,000 x 128 L2 Gaussian-mixture base, one GPU.
+"""Benchmark: QPS at recall@10 = 0.95 on the 1,000,000 x 128 L2 Gaussian-mixture base, one GPU:
+learned termination vs fixed ef vs the host CPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--nq Q]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-learned] [--no-cpu]
 
-One step is one batched GPU HNSW search of all Q queries (10,000) at the operating ef: the
-smallest ef of the sweep {16, 32, 64, 128, 512} whose recall@10 against exact ground truth is
-at least 0.95 (how vecdex reads "QPS at recall 0.95", README.md:120-123). Queries, graph and
+One step is one batched GPU HNSW search of the 10,000 evaluation queries. Queries, graph and
 results stay in HBM for the timed region; the launches go through the library's device entry
-point on its context stream, bracketed by HIP events recorded on that same stream and by
-stream synchronizations. (The library owns device memory too: one HIP runtime per process.)
+point on its context stream, bracketed by HIP events recorded on that stream and by stream
+synchronizations. (The library owns device memory too: one HIP runtime per process.)
 
-Also measured in the same run and printed on the same JSON line:
-  * the ef sweep (recall by id, threshold recall, QPS),
-  * roofline: algorithmic bytes of the beam kernel per launch (query vectors, link blocks read,
-    vectors of evaluated neighbours, results, from the kernel's own counters in a counting pass
-    with a near-exact visited cache) over the measured launch time, against 8 TB/s,
-  * cpu_baseline: the oracle (a C restatement of vecdex's search_batch with its AVX2 order)
-    on the host cores of this box, same graph, same queries, same ef, vecdex's timing protocol
-    (one warmup pass, median of 5),
-  * ground truth: exact brute force on the GPU (bit-equal to vecdex's BruteForceIndex).
-Graph build (restated vecdex CPU build, all host cores) is timed and reported, not in `value`.
+Workloads (seeded; graph M=16, efConstruction=200, built on all host cores, cached in /tmp):
+  * 1M x 128 L2: 1,024-component Gaussian mixture, integer-valued; 10k evaluation queries.
+  * 1.2M x 100 cosine: anisotropic Gaussian (lambda_i ~ i^-0.8, random rotation), unit-norm;
+    10k evaluation queries (learned termination only).
+Ground truth: the library's BF16 brute force, whose certification makes it equal to the exact
+scan (vecdex's BruteForceIndex); the 1M x 128 leg also times FP32 vs BF16 brute force.
+
+Fixed ef: the sweep {16, 32, 64, 128, 512} and a finer {40, 48, 56}; the operating point is the
+smallest ef with recall@10 >= 0.95. Learned termination: traces of 100k training queries per base
+at ef=128, a stop model trained 20 epochs (Adam, lr 1e-3, batch 4096) on up to 8M trace steps,
+the threshold calibrated on 10k further queries per base for recall 0.955 at ef caps 64 and 128,
+then timed on the 10k evaluation queries. The headline `value` is the learned QPS when its
+evaluation recall is >= 0.95, else the fixed-ef QPS.
+
+roofline: algorithmic bytes of the headline beam launch (query, link blocks read, vectors of
+evaluated neighbours, results, from the kernel's counters in a counting pass with an 8K-slot
+visited cache) over its measured launch time, against 8 TB/s. cpu_baseline: the oracle's C
+restatement of vecdex's search_batch (AVX2 order) on this box's cores, fixed-ef operating
+point, vecdex's protocol (one warmup, median of 5).
 """
 
 from __future__ import annotations
@@ -37,7 +45,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_TFLOPS = 157.3
+BF16_MFMA_TFLOPS = 2500.0
 EF_SWEEP = (16, 32, 64, 128, 512)
+EF_FINE = (40, 48, 56)
 TARGET_RECALL = 0.95
 K = 10
 CACHE = Path(os.environ.get("VDX_BENCH_CACHE", "/tmp/vdx_bench_cache"))
@@ -51,41 +62,93 @@ def host_threads() -> int:
     return max(1, min(os.cpu_count() or 1, 16))
 
 
-def build_or_load_graph(vdx, base, n, threads):
+def build_or_load_graph(vdx, name, base, metric, threads):
     CACHE.mkdir(parents=True, exist_ok=True)
-    path = CACHE / f"c1_n{n}_M16_efc200_seed0.vdx"
+    path = CACHE / f"{name}_n{len(base)}_M16_efc200_seed0.vdx"
     if path.exists():
         t = time.perf_counter()
         index = vdx.HnswIndex.load(path)
-        log(f"loaded cached graph {path} in {time.perf_counter() - t:.1f} s")
+        log(f"{name}: loaded cached graph in {time.perf_counter() - t:.1f} s")
         return index, None
-    index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=200, seed=0)
+    index = vdx.HnswIndex(base.shape[1], metric, M=16, ef_construction=200, seed=0)
     t = time.perf_counter()
     index.add(base, num_threads=threads)
     build_s = time.perf_counter() - t
-    log(f"built graph n={n} on {threads} threads in {build_s:.1f} s")
+    log(f"{name}: built graph n={len(base)} on {threads} threads in {build_s:.1f} s")
     index.save(path)
     return index, build_s
 
 
-def search_device(nat, handle, dq, nq, ef, out, cnt, opts=None):
-    nat.check(nat.lib.vdx_gpu_hnsw_search_device(
-        handle, dq.ptr, nq, None, K, None, ef, K, out.ptr, K, cnt.ptr,
-        ctypes.byref(opts) if opts is not None else None, None))
+class Runner:
+    """Device-resident queries and results for one query set, and timed launches."""
+
+    def __init__(self, nat, index, queries, k=K):
+        self.nat, self.index, self.k = nat, index, k
+        self.ctx = nat.Context.get(0)
+        self.nq = len(queries)
+        self.dq = nat.DeviceArray.from_numpy(np.ascontiguousarray(queries, dtype=np.float32), self.ctx)
+        self.out = nat.DeviceArray((self.nq, k), nat.NEIGHBOR_DTYPE, self.ctx)
+        self.cnt = nat.DeviceArray(self.nq, np.uint32, self.ctx)
+
+    def launch(self, ef, opts=None):
+        self.nat.check(self.nat.lib.vdx_gpu_hnsw_search_device(
+            self.index.gpu_handle(), self.dq.ptr, self.nq, None, self.k, None, ef, self.k,
+            self.out.ptr, self.k, self.cnt.ptr, ctypes.byref(opts) if opts is not None else None, None))
+
+    def results(self):
+        return self.out.numpy()
+
+    def timed(self, ef, steps, warmup, opts=None):
+        """Milliseconds per launch over exactly `steps` launches (HIP events on the launch
+        stream, stream drained before and after), and wall milliseconds per launch."""
+        nat = self.nat
+        for _ in range(warmup):
+            self.launch(ef, opts)
+        nat.check(nat.lib.vdx_gpu_synchronize(self.ctx.handle))
+        t0 = time.perf_counter()
+        nat.check(nat.lib.vdx_gpu_event_record(self.ctx.handle, 0))
+        for _ in range(steps):
+            self.launch(ef, opts)
+        nat.check(nat.lib.vdx_gpu_event_record(self.ctx.handle, 1))
+        nat.check(nat.lib.vdx_gpu_synchronize(self.ctx.handle))
+        wall = (time.perf_counter() - t0) * 1000 / steps
+        return nat.elapsed_ms(self.ctx, 0, 1) / steps, wall
+
+    def counters(self, ef, opts=None, visited_log2=0):
+        nat = self.nat
+        stats = (ctypes.c_uint64 * 8)()
+        o = opts if opts is not None else nat.SearchOpts()
+        saved = (o.stats, o.visited_log2)
+        o.stats = ctypes.cast(stats, ctypes.c_void_p)
+        o.visited_log2 = visited_log2
+        self.launch(ef, o)
+        o.stats, o.visited_log2 = saved
+        return [int(stats[i]) for i in range(5)]
 
 
-def timed(nat, ctx, steps, launch):
-    """Milliseconds per launch over exactly `steps` launches, by HIP events on the stream the
-    launches use, with the stream drained before and after."""
-    nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
-    t0 = time.perf_counter()
-    nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 0))
-    for _ in range(steps):
-        launch()
-    nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 1))
-    nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
-    wall = (time.perf_counter() - t0) * 1000 / steps
-    return nat.elapsed_ms(ctx, 0, 1) / steps, wall
+def recall(found, truth):
+    return float((truth["id"][:, :, None] == found["id"][:, None, :]).any(axis=2).mean())
+
+
+def threshold_recall(found, truth):
+    kth = truth["distance"][:, K - 1]
+    thr = kth + np.float32(1e-5) * np.maximum(np.abs(kth), np.float32(1))
+    return float((found["distance"] <= thr[:, None]).sum(axis=1).mean() / K)
+
+
+def prepared(nat, queries, cosine):
+    """Queries as the device entry points take them: unit length under cosine (detail::normalize)."""
+    q = np.array(queries, dtype=np.float32, order="C", copy=True)
+    if cosine:
+        nat.check(nat.lib.vdx_host_normalize_rows(nat.ptr(q), q.shape[0], q.shape[1]))
+    return q
+
+
+def ground_truth(vdx, base, metric, queries, k=K):
+    bf = vdx.BruteForceIndex(base.shape[1], metric)
+    bf.add(base)
+    truth, _ = bf.search_arrays(queries, k, "bf16")
+    return truth, bf
 
 
 def main(argv=None) -> int:
@@ -95,8 +158,12 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--n-cos", type=int, default=1_200_000)
+    ap.add_argument("--train-queries", type=int, default=100_000, help="per base")
+    ap.add_argument("--max-train-steps", type=int, default=8_000_000)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP)))
+    ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (HNSW search "
@@ -105,101 +172,111 @@ def main(argv=None) -> int:
 
     import hnsw_vector_search_amd as vdx
     from hnsw_vector_search_amd import _native as nat
-    from hnsw_vector_search_amd import datasets
+    from hnsw_vector_search_amd import datasets, learned
 
     threads = host_threads()
     t = time.perf_counter()
-    base, queries = datasets.gmm_base_and_queries(datasets.c1_mixture(), args.n, args.nq)
+    mix = datasets.c1_mixture()
+    base = mix.sample(args.n, 1)
+    queries = mix.sample(args.nq, 2)
     log(f"generated {args.n} x 128 base and {args.nq} queries in {time.perf_counter() - t:.1f} s")
-    index, build_s = build_or_load_graph(vdx, base, args.n, threads)
+    index, build_s = build_or_load_graph(vdx, "c1", base, vdx.Metric.L2, threads)
 
-    # Ground truth: exact brute force on the GPU.
-    bf = vdx.BruteForceIndex(128)
-    bf.add(base)
+    # Ground truth (and the brute-force leg): FP32 vs BF16 MFMA shortlist, identical ids.
     t = time.perf_counter()
-    truth, _ = bf.search_arrays(queries, K, "exact")
+    truth, bf = ground_truth(vdx, base, vdx.Metric.L2, queries)
     gt_s = time.perf_counter() - t
-    log(f"ground truth (GPU exact brute force) in {gt_s:.2f} s")
-    del bf
-
-    handle = index.gpu_handle()
+    bf_leg = {}
     ctx = nat.Context.get(0)
-    nq = args.nq
     dq = nat.DeviceArray.from_numpy(queries, ctx)
-    out = nat.DeviceArray((nq, K), nat.NEIGHBOR_DTYPE, ctx)
-    cnt = nat.DeviceArray(nq, np.uint32, ctx)
+    bout = nat.DeviceArray((args.nq, K), nat.NEIGHBOR_DTYPE, ctx)
+    bcnt = nat.DeviceArray(args.nq, np.uint32, ctx)
+    for prec in ("fp32", "bf16"):
+        st = nat.BfStats()
+        times = []
+        for _ in range(3):
+            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+            t0 = time.perf_counter()
+            nat.check(nat.lib.vdx_gpu_bf_search_device(bf.gpu_handle(), dq.ptr, args.nq, K,
+                                                       nat.PRECISIONS[prec], bout.ptr, K, bcnt.ptr,
+                                                       ctypes.byref(st)))
+            times.append((time.perf_counter() - t0) * 1000)
+        res = bout.numpy()
+        ms = min(times)
+        flops = 2.0 * args.nq * args.n * 128
+        peak = FP32_MFMA_TFLOPS if prec == "fp32" else BF16_MFMA_TFLOPS
+        bf_leg[prec] = {"ms": round(ms, 2), "qps": round(args.nq / ms * 1000, 1),
+                        "identical_to_exact": bool(res.tobytes() == truth.tobytes()),
+                        "exact_fallbacks": int(st.exact_fallback), "shortlist": int(st.shortlist),
+                        "end_to_end_tflops": round(flops / ms / 1e9, 1),
+                        "frac_of_mfma_peak": round(flops / ms / 1e9 / peak, 4)}
+        log(f"brute force {prec}: {ms:.1f} ms per 10k queries, identical={bf_leg[prec]['identical_to_exact']}, "
+            f"fallbacks {st.exact_fallback}")
+    bf_leg["bf16_speedup_vs_fp32"] = round(bf_leg["fp32"]["ms"] / bf_leg["bf16"]["ms"], 2)
+    del bf, dq, bout, bcnt
 
-    def launch(ef):
-        return lambda: search_device(nat, handle, dq, nq, ef, out, cnt)
-
-    kth_true = truth["distance"][:, K - 1]
+    # Fixed-ef sweep.
+    run = Runner(nat, index, queries)
     sweep = []
-    for ef in map(int, args.efs.split(",")):
-        launch(ef)()
-        res = out.numpy()
-        recall = float((truth["id"][:, :, None] == res["id"][:, None, :]).any(axis=2).mean())
-        thr = kth_true + np.float32(1e-5) * np.maximum(np.abs(kth_true), np.float32(1))
-        recall_thr = float((res["distance"] <= thr[:, None]).sum(axis=1).mean() / K)
-        for _ in range(args.warmup):
-            launch(ef)()
-        ms, _ = timed(nat, ctx, args.steps, launch(ef))
-        sweep.append({"ef": ef, "recall": round(recall, 4), "recall_threshold": round(recall_thr, 4),
-                      "ms": round(ms, 4), "qps": round(nq / (ms / 1000.0), 1)})
-        log(f"ef {ef}: recall {recall:.4f} (threshold {recall_thr:.4f}), {ms:.3f} ms/pass, "
-            f"{nq / (ms / 1000):,.0f} QPS")
+    for ef in sorted(set(map(int, args.efs.split(",")))):
+        run.launch(ef)
+        res = run.results()
+        ms, _ = run.timed(ef, args.steps, args.warmup)
+        row = {"ef": ef, "recall": round(recall(res, truth), 4),
+               "recall_threshold": round(threshold_recall(res, truth), 4),
+               "ms": round(ms, 4), "qps": round(args.nq / ms * 1000, 1)}
+        sweep.append(row)
+        log(f"fixed ef {ef}: recall {row['recall']:.4f}, {ms:.3f} ms/pass, {row['qps']:,.0f} QPS")
+    config_points = [s for s in sweep if s["ef"] in EF_SWEEP and s["recall"] >= TARGET_RECALL]
+    fixed = min(config_points, key=lambda s: s["ef"]) if config_points else sweep[-1]
+    fine_points = [s for s in sweep if s["recall"] >= TARGET_RECALL]
+    fixed_best = min(fine_points, key=lambda s: s["ef"]) if fine_points else fixed
 
-    chosen = next((s for s in sweep if s["recall"] >= TARGET_RECALL), sweep[-1])
-    ef = chosen["ef"]
+    # Learned termination.
+    learned_leg = None
+    headline = {"mode": "fixed-ef", "ef": fixed["ef"], "opts": None, "recall": fixed["recall"]}
+    if not args.no_learned:
+        learned_leg, headline = run_learned(args, vdx, nat, datasets, learned, index, base, queries,
+                                            truth, mix, threads, fixed, fixed_best, headline)
 
-    # Headline timing at the operating ef: exactly `steps` steps between synchronizations.
-    for _ in range(args.warmup):
-        launch(ef)()
-    ms, wall_ms = timed(nat, ctx, args.steps, launch(ef))
-    qps = nq / (ms / 1000.0)
-    gpu_results = out.numpy()
-
-    # Counting pass: the kernel's own counters with an 8K-slot visited cache (near-exact visited
-    # set at these ef), so the byte count is the algorithm's, not the lossy cache's recomputes.
-    stats = (ctypes.c_uint64 * 8)()
-    opts = nat.SearchOpts()
-    opts.visited_log2 = 13
-    opts.stats = ctypes.cast(stats, ctypes.c_void_p)
-    search_device(nat, handle, dq, nq, ef, out, cnt, opts)
-    expansions, distances, upper_blocks, dups = (int(stats[i]) for i in range(4))
-    stats_default = (ctypes.c_uint64 * 8)()
-    opts_default = nat.SearchOpts()
-    opts_default.stats = ctypes.cast(stats_default, ctypes.c_void_p)
-    search_device(nat, handle, dq, nq, ef, out, cnt, opts_default)
-    d, M = 128, 16
+    # Headline timing: exactly `steps` launches.
+    ms, wall_ms = run.timed(headline["ef"], args.steps, args.warmup, headline["opts"])
+    qps = args.nq / ms * 1000
+    res = run.results()
+    rec = recall(res, truth)
+    counts = run.counters(headline["ef"], headline["opts"], visited_log2=13)
+    expansions, distances, upper_blocks, dups = counts[:4]
+    d, M, nq = 128, 16, args.nq
     alg_bytes = (nq * 4 * d + upper_blocks * 4 * (1 + M) + expansions * 4 * (1 + 2 * M)
                  + (distances - dups) * 4 * d + nq * 8 * K)
     achieved = alg_bytes / (ms / 1000.0) / 1e9
-    log(f"counters: {expansions / nq:.1f} expansions, {(distances - dups) / nq:.1f} distances, "
-        f"{upper_blocks / nq:.1f} upper blocks per query; {alg_bytes / nq / 1e3:.1f} KB/query; "
-        f"default cache computes {int(stats_default[1]) / nq:.1f} distances/query")
+    log(f"headline ({headline['mode']}, ef {headline['ef']}): recall {rec:.4f}, {ms:.3f} ms, "
+        f"{qps:,.0f} QPS; {expansions / nq:.1f} expansions, {(distances - dups) / nq:.1f} distances, "
+        f"{alg_bytes / nq / 1e3:.1f} KB per query -> {achieved:.0f} GB/s")
 
-    # CPU baseline: the oracle's restatement of vecdex search_batch on this box's cores.
+    # CPU baseline at the fixed-ef operating point.
     cpu = None
     identical = None
     if not args.no_cpu:
         from oracle import oracle as o
         g = index.graph_arrays()
-        o.hnsw_search(g, queries, K, ef, threads=threads)  # warmup
+        ef = fixed["ef"]
+        o.hnsw_search(g, queries, K, ef, threads=threads)
         times = []
         for _ in range(5):
             t = time.perf_counter()
-            ref, ref_cnt = o.hnsw_search(g, queries, K, ef, threads=threads)
+            ref, _ = o.hnsw_search(g, queries, K, ef, threads=threads)
             times.append(time.perf_counter() - t)
         med = sorted(times)[len(times) // 2]
-        same = [gpu_results[q].tobytes() == ref[q].tobytes() for q in range(nq)]
-        identical = float(np.mean(same))
-        cpu = {"value": round(nq / med, 1), "unit": "queries/s", "cores": threads,
-               "kind": "port",
-               "sample": f"all {nq} queries at ef={ef}, warmup + median of 5 passes "
-                         f"(vecdex bench/hnsw_eval.cpp:493-512), {os.cpu_count()} host cpus, "
-                         f"{threads} threads"}
-        log(f"CPU baseline: {nq / med:,.0f} QPS on {threads} threads; GPU results identical to "
-            f"CPU for {identical * 100:.2f}% of queries")
+        run.launch(ef)
+        gpu_fixed = run.results()
+        identical = float(np.mean([gpu_fixed[q].tobytes() == ref[q].tobytes() for q in range(nq)]))
+        cpu = {"value": round(nq / med, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"all {nq} queries at fixed ef={ef} (recall {fixed['recall']}), warmup + "
+                         f"median of 5 passes (vecdex bench/hnsw_eval.cpp:493-512), "
+                         f"{os.cpu_count()} host cpus, {threads} threads"}
+        log(f"CPU baseline: {nq / med:,.0f} QPS on {threads} threads; GPU fixed-ef results identical "
+            f"to the CPU's for {identical * 100:.2f}% of queries")
 
     line = {
         "metric": "QPS @ recall@10=0.95, 1M×128 L2, 1 GPU: learned vs fixed-ef vs host CPU",
@@ -214,16 +291,21 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"GPU HNSW search, fixed ef: {args.n:,} x 128 L2 Gaussian mixture "
-                               f"(1,024 components, integer-valued), {nq:,} queries, graph M=16 "
-                               f"efConstruction=200 seed 0, k=10",
-                   "ef": ef, "recall@10": chosen["recall"], "termination": "fixed-ef",
-                   "parallelism": "single GPU"},
+        "config": {"workload": f"GPU HNSW search ({headline['mode']}): {args.n:,} x 128 L2 Gaussian "
+                               f"mixture (1,024 components, integer-valued), {nq:,} queries, graph "
+                               f"M=16 efConstruction=200 seed 0, k=10",
+                   "ef": headline["ef"], "termination": headline["mode"],
+                   "recall@10": round(rec, 4), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "fixed_ef": {"ef": fixed["ef"], "recall": fixed["recall"], "qps": fixed["qps"],
+                     "best_fine_ef": fixed_best["ef"], "best_fine_qps": fixed_best["qps"],
+                     "best_fine_recall": fixed_best["recall"]},
+        "learned": learned_leg,
         "sweep": sweep,
+        "bruteforce": bf_leg,
         "gpu_cpu_identical_fraction": identical,
         "wall_ms_per_step": round(wall_ms, 4),
         "ground_truth_s": round(gt_s, 3),
@@ -231,6 +313,100 @@ def main(argv=None) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, mix, threads,
+                fixed, fixed_best, headline):
+    """Traces, training, calibration and evaluation of the learned stop (both bases)."""
+    t_all = time.perf_counter()
+    cos_gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
+    cos_base = cos_gen.sample(args.n_cos, 1)
+    cos_index, _ = build_or_load_graph(vdx, "c2b", cos_base, vdx.Metric.Cosine, threads)
+    ntr = args.train_queries
+    sets = {
+        "l2": {"index": index, "base": base, "metric": vdx.Metric.L2,
+               "train": mix.sample(ntr, 3), "calib": mix.sample(args.nq, 4), "eval": queries},
+        "cos": {"index": cos_index, "base": cos_base, "metric": vdx.Metric.Cosine,
+                "train": prepared(nat, cos_gen.sample(ntr, 3), True),
+                "calib": prepared(nat, cos_gen.sample(args.nq, 4), True),
+                "eval": prepared(nat, cos_gen.sample(args.nq, 2), True)},
+    }
+    feats, labels = [], []
+    for name, s in sets.items():
+        allq = np.concatenate([s["train"], s["calib"], s["eval"]])
+        t = time.perf_counter()
+        tr, _ = ground_truth(vdx, s["base"], s["metric"], allq)
+        s["truth_train"] = tr[:ntr]
+        s["truth_calib"] = tr[ntr:ntr + args.nq]
+        s["truth_eval"] = truth if name == "l2" else tr[ntr + args.nq:]
+        log(f"learned/{name}: ground truth for {len(allq)} queries in {time.perf_counter() - t:.1f} s")
+        t = time.perf_counter()
+        f, lab, steps, final = learned.collect_traces(s["index"], s["train"], s["truth_train"]["id"],
+                                                      k=K, ef=128, max_steps=256)
+        log(f"learned/{name}: traces {len(f):,} steps ({steps.mean():.1f} expansions/query, "
+            f"label mean {lab.mean():.3f}) in {time.perf_counter() - t:.1f} s")
+        feats.append(f)
+        labels.append(lab)
+    f = np.concatenate(feats)
+    lab = np.concatenate(labels)
+    if len(f) > args.max_train_steps:
+        pick = np.random.default_rng(0).choice(len(f), args.max_train_steps, replace=False)
+        f, lab = f[pick], lab[pick]
+    t = time.perf_counter()
+    model, losses = learned.train_stop_model(f, lab, epochs=20, batch=4096, lr=1e-3, seed=0)
+    train_s = time.perf_counter() - t
+    log(f"learned: trained on {len(f):,} steps, 20 epochs in {train_s:.1f} s; loss "
+        f"{losses[0]:.4f} -> {losses[-1]:.4f}")
+
+    # Calibrate per base and ef cap on the calibration queries (target 0.955 for margin).
+    results = {}
+    for name, s in sets.items():
+        best = None
+        for ef_cap in (64, 128):
+            thr, rec_c, work, table = learned.calibrate(s["index"], model, s["calib"],
+                                                        s["truth_calib"]["id"], K, ef_cap, 0.955)
+            r = Runner(nat, s["index"], s["eval"])
+            opts = learned.learned_opts(model, thr, 1)
+            r.launch(ef_cap, opts)
+            rec_e = recall(r.results(), s["truth_eval"])
+            ms, _ = r.timed(ef_cap, args.steps, args.warmup, opts)
+            exp = r.counters(ef_cap, opts)[0] / len(s["eval"])
+            cand = {"ef_cap": ef_cap, "threshold": thr, "calib_recall": round(rec_c, 4),
+                    "eval_recall": round(rec_e, 4), "ms": round(ms, 4),
+                    "qps": round(len(s["eval"]) / ms * 1000, 1), "expansions": round(exp, 1),
+                    "_opts": opts}
+            log(f"learned/{name} ef cap {ef_cap}: threshold {thr}, calib recall {rec_c:.4f}, eval "
+                f"recall {rec_e:.4f}, {ms:.3f} ms, {cand['qps']:,.0f} QPS, {exp:.1f} expansions")
+            # choose by the calibration set only
+            if rec_c >= 0.955 and (best is None or work < best["_work"]):
+                cand["_work"] = work
+                best = cand
+        results[name] = best
+        if name == "cos":
+            fixed_cos = []
+            r = Runner(nat, s["index"], s["eval"])
+            for ef in (32, 48, 64, 96, 128, 256):
+                r.launch(ef)
+                rc = recall(r.results(), s["truth_eval"])
+                ms, _ = r.timed(ef, args.steps, args.warmup)
+                fixed_cos.append({"ef": ef, "recall": round(rc, 4), "qps": round(len(s["eval"]) / ms * 1000, 1)})
+                log(f"cos fixed ef {ef}: recall {rc:.4f}, {fixed_cos[-1]['qps']:,.0f} QPS")
+            results["cos_fixed_sweep"] = fixed_cos
+    l2 = results["l2"]
+    leg = {
+        "train_steps": len(f), "train_s": round(train_s, 1),
+        "loss_first_last": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
+        "l2": {k: v for k, v in (l2 or {}).items() if not k.startswith("_")},
+        "cos": {k: v for k, v in (results["cos"] or {}).items() if not k.startswith("_")},
+        "cos_fixed_sweep": results["cos_fixed_sweep"],
+        "total_s": round(time.perf_counter() - t_all, 1),
+    }
+    if l2 is not None and l2["eval_recall"] >= TARGET_RECALL:
+        leg["speedup_vs_fixed_ef"] = round(l2["qps"] / fixed["qps"], 3)
+        leg["speedup_vs_best_fine_fixed_ef"] = round(l2["qps"] / fixed_best["qps"], 3)
+        headline = {"mode": "learned", "ef": l2["ef_cap"], "opts": l2["_opts"],
+                    "recall": l2["eval_recall"], "model": model}  # the model must stay alive
+    return leg, headline
 
 
 if __name__ == "__main__":
