@@ -74,6 +74,13 @@ _SIGS = {
     "vdx_host_check_new_vectors": (c_int, [c_char_p, c_u32, c_u64, c_u64, c_ptr, c_u64, c_u64]),
     "vdx_host_normalize_rows": (c_int, [c_ptr, c_u64, c_u64]),
     "vdx_host_distance": (c_float, [c_u32, c_ptr, c_ptr, c_u64]),
+    "vdx_recall_at_k": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_u64, ctypes.POINTER(ctypes.c_double)]),
+    "vdx_threshold_recall_at_k": (c_int, [c_ptr, c_u64, c_float, c_u64, c_float,
+                                          ctypes.POINTER(ctypes.c_double)]),
+    "vdx_mean_recall_at_k": (c_int, [c_ptr, c_u64, c_ptr, c_ptr, c_u64, c_u64, c_u64,
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "vdx_mean_threshold_recall_at_k": (c_int, [c_ptr, c_u64, c_ptr, c_ptr, c_u64, c_u64, c_float,
+                                               ctypes.POINTER(ctypes.c_double)]),
     "vdx_gpu_open": (c_int, [c_int, c_ptr]),
     "vdx_gpu_close": (None, [c_ptr]),
     "vdx_gpu_stream": (c_ptr, [c_ptr]),

@@ -107,6 +107,26 @@ VDX_API int vdx_host_normalize_rows(float* data, uint64_t rows, uint64_t dim);
 /* The library's distance (the arithmetic every kernel reproduces). */
 VDX_API float vdx_host_distance(uint32_t metric, const float* a, const float* b, uint64_t dim);
 
+/* recall_at_k (src/recall.cpp:11-34): |first k of found ∩ first k of truth| / k; a duplicate
+ * found id counts once. Rejects k == 0 and truth_count < k (the reference's messages). */
+VDX_API int vdx_recall_at_k(const vdx_neighbor* found, uint64_t found_count, const int32_t* truth,
+                            uint64_t truth_count, uint64_t k, double* out);
+/* threshold_recall_at_k (src/recall.cpp:36-51): the fraction of the first k found whose distance
+ * is <= kth_true + tolerance * max(|kth_true|, 1). Rejects k == 0, a non-finite kth_true or
+ * tolerance, and a negative tolerance. */
+VDX_API int vdx_threshold_recall_at_k(const vdx_neighbor* found, uint64_t found_count,
+                                      float kth_true, uint64_t k, float tolerance, double* out);
+/* Means of the two over nq result rows (no counterpart in the reference, which evaluates one
+ * query at a time in bench/hnsw_eval.cpp:513-520): row q holds found_counts[q] results (all
+ * found_stride when found_counts is NULL) at found + q * found_stride; truth rows are
+ * truth_stride ids (recall) or one k-th true distance per query (threshold recall). */
+VDX_API int vdx_mean_recall_at_k(const vdx_neighbor* found, uint64_t found_stride,
+                                 const uint32_t* found_counts, const int32_t* truth,
+                                 uint64_t truth_stride, uint64_t nq, uint64_t k, double* out);
+VDX_API int vdx_mean_threshold_recall_at_k(const vdx_neighbor* found, uint64_t found_stride,
+                                           const uint32_t* found_counts, const float* kth_true,
+                                           uint64_t nq, uint64_t k, float tolerance, double* out);
+
 /* ================================================================================= GPU side */
 
 typedef struct vdx_gpu_ctx vdx_gpu_ctx;
