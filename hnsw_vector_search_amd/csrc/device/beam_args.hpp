@@ -38,10 +38,14 @@ struct BeamArgs {
     std::uint32_t k_cap;  // per-query k is clamped to this (<= out_stride, <= ef_cap)
     // launch state
     std::uint32_t* queue;  // next query to take; zero before the launch
+    // optional work list: the launch serves queries order[range[0] .. range[1]) (device
+    // values, written by an earlier launch on the stream); null: queries 0 .. nq
+    const std::uint32_t* order;
+    const std::uint32_t* range;
     std::uint32_t ef_cap;  // R capacity per wave (>= every max(ef, k)), a multiple of 64
     std::uint32_t visited_log2;
     std::uint32_t lds_per_wave;
-    unsigned long long* stats;  // 5 counters, or null
+    unsigned long long* stats;  // kBeamStats counters, or null
     // trace mode: per step features and top-10 hits against `truth` (10 ids per query)
     const std::uint32_t* truth;
     float* trace_feat;           // [nq][trace_cap][16]
@@ -56,6 +60,8 @@ struct BeamArgs {
     std::uint32_t min_steps;
 };
 
+constexpr int kBeamStats = 6;
+
 enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
 
 enum class BeamShape { D128, D100, Generic };
@@ -66,11 +72,26 @@ inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited
            (std::size_t{4} << visited_log2);
 }
 
+// The register-resident beam (hnsw_beam_reg.hpp) holds R in registers in chunks of 64 keys;
+// it serves ef' <= kRegMaxEf. Its LDS per wave: the merge buffer, fresh ids and distances,
+// features and the visited cache.
+constexpr std::uint32_t kRegMaxEf = 512;
+inline std::size_t beam_reg_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {
+    return 8u * static_cast<std::size_t>(ef_cap) + 4u * 64 + 4u * 64 + 4u * 16 +
+           (std::size_t{4} << visited_log2);
+}
+
 // LDS a workgroup needs besides its waves' regions (the stop model in learned mode).
 std::size_t beam_lds_shared(int mode);
 
 hipError_t launch_hnsw_beam(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
                             std::size_t lds_bytes, hipStream_t stream, BeamShape shape, int mode);
 const void* hnsw_beam_kernel_ptr(BeamShape shape, int mode);
+
+// The register-resident kernel for R capacity 64 * chunks (chunks 1, 2, 4 or 8).
+const void* hnsw_beam_reg_kernel_ptr(BeamShape shape, int mode, int chunks);
+hipError_t launch_hnsw_beam_reg(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
+                                std::size_t lds_bytes, hipStream_t stream, BeamShape shape, int mode,
+                                int chunks);
 
 }  // namespace vdx::dev

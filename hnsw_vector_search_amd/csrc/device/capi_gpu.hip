@@ -70,18 +70,32 @@ std::uint32_t default_visited_log2(std::uint32_t ef_cap) {
 
 struct BeamPlan {
     std::uint32_t ef_cap, visited_log2, waves, grid;
+    int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
     std::size_t lds_per_wave, lds_bytes;
 };
 
+// R capacity and kernel for a batch whose largest max(ef, k) is max_efp: the register kernel
+// with the smallest power-of-two chunk count that holds it, else the LDS kernel.
 BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t nq,
                    const vdx_search_opts* opts, int mode) {
     BeamPlan p{};
-    p.ef_cap = (max_efp + 63u) / 64u * 64u;
+    const std::uint32_t need = (max_efp + 63u) / 64u;
+    if (need * 64u <= vdx::dev::kRegMaxEf) {
+        p.chunks = 1;
+        while (static_cast<std::uint32_t>(p.chunks) < need) {
+            p.chunks *= 2;
+        }
+        p.ef_cap = 64u * static_cast<std::uint32_t>(p.chunks);
+    } else {
+        p.chunks = 0;
+        p.ef_cap = need * 64u;
+    }
     p.visited_log2 = opts && opts->visited_log2 ? opts->visited_log2 : default_visited_log2(p.ef_cap);
     require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
     p.waves = opts && opts->waves_per_group ? opts->waves_per_group : 4;
     require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
-    p.lds_per_wave = vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
+    p.lds_per_wave = p.chunks > 0 ? vdx::dev::beam_reg_lds_per_wave(p.ef_cap, p.visited_log2)
+                                  : vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
     const std::size_t limit = h.ctx->lds_per_block;
     const std::size_t shared = vdx::dev::beam_lds_shared(mode);
     while (p.waves > 1 && p.lds_per_wave * p.waves + shared > limit) {
@@ -89,10 +103,10 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     }
     p.lds_bytes = p.lds_per_wave * p.waves + shared;
     require(p.lds_bytes <= limit, "ef too large for the GPU beam's LDS");
+    const void* kernel = p.chunks > 0 ? vdx::dev::hnsw_beam_reg_kernel_ptr(beam_shape(h.dim), mode, p.chunks)
+                                      : vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim), mode);
     int per_cu = 0;
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &per_cu, vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim), mode), p.waves * 64,
-              p.lds_bytes),
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, p.waves * 64, p.lds_bytes),
           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     per_cu = std::max(per_cu, 1);
     const std::uint64_t resident = static_cast<std::uint64_t>(per_cu) * h.ctx->cu_count;
@@ -171,9 +185,15 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
         a.min_steps = opts->min_steps;
     }
     check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
-    check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                     beam_shape(h.dim), mode),
-          "hnsw_beam launch");
+    if (plan.chunks > 0) {
+        check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,
+                                             beam_shape(h.dim), mode, plan.chunks),
+              "hnsw_beam_reg launch");
+    } else {
+        check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
+                                         beam_shape(h.dim), mode),
+              "hnsw_beam launch");
+    }
 }
 
 // Validates and prepares nq host queries into pinned staging (rows of dim floats).
@@ -319,20 +339,47 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_upload: too many upper blocks");
         require(g->n == 0 || g->levels[g->entry_point] == g->max_level,
                 "vdx_gpu_hnsw_upload: the entry point is not on the top layer");
-        // Every block the kernel may follow: counts within capacity, ids naming nodes. The kernel
-        // relies on both to keep its gathers in bounds.
-        for (std::uint64_t i = 0; i < g->n; ++i) {
-            const std::uint32_t* b = g->layer0 + i * (1 + 2 * g->M);
-            require(b[0] <= 2 * g->M, "vdx_gpu_hnsw_upload: a layer-0 block exceeds 2M links");
-            for (std::uint32_t j = 0; j < b[0]; ++j) {
-                require(b[1 + j] < g->n, "vdx_gpu_hnsw_upload: a layer-0 link names a missing node");
-            }
-        }
-        for (std::uint64_t blk = 0; blk < g->upper_blocks; ++blk) {
-            const std::uint32_t* b = g->upper + blk * (1 + g->M);
-            require(b[0] <= g->M, "vdx_gpu_hnsw_upload: an upper block exceeds M links");
-            for (std::uint32_t j = 0; j < b[0]; ++j) {
-                require(b[1 + j] < g->n, "vdx_gpu_hnsw_upload: an upper link names a missing node");
+        // Every block the kernel may follow, with the checks of HnswIndex::check_links
+        // (src/hnsw_io.cpp:472-504): counts within capacity, ids naming nodes that exist on the
+        // block's layer, no self-links, no id twice in a block. The kernels rely on these: an
+        // upper link to a node below the layer would index past that node's blocks, and a
+        // repeated id would enter one expansion's fresh batch twice.
+        {
+            std::vector<std::uint32_t> stamp(g->n, 0);
+            std::uint32_t epoch = 0;
+            const auto check_block = [&](const std::uint32_t* b, std::uint64_t node, std::uint64_t layer,
+                                         std::uint64_t cap) {
+                if (b[0] > cap) {
+                    throw std::invalid_argument(vdx::fmt(
+                        "vdx_gpu_hnsw_upload: node %llu has %u links on layer %llu (max %llu)",
+                        static_cast<unsigned long long>(node), b[0], static_cast<unsigned long long>(layer),
+                        static_cast<unsigned long long>(cap)));
+                }
+                if (++epoch == 0) {
+                    std::fill(stamp.begin(), stamp.end(), 0u);
+                    epoch = 1;
+                }
+                for (std::uint32_t j = 0; j < b[0]; ++j) {
+                    const std::uint32_t id = b[1 + j];
+                    const char* why = id >= g->n               ? "names a missing node"
+                                      : id == node             ? "is a self-link"
+                                      : g->levels[id] < layer  ? "names a node below the layer"
+                                      : stamp[id] == epoch     ? "repeats a node"
+                                                               : nullptr;
+                    if (why != nullptr) {
+                        throw std::invalid_argument(vdx::fmt(
+                            "vdx_gpu_hnsw_upload: link %u of node %llu on layer %llu %s", id,
+                            static_cast<unsigned long long>(node), static_cast<unsigned long long>(layer), why));
+                    }
+                    stamp[id] = epoch;
+                }
+            };
+            std::uint64_t off = 0;
+            for (std::uint64_t i = 0; i < g->n; ++i) {
+                check_block(g->layer0 + i * (1 + 2 * g->M), i, 0, 2 * g->M);
+                for (std::uint64_t layer = 1; layer <= g->levels[i]; ++layer, ++off) {
+                    check_block(g->upper + off * (1 + g->M), i, layer, g->M);
+                }
             }
         }
 
@@ -439,7 +486,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         unsigned long long* d_stats = nullptr;
         if (opts && opts->stats) {
             d_stats = reinterpret_cast<unsigned long long*>(ws->counters.as<char>() + 16);
-            check(hipMemsetAsync(d_stats, 0, 5 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+            check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
         }
         enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
@@ -450,7 +497,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
               "hipMemcpyAsync(out)");
         check(hipMemcpyAsync(out_count, ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(count)");
-        unsigned long long host_stats[5] = {0, 0, 0, 0, 0};
+        unsigned long long host_stats[vdx::dev::kBeamStats] = {};
         if (d_stats) {
             check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(stats)");
@@ -461,7 +508,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             std::memcpy(out + q * out_stride, staged + q * kmax, out_count[q] * sizeof(vdx_neighbor));
         }
         if (d_stats) {
-            for (int i = 0; i < 5; ++i) {
+            for (int i = 0; i < vdx::dev::kBeamStats; ++i) {
                 opts->stats[i] += host_stats[i];
             }
         }
@@ -496,17 +543,17 @@ int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t
         }
         const bool want_stats = opts && opts->stats;
         if (want_stats) {
-            check(hipMemsetAsync(d_stats, 0, 5 * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+            check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
         }
         enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
                      reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
                      want_stats ? d_stats : nullptr, opts, s);
         if (want_stats) {
-            unsigned long long host_stats[5];
+            unsigned long long host_stats[vdx::dev::kBeamStats];
             check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(stats)");
             check(hipStreamSynchronize(s), "hnsw search stats");
-            for (int i = 0; i < 5; ++i) {
+            for (int i = 0; i < vdx::dev::kBeamStats; ++i) {
                 opts->stats[i] += host_stats[i];
             }
         }

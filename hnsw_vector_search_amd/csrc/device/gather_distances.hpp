@@ -95,3 +95,150 @@ __device__ __forceinline__ void distances(const RowSource& a, const std::uint32_
 }
 
 }  // namespace vdx::dev
+
+namespace vdx::dev {
+
+// Cross-lane moves inside a row of 16 lanes (DPP): the value of lane i ^ 1, lane i ^ 2 and
+// lane i + 4 (lanes 12..15 of a row read 0; no caller uses them).
+__device__ __forceinline__ float dpp_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_next4(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x104, 0xF, 0xF, true));
+}
+
+// finish_distance with the partner values taken by DPP instead of LDS permutes. The sums are
+// the same additions in the same association, so the result is bit-identical; only lanes
+// m = 0 and 1 of each group hold meaningful h, and lane m = 0 the distance.
+__device__ __forceinline__ float finish_distance_dpp(bool l2, float4 acc, const float (&qt)[8],
+                                                     const float (&xt)[8], int tail) {
+    // lanes m and m ^ 2: (acc0 + acc1) on m = 0, 1 and (acc2 + acc3) on m = 4, 5
+    const float4 t = make_float4(acc.x + dpp_xor2(acc.x), acc.y + dpp_xor2(acc.y),
+                                 acc.z + dpp_xor2(acc.z), acc.w + dpp_xor2(acc.w));
+    // lanes m and m + 4: v[l] = (acc0 + acc1) + (acc2 + acc3) on m = 0 (l < 4) and 1 (l >= 4)
+    const float4 v = make_float4(t.x + dpp_next4(t.x), t.y + dpp_next4(t.y), t.z + dpp_next4(t.z),
+                                 t.w + dpp_next4(t.w));
+    // lanes m and m ^ 1: h[i] = v[i] + v[i + 4] on m = 0
+    const float4 h = make_float4(v.x + dpp_xor1(v.x), v.y + dpp_xor1(v.y), v.z + dpp_xor1(v.z),
+                                 v.w + dpp_xor1(v.w));
+    float sum = (h.x + h.z) + (h.y + h.w);
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        if (e < tail) {
+            if (l2) {
+                const float d = qt[e] - xt[e];
+                const float sq = d * d;
+                sum = sum + sq;
+            } else {
+                const float p = qt[e] * xt[e];
+                sum = sum + p;
+            }
+        }
+    }
+    return l2 ? sum : 1.0f - sum;
+}
+
+// distances() for the register-resident beam: the candidates' ids are read from LDS first,
+// then every row's loads are issued under one predicate per row, then the arithmetic runs with
+// DPP reductions. FIXED: the runtime shape equals NB/N8/TAIL (dimensions 128 and 100), so
+// every block test folds away.
+template <int NB, int N8, int TAIL, int R, bool FIXED>
+__device__ __forceinline__ void distances_rows(const RowSource& a, const std::uint32_t* ids,
+                                               std::uint32_t cnt, const float4 (&qb)[NB > 0 ? NB : 1],
+                                               const float4 (&q8)[N8 > 0 ? N8 : 1],
+                                               const float (&qt)[8], int nb, int n8, int tail,
+                                               int first_tail, float* out) {
+    const int lane = lane_id();
+    const int g = lane >> 3;
+    const int m = lane & 7;
+    if (FIXED) {
+        nb = NB;
+        n8 = N8;
+        tail = TAIL;
+    }
+    std::uint32_t rid[R];
+    bool ok[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const std::uint32_t c = static_cast<std::uint32_t>(r * 8 + g);
+        ok[r] = c < cnt;
+        rid[r] = ok[r] ? ids[c] : 0u;
+    }
+    float4 x[R][NB > 0 ? NB : 1];
+    float4 x8[R][N8 > 0 ? N8 : 1];
+    float xt[R][8];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float* row = a.vectors + static_cast<std::size_t>(rid[r]) * a.ld;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            x[r][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < N8; ++t) {
+            x8[r][t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            xt[r][e] = 0.f;
+        }
+        if (ok[r]) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                if (FIXED || i < nb) {
+                    x[r][i] = *reinterpret_cast<const float4*>(row + 32 * i + 4 * m);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < N8; ++t) {
+                if ((FIXED || t < n8) && m < 2) {
+                    x8[r][t] = *reinterpret_cast<const float4*>(row + 32 * nb + 8 * t + 4 * m);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (e < TAIL && (FIXED || e < tail) && m == 0) {
+                    xt[r][e] = row[first_tail + e];
+                }
+            }
+        }
+    }
+    const bool l2 = a.l2 != 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (static_cast<std::uint32_t>(r * 8) < cnt) {
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                if (FIXED || i < nb) {
+                    if (l2) {
+                        const float4 d = f4_sub(qb[i], x[r][i]);
+                        acc = f4_fma(d, d, acc);
+                    } else {
+                        acc = f4_fma(qb[i], x[r][i], acc);
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < N8; ++t) {
+                if (FIXED || t < n8) {
+                    if (l2) {
+                        const float4 d = f4_sub(q8[t], x8[r][t]);
+                        acc = f4_fma(d, d, acc);
+                    } else {
+                        acc = f4_fma(q8[t], x8[r][t], acc);
+                    }
+                }
+            }
+            const float d = finish_distance_dpp(l2, acc, qt, xt[r], TAIL > 0 ? tail : 0);
+            if (m == 0 && ok[r]) {
+                out[r * 8 + g] = d;
+            }
+        }
+    }
+}
+
+}  // namespace vdx::dev

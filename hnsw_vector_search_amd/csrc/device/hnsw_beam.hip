@@ -184,15 +184,21 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
 
     unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0;
 
+    std::uint32_t begin = 0, end = a.nq;
+    if (a.range != nullptr) {
+        begin = uniform(a.range[0]);
+        end = uniform(a.range[1]);
+    }
     for (;;) {
-        std::uint32_t q = 0;
+        std::uint32_t t = 0;
         if (lane == 0) {
-            q = atomicAdd(a.queue, 1u);
+            t = atomicAdd(a.queue, 1u);
         }
-        q = uniform(broadcast(q, 0));
-        if (q >= a.nq) {
+        t = uniform(broadcast(t, 0));
+        if (t >= end - begin) {
             break;
         }
+        const std::uint32_t q = a.order != nullptr ? uniform(a.order[begin + t]) : t;
         // The host sized R and the output rows; clamping keeps a caller's larger per-query
         // value from reaching past them.
         const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
@@ -478,6 +484,32 @@ hipError_t launch_hnsw_beam(const BeamArgs& args, unsigned grid, unsigned waves_
     void* params[] = {const_cast<BeamArgs*>(&args)};
     return hipLaunchKernel(hnsw_beam_kernel_ptr(shape, mode), dim3(grid), dim3(waves_per_group * kWave),
                            params, lds_bytes, stream);
+}
+
+}  // namespace vdx::dev
+
+namespace vdx::dev {
+
+const void* hnsw_beam_reg_ptr_c1(BeamShape shape, int mode);
+const void* hnsw_beam_reg_ptr_c2(BeamShape shape, int mode);
+const void* hnsw_beam_reg_ptr_c4(BeamShape shape, int mode);
+const void* hnsw_beam_reg_ptr_c8(BeamShape shape, int mode);
+
+const void* hnsw_beam_reg_kernel_ptr(BeamShape shape, int mode, int chunks) {
+    switch (chunks) {
+        case 1: return hnsw_beam_reg_ptr_c1(shape, mode);
+        case 2: return hnsw_beam_reg_ptr_c2(shape, mode);
+        case 4: return hnsw_beam_reg_ptr_c4(shape, mode);
+        default: return hnsw_beam_reg_ptr_c8(shape, mode);
+    }
+}
+
+hipError_t launch_hnsw_beam_reg(const BeamArgs& args, unsigned grid, unsigned waves_per_group,
+                                std::size_t lds_bytes, hipStream_t stream, BeamShape shape, int mode,
+                                int chunks) {
+    void* params[] = {const_cast<BeamArgs*>(&args)};
+    return hipLaunchKernel(hnsw_beam_reg_kernel_ptr(shape, mode, chunks), dim3(grid),
+                           dim3(waves_per_group * kWave), params, lds_bytes, stream);
 }
 
 }  // namespace vdx::dev

@@ -1,0 +1,539 @@
+#pragma once
+
+// K1 for ef' = max(ef, k) <= 512: the HNSW query path with the result list R in registers.
+//
+// Same algorithm, same results as hnsw_beam.hip (vecdex src/hnsw.cpp:504-522: greedy descent,
+// layer-0 beam, closest min(k, |results|) in `closer` order; bit-identical to the CPU), laid out
+// for the wavefront instead of LDS:
+//
+//  * R, the sorted best ef' keys, lives in CH chunks of 64 keys, lane l of chunk c holding
+//    R[64c + l] (entries past the end hold an all-ones sentinel whose "expanded" bit is set).
+//    The closest unexpanded member is one ballot per chunk.
+//  * Merging one expansion's fresh keys: keys that cannot enter (R full and not closer than its
+//    worst) drop at once; for each remaining key j (a scalar loop over a ballot mask), ballots
+//    over R count the members below it and find an exact duplicate (a node the lossy visited
+//    cache forgot), and every lane counts how many inserted keys fall below its own R member and
+//    its own fresh key. Those counts are the final positions: R and the new keys are scattered
+//    to an LDS buffer by position and read back as chunks.
+//  * The link block of the next expansion is prefetched: when an expansion starts, the block of
+//    the next unexpanded member of R is loaded alongside this expansion's rows; after the merge
+//    ranks, if a fresh key lands ahead of that member, its block is loaded instead. The next
+//    expansion uses the prefetched block only if it names the node it selects.
+//  * Row distances reduce across the 8 lanes of a candidate with DPP moves (same additions,
+//    same association as the reference's AVX2 horizontal sum).
+//
+// LDS per wave: the merge buffer (8 B x 64 CH), fresh ids and distances (64 each), features,
+// the visited cache.
+
+#include <hip/hip_runtime.h>
+
+#include "beam_args.hpp"
+#include "device_common.hpp"
+#include "gather_distances.hpp"
+#include "mlp_stop.hpp"
+
+namespace vdx::dev {
+
+namespace reg {
+
+constexpr std::uint32_t kNone = 0xFFFFFFFFu;
+constexpr std::uint64_t kSentinel = ~static_cast<std::uint64_t>(0);
+
+__device__ __forceinline__ std::uint32_t uniform(std::uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Lane l's 64-bit value, l uniform.
+__device__ __forceinline__ std::uint64_t readlane64(std::uint64_t v, std::uint32_t l) {
+    const std::uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<std::uint32_t>(v)), l);
+    const std::uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<std::uint32_t>(v >> 32)), l);
+    return (static_cast<std::uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ bool is_open(std::uint64_t key) {
+    return (static_cast<std::uint32_t>(key) & kExpanded) == 0;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fminf(v, __shfl_xor(v, off, kWave));
+    }
+    return v;
+}
+
+// R[i] for a uniform index i.
+template <int CH>
+__device__ __forceinline__ std::uint64_t r_at(const std::uint64_t (&Rk)[CH], std::uint32_t i) {
+    std::uint64_t v = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        if (static_cast<std::uint32_t>(c) == (i >> 6)) {
+            v = readlane64(Rk[c], i & 63u);
+        }
+    }
+    return v;
+}
+
+// The first open member of R at or after chunk `from`: (chunk, lane), chunk -1 if none.
+template <int CH>
+__device__ __forceinline__ void first_open(const std::uint64_t (&Rk)[CH], int from, int& pc,
+                                           std::uint32_t& pl) {
+    pc = -1;
+    pl = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        if (pc < 0 && c >= from) {
+            const std::uint64_t bal = ballot(is_open(Rk[c]));
+            if (bal != 0) {
+                pc = c;
+                pl = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(bal)) - 1);
+            }
+        }
+    }
+}
+
+}  // namespace reg
+
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE>
+__global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
+    using namespace reg;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int m = lane & 7;
+    unsigned char* base = lds + static_cast<std::size_t>(wave) * a.lds_per_wave;
+    std::uint64_t* B = reinterpret_cast<std::uint64_t*>(base);
+    std::uint32_t* fid = reinterpret_cast<std::uint32_t*>(B + 64 * CH);
+    float* fd = reinterpret_cast<float*>(fid + 64);
+    float* feat = fd + 64;
+    std::uint32_t* vis = reinterpret_cast<std::uint32_t*>(feat + 16);
+    MlpWeights* model = nullptr;
+    if constexpr (MODE == kBeamLearned) {
+        model = reinterpret_cast<MlpWeights*>(lds + static_cast<std::size_t>(blockDim.x >> 6) * a.lds_per_wave);
+        const std::uint32_t* src_words = static_cast<const std::uint32_t*>(a.mlp);
+        std::uint32_t* dst_words = reinterpret_cast<std::uint32_t*>(model);
+        for (std::uint32_t i = threadIdx.x; i < sizeof(MlpWeights) / 4; i += blockDim.x) {
+            dst_words[i] = src_words[i];
+        }
+        __syncthreads();
+    }
+    const std::uint32_t vslots = 1u << a.visited_log2;
+    const std::uint32_t vshift = 32u - a.visited_log2;
+    const int nb = FIXED ? NB : static_cast<int>(a.nb);
+    const int n8 = FIXED ? N8 : static_cast<int>(a.n8);
+    const int tail = FIXED ? TAIL : static_cast<int>(a.tail);
+    const int first_tail = static_cast<int>(a.first_tail);
+    const RowSource src{a.vectors, a.ld, a.l2};
+    const std::uint32_t b0 = a.b0;
+
+    std::uint32_t begin = 0, end = a.nq;
+    if (a.range != nullptr) {
+        begin = uniform(a.range[0]);
+        end = uniform(a.range[1]);
+    }
+
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0;
+
+    for (;;) {
+        std::uint32_t t = 0;
+        if (lane == 0) {
+            t = atomicAdd(a.queue, 1u);
+        }
+        t = uniform(broadcast(t, 0));
+        if (t >= end - begin) {
+            break;
+        }
+        const std::uint32_t q = a.order != nullptr ? uniform(a.order[begin + t]) : t;
+        const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
+        const std::uint32_t ef = uniform(a.efs ? a.efs[q] : a.ef);
+        const std::uint32_t efp = min(max(ef, k), static_cast<std::uint32_t>(64 * CH));
+        Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
+
+        // The query, spread over the 8 lanes of each group (every group holds a full copy).
+        const float* qp = a.queries + static_cast<std::size_t>(q) * a.qld;
+        float4 qb[NB > 0 ? NB : 1];
+        float4 q8[N8 > 0 ? N8 : 1];
+        float qt[8];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            qb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (FIXED || i < nb) {
+                const float* p = qp + 32 * i + 4 * m;
+                qb[i] = make_float4(p[0], p[1], p[2], p[3]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < N8; ++s) {
+            q8[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((FIXED || s < n8) && m < 2) {
+                const float* p = qp + 32 * nb + 8 * s + 4 * m;
+                q8[s] = make_float4(p[0], p[1], p[2], p[3]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            qt[e] = (e < TAIL && (FIXED || e < tail)) ? qp[first_tail + e] : 0.f;
+        }
+
+        // Entry point and greedy descent (src/hnsw.cpp:513-516, 416-431).
+        std::uint32_t cur = a.entry_point;
+        if (lane == 0) {
+            fid[0] = cur;
+        }
+        wave_lds_fence();
+        distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid, 1, qb, q8, qt, nb, n8, tail, first_tail, fd);
+        wave_lds_fence();
+        float cur_d = fd[0];
+        ++n_dist;
+        for (std::uint32_t layer = a.max_level; layer > 0; --layer) {
+            for (;;) {
+                const std::uint32_t* blk =
+                    a.upper + (static_cast<std::size_t>(a.upper_offset[cur]) + (layer - 1)) * a.bu;
+                const std::uint32_t cnt = uniform(blk[0]);
+                std::uint32_t id = 0;
+                if (static_cast<std::uint32_t>(lane) < cnt) {
+                    id = blk[1 + lane];
+                    fid[lane] = id;
+                }
+                ++n_up;
+                n_dist += cnt;
+                wave_lds_fence();
+                for (std::uint32_t c0 = 0; c0 < cnt; c0 += 8 * RR) {
+                    distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid + c0, min(8u * RR, cnt - c0), qb, q8,
+                                                            qt, nb, n8, tail, first_tail, fd + c0);
+                }
+                wave_lds_fence();
+                const float d = static_cast<std::uint32_t>(lane) < cnt ? fd[lane] : __int_as_float(0x7f800000);
+                const float best = wave_min(d);
+                if (!(best < cur_d)) {
+                    break;
+                }
+                // The first neighbour in link order at the minimum: the one the sequential
+                // scan with a strict '<' settles on.
+                const std::uint64_t at = ballot(static_cast<std::uint32_t>(lane) < cnt && d == best);
+                const int first = __ffsll(static_cast<long long>(at)) - 1;
+                cur = uniform(broadcast(id, first));
+                cur_d = best;
+            }
+        }
+
+        // Layer 0 (src/hnsw.cpp:438-482) with ef' = max(ef, k) (src/hnsw.cpp:517).
+        for (std::uint32_t i = static_cast<std::uint32_t>(lane) * 4; i < vslots; i += kWave * 4) {
+            *reinterpret_cast<uint4*>(vis + i) = make_uint4(kNone, kNone, kNone, kNone);
+        }
+        wave_lds_fence();
+        if (lane == 0) {
+            vis[(cur * 2654435761u) >> vshift] = cur;
+        }
+        std::uint64_t Rk[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            Rk[c] = kSentinel;
+        }
+        if (lane == 0) {
+            Rk[0] = make_key(cur_d, cur);
+        }
+        std::uint32_t size = 1;
+        int hint_c = 0;
+        std::uint32_t pref_node = kNone;
+        std::uint32_t pref_cnt = 0, pref_id = 0;
+        // search-state features (trace and learned modes)
+        const float d_scale = fmaxf(fabsf(cur_d), 1e-6f);
+        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0;
+        std::uint32_t truth_id = kNone;
+        if constexpr (MODE == kBeamTrace) {
+            if (lane < 10) {
+                truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
+            }
+        }
+        for (;;) {
+            // The closest unexpanded member of R; mark it expanded.
+            int pc;
+            std::uint32_t pl;
+            first_open<CH>(Rk, hint_c, pc, pl);
+            if (pc < 0) {
+                break;
+            }
+            std::uint64_t kp = 0;
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (c == pc) {
+                    kp = readlane64(Rk[c], pl);
+                    if (static_cast<std::uint32_t>(lane) == pl) {
+                        Rk[c] |= kExpanded;
+                    }
+                }
+            }
+            hint_c = pc;
+            const std::uint32_t p = static_cast<std::uint32_t>(pc) * 64u + pl;
+            const std::uint32_t node = key_id(kp);
+            ++n_exp;
+
+            // Its link block: prefetched, or read now.
+            std::uint32_t cnt, id;
+            if (node == pref_node) {
+                cnt = uniform(pref_cnt);
+                id = pref_id;
+                ++n_pref;
+            } else {
+                const std::uint32_t* blk = a.layer0 + static_cast<std::size_t>(node) * b0;
+                cnt = uniform(blk[0]);
+                id = static_cast<std::uint32_t>(lane) < cnt ? blk[1 + lane] : 0u;
+            }
+            // Prefetch the block of the next open member of R (the next expansion unless a
+            // fresh key lands ahead of it).
+            std::uint64_t spec_key = kSentinel;
+            {
+                int sc;
+                std::uint32_t sl;
+                first_open<CH>(Rk, pc, sc, sl);
+                pref_node = kNone;
+                if (sc >= 0) {
+                    std::uint64_t ks = 0;
+#pragma unroll
+                    for (int c = 0; c < CH; ++c) {
+                        if (c == sc) {
+                            ks = readlane64(Rk[c], sl);
+                        }
+                    }
+                    spec_key = ks & kKeyMask;
+                    pref_node = key_id(ks);
+                    const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(pref_node) * b0;
+                    pref_cnt = nb2[0];
+                    pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
+                }
+            }
+
+            // Fresh neighbours: not in the visited cache.
+            const bool valid = static_cast<std::uint32_t>(lane) < cnt;
+            bool fresh = false;
+            if (valid) {
+                const std::uint32_t slot = (id * 2654435761u) >> vshift;
+                fresh = vis[slot] != id;
+                if (fresh) {
+                    vis[slot] = id;
+                }
+            }
+            const std::uint64_t fb = ballot(fresh);
+            const std::uint32_t nf = static_cast<std::uint32_t>(__popcll(fb));
+            if (fresh) {
+                fid[lanes_below(fb)] = id;
+            }
+            wave_lds_fence();
+            n_dist += nf;
+            for (std::uint32_t f0 = 0; f0 < nf; f0 += 8 * RR) {
+                distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid + f0, min(8u * RR, nf - f0), qb, q8, qt,
+                                                        nb, n8, tail, first_tail, fd + f0);
+            }
+            wave_lds_fence();
+            std::uint64_t f = kSentinel;
+            if (static_cast<std::uint32_t>(lane) < nf) {
+                f = make_key(fd[lane], fid[lane]);
+            }
+
+            // Merge: keys that cannot enter drop first (R full and not closer than its worst).
+            const std::uint64_t worst = r_at<CH>(Rk, size - 1) & kKeyMask;
+            const bool cand = static_cast<std::uint32_t>(lane) < nf && (size < efp || f < worst);
+            std::uint64_t rem = ballot(cand);
+            std::uint32_t gt[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                gt[c] = 0;
+            }
+            std::uint32_t lb = 0, below = 0, nins = 0, fmin_pos = kNone;
+            std::uint64_t dupmask = 0, fmin = kSentinel;
+            while (rem != 0) {
+                const std::uint32_t j = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(rem)) - 1);
+                rem &= rem - 1;
+                const std::uint64_t fj = readlane64(f, j);
+                bool eq = false;
+                std::uint32_t lt = 0;
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    const std::uint64_t rm = Rk[c] & kKeyMask;
+                    eq |= ballot(rm == fj) != 0;
+                    lt += static_cast<std::uint32_t>(__popcll(ballot(rm < fj)));
+                }
+                if (eq) {
+                    dupmask |= 1ull << j;  // already in R: a node the visited cache forgot
+                    continue;
+                }
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    gt[c] += fj < (Rk[c] & kKeyMask) ? 1u : 0u;
+                }
+                below += fj < f ? 1u : 0u;
+                if (static_cast<std::uint32_t>(lane) == j) {
+                    lb = lt;
+                }
+                ++nins;
+                if (fj < fmin) {
+                    fmin = fj;
+                    fmin_pos = lt;
+                }
+            }
+            n_dup += static_cast<unsigned long long>(__popcll(dupmask));
+            const bool kept = cand && ((dupmask >> lane) & 1u) == 0;
+            const std::uint32_t pos = lb + below;
+            const bool ins = kept && pos < efp;
+            std::uint32_t inserted = 0;
+            if (nins > 0) {
+                // A fresh key ahead of the prefetched member is the next expansion: fetch its
+                // block now, while the merge completes.
+                if (fmin_pos < efp && fmin < spec_key) {
+                    pref_node = key_id(fmin);
+                    const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(pref_node) * b0;
+                    pref_cnt = nb2[0];
+                    pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
+                }
+                inserted = static_cast<std::uint32_t>(__popcll(ballot(ins)));
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    const std::uint32_t i = static_cast<std::uint32_t>(c * 64 + lane);
+                    if (i < size && i + gt[c] < efp) {
+                        B[i + gt[c]] = Rk[c];
+                    }
+                }
+                if (ins) {
+                    B[pos] = f;
+                }
+                wave_lds_fence();
+                size = min(efp, size + nins);
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    const std::uint32_t i = static_cast<std::uint32_t>(c * 64 + lane);
+                    Rk[c] = i < size ? B[i] : kSentinel;
+                }
+                wave_lds_fence();
+                if (fmin_pos < efp) {
+                    hint_c = min(hint_c, static_cast<int>(fmin_pos >> 6));
+                }
+            }
+            if constexpr (MODE != kBeamPlain) {
+                ++steps;
+                const std::uint32_t kk = min(k, size);
+                const bool improved = inserted > 0 && fmin_pos < kk;
+                improved_hist = ((improved_hist << 1) | (improved ? 1u : 0u)) & 0xFFu;
+                since_improve = improved ? 0u : since_improve + 1u;
+                const float d_best = key_distance(r_at<CH>(Rk, 0));
+                const float d_k = key_distance(r_at<CH>(Rk, kk - 1));
+                const float d_worst = key_distance(r_at<CH>(Rk, size - 1));
+                const float d_cur = key_distance(kp);
+                // Lane i computes feature i (all inputs are wave-uniform).
+                if (lane < kFeatures) {
+                    const float sk = fmaxf(fabsf(d_k), 1e-6f);
+                    float v;
+                    switch (lane) {
+                        case 0: v = d_best / d_scale; break;
+                        case 1: v = d_k / d_scale; break;
+                        case 2: v = d_cur / sk; break;
+                        case 3: v = d_best / sk; break;
+                        case 4: v = static_cast<float>(steps) / efp; break;
+                        case 5: v = __log2f(1.0f + steps) * 0.125f; break;
+                        case 6: v = static_cast<float>(size) / efp; break;
+                        case 7: v = __popc(improved_hist) * 0.125f; break;
+                        case 8: v = static_cast<float>(p) / efp; break;
+                        case 9: v = static_cast<float>(inserted) / a.M2; break;
+                        case 10: v = static_cast<float>(nf) / a.M2; break;
+                        case 11: v = min(since_improve, 64u) * 0.0625f; break;
+                        case 12: v = efp * (1.0f / 512.0f); break;
+                        case 13: v = k * 0.01f; break;
+                        case 14: v = d_worst / sk; break;
+                        default: v = a.l2 ? 1.0f : 0.0f; break;
+                    }
+                    feat[lane] = v;
+                }
+                wave_lds_fence();
+                if constexpr (MODE == kBeamTrace) {
+                    // top-10 hits of the current result against the ground truth
+                    const std::uint32_t kt = min(10u, size);
+                    const std::uint32_t rid = static_cast<std::uint32_t>(lane) < kt ? key_id(Rk[0]) : 0xFFFFFFFEu;
+                    bool found = false;
+                    for (int j = 0; j < 10; ++j) {
+                        found = found || (static_cast<std::uint32_t>(__shfl(static_cast<int>(truth_id), j, kWave)) == rid);
+                    }
+                    const std::uint32_t hits = static_cast<std::uint32_t>(
+                        __popcll(ballot(found && static_cast<std::uint32_t>(lane) < kt)));
+                    if (steps <= a.trace_cap) {
+                        const std::size_t row = static_cast<std::size_t>(q) * a.trace_cap + (steps - 1);
+                        if (lane < kFeatures) {
+                            a.trace_feat[row * kFeatures + lane] = feat[lane];
+                        }
+                        if (lane == 0) {
+                            a.trace_hits[row] = static_cast<std::uint8_t>(hits);
+                        }
+                    }
+                    if (lane == 0) {
+                        a.trace_final[q] = hits;  // the last step's value is the result's
+                    }
+                }
+                if constexpr (MODE == kBeamLearned) {
+                    if (steps >= a.min_steps && size >= k) {
+                        float x[kFeatures];
+#pragma unroll
+                        for (int i = 0; i < kFeatures; ++i) {
+                            x[i] = mlp_input(*model, i, feat[i]);
+                        }
+                        float h1, h2;
+                        const float z = mlp_forward(*model, x, lane, h1, h2);
+                        if (z > a.stop_logit) {
+                            ++n_stop;
+                            break;  // stop: the model judges further expansions useless
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (MODE == kBeamTrace) {
+            if (lane == 0) {
+                a.trace_steps[q] = steps;
+            }
+        }
+
+        const std::uint32_t count = min(k, size);
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const std::uint32_t i = static_cast<std::uint32_t>(c * 64 + lane);
+            if (i < count) {
+                out[i] = Neighbor{key_distance(Rk[c]), key_id(Rk[c])};
+            }
+        }
+        if (lane == 0) {
+            a.out_count[q] = count;
+        }
+        wave_lds_fence();
+    }
+    if (lane == 0 && a.stats != nullptr) {
+        atomicAdd(a.stats + 0, n_exp);
+        atomicAdd(a.stats + 1, n_dist);
+        atomicAdd(a.stats + 2, n_up);
+        atomicAdd(a.stats + 3, n_dup);
+        atomicAdd(a.stats + 4, n_stop);
+        atomicAdd(a.stats + 5, n_pref);
+    }
+}
+
+// Kernel pointers of one chunk count, by shape and mode.
+template <int CH>
+const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
+#define VDX_REG_KERNEL(MODE)                                                                     \
+    switch (shape) {                                                                             \
+        case BeamShape::D128:                                                                    \
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<4, 0, 0, 4, true, CH, MODE>); \
+        case BeamShape::D100:                                                                    \
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<3, 0, 4, 4, true, CH, MODE>); \
+        default:                                                                                 \
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<8, 3, 7, 1, false, CH, MODE>); \
+    }
+    if (mode == kBeamTrace) {
+        VDX_REG_KERNEL(kBeamTrace)
+    }
+    if (mode == kBeamLearned) {
+        VDX_REG_KERNEL(kBeamLearned)
+    }
+    VDX_REG_KERNEL(kBeamPlain)
+#undef VDX_REG_KERNEL
+}
+
+}  // namespace vdx::dev

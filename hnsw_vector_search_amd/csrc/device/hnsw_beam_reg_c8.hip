@@ -1,0 +1,9 @@
+// Instantiations of the register-resident beam (hnsw_beam_reg.hpp) for R capacity 512.
+
+#include "hnsw_beam_reg.hpp"
+
+namespace vdx::dev {
+
+const void* hnsw_beam_reg_ptr_c8(BeamShape shape, int mode) { return hnsw_beam_reg_ptr<8>(shape, mode); }
+
+}  // namespace vdx::dev

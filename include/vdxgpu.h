@@ -175,6 +175,7 @@ enum vdx_stat {
     VDX_STAT_UPPER_BLOCKS = 2, /* upper-layer link blocks read */
     VDX_STAT_DUPLICATES = 3,   /* recomputed members of the result list the lossy visited cache let through */
     VDX_STAT_EARLY_STOPS = 4,  /* queries the learned model stopped before the fixed-ef stop */
+    VDX_STAT_LINK_PREFETCHED = 5, /* layer-0 link blocks that were already prefetched when needed */
     VDX_STAT_COUNT = 8
 };
 
