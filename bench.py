@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_TFLOPS = 157.3
 BF16_MFMA_TFLOPS = 2500.0
 EF_SWEEP = (16, 32, 64, 128, 512)
-EF_FINE = (40, 48, 56)
+EF_FINE = (34, 36, 38, 40, 44, 48, 56)
 TARGET_RECALL = 0.95
 K = 10
 CACHE = Path(os.environ.get("VDX_BENCH_CACHE", "/tmp/vdx_bench_cache"))
@@ -138,6 +138,19 @@ def threshold_recall(found, truth):
     return r.mean_threshold_recall_at_k(found, truth["distance"][:, K - 1], K)
 
 
+def measured_traffic(mode, ef, nq):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), when it covers this
+    workload; else None."""
+    path = ROOT / "profiles" / "hnsw_beam_traffic.json"
+    if not path.exists():
+        return None
+    for row in json.loads(path.read_text()).get("workloads", []):
+        if row.get("mode") == mode and row.get("ef") == ef and row.get("nq") == nq:
+            return row.get("traffic_bytes")
+    return None
+
+
 def prepared(nat, queries, cosine):
     """Queries as the device entry points take them: unit length under cosine (detail::normalize)."""
     q = np.array(queries, dtype=np.float32, order="C", copy=True)
@@ -166,6 +179,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run the library's copy kernel once over this many MiB (counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (HNSW search "
@@ -236,10 +251,19 @@ def main(argv=None) -> int:
 
     # Learned termination.
     learned_leg = None
-    headline = {"mode": "fixed-ef", "ef": fixed["ef"], "opts": None, "recall": fixed["recall"]}
+    headline = {"mode": "fixed-ef", "ef": fixed_best["ef"], "opts": None, "recall": fixed_best["recall"],
+                "qps": fixed_best["qps"]}
     if not args.no_learned:
         learned_leg, headline = run_learned(args, vdx, nat, datasets, learned, index, base, queries,
                                             truth, mix, threads, fixed, fixed_best, headline)
+
+    if args.copy_probe_mb > 0:
+        nbytes = args.copy_probe_mb << 20
+        src_buf = nat.DeviceArray(nbytes // 4, np.uint32, run.ctx)
+        dst_buf = nat.DeviceArray(nbytes // 4, np.uint32, run.ctx)
+        nat.check(nat.lib.vdx_gpu_copy_device(run.ctx.handle, dst_buf.ptr, src_buf.ptr, nbytes))
+        log(f"copy probe: {nbytes} bytes copied by the library's copy kernel")
+        del src_buf, dst_buf
 
     # Headline timing: exactly `steps` launches.
     ms, wall_ms = run.timed(headline["ef"], args.steps, args.warmup, headline["opts"])
@@ -262,7 +286,7 @@ def main(argv=None) -> int:
     if not args.no_cpu:
         from oracle import oracle as o
         g = index.graph_arrays()
-        ef = fixed["ef"]
+        ef = fixed_best["ef"]
         o.hnsw_search(g, queries, K, ef, threads=threads)
         times = []
         for _ in range(5):
@@ -274,7 +298,7 @@ def main(argv=None) -> int:
         gpu_fixed = run.results()
         identical = float(np.mean([gpu_fixed[q].tobytes() == ref[q].tobytes() for q in range(nq)]))
         cpu = {"value": round(nq / med, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"all {nq} queries at fixed ef={ef} (recall {fixed['recall']}), warmup + "
+               "sample": f"all {nq} queries at fixed ef={ef} (recall {fixed_best['recall']}), warmup + "
                          f"median of 5 passes (vecdex bench/hnsw_eval.cpp:493-512), "
                          f"{os.cpu_count()} host cpus, {threads} threads"}
         log(f"CPU baseline: {nq / med:,.0f} QPS on {threads} threads; GPU fixed-ef results identical "
@@ -299,7 +323,8 @@ def main(argv=None) -> int:
                    "ef": headline["ef"], "termination": headline["mode"],
                    "recall@10": round(rec, 4), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(headline["mode"], headline["ef"], nq),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "fixed_ef": {"ef": fixed["ef"], "recall": fixed["recall"], "qps": fixed["qps"],
@@ -406,8 +431,9 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
     if l2 is not None and l2["eval_recall"] >= TARGET_RECALL:
         leg["speedup_vs_fixed_ef"] = round(l2["qps"] / fixed["qps"], 3)
         leg["speedup_vs_best_fine_fixed_ef"] = round(l2["qps"] / fixed_best["qps"], 3)
-        headline = {"mode": "learned", "ef": l2["ef_cap"], "opts": l2["_opts"],
-                    "recall": l2["eval_recall"], "model": model}  # the model must stay alive
+        if l2["qps"] > headline["qps"]:
+            headline = {"mode": "learned", "ef": l2["ef_cap"], "opts": l2["_opts"], "qps": l2["qps"],
+                        "recall": l2["eval_recall"], "model": model}  # the model must stay alive
     return leg, headline
 
 

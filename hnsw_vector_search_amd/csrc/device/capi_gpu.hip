@@ -17,6 +17,7 @@
 #include "beam_args.hpp"
 #include "bf_args.hpp"
 #include "bf_host.hpp"
+#include "fill_copy.hpp"
 #include "gpu_index.hpp"
 #include "gpu_state.hpp"
 
@@ -264,6 +265,18 @@ int vdx_gpu_copy_to_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_
         require(ctx != nullptr, "vdx_gpu_copy_to_device: null context");
         DeviceGuard guard(ctx->device);
         check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int vdx_gpu_copy_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr, "vdx_gpu_copy_device: null context");
+        require(bytes == 0 || (dst != nullptr && src != nullptr), "vdx_gpu_copy_device: null pointer");
+        require(reinterpret_cast<std::uintptr_t>(dst) % 16 == 0 && reinterpret_cast<std::uintptr_t>(src) % 16 == 0,
+                "vdx_gpu_copy_device: pointers must be 16-byte aligned");
+        DeviceGuard guard(ctx->device);
+        check(vdx::dev::launch_copy_bytes(dst, src, bytes, ctx->stream), "copy_bytes launch");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }

@@ -1,0 +1,15 @@
+#pragma once
+
+// Fill and copy kernels (fill_copy.hip).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vdx::dev {
+
+hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_t count, hipStream_t s);
+// dst and src 16-byte aligned (device allocations are).
+hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
+
+}  // namespace vdx::dev

@@ -143,6 +143,9 @@ VDX_API int vdx_gpu_alloc(vdx_gpu_ctx* ctx, uint64_t bytes, void** out);
 VDX_API void vdx_gpu_free(vdx_gpu_ctx* ctx, void* ptr);
 VDX_API int vdx_gpu_copy_to_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 VDX_API int vdx_gpu_copy_to_host(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy by a plain kernel (16-byte aligned pointers): also the reference
+ * transfer of known size for calibrating HBM traffic counters. No counterpart in vecdex. */
+VDX_API int vdx_gpu_copy_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 VDX_API int vdx_gpu_synchronize(vdx_gpu_ctx* ctx);
 /* Records event `slot` (0..15) on the context's stream; elapsed milliseconds between two
  * recorded events once both completed. */
