@@ -52,6 +52,11 @@ class SearchOpts(ctypes.Structure):
                 ("reserved_f", c_float), ("stats", c_ptr)]
 
 
+class BatchClass(ctypes.Structure):
+    _fields_ = [("ef_cap", c_u32), ("queries", c_u32), ("visited_log2", c_u32),
+                ("waves_per_group", c_u32), ("blocks_per_cu", c_u32), ("ms", c_float)]
+
+
 class BfStats(ctypes.Structure):
     _fields_ = [("shortlist", c_u64), ("widened", c_u64), ("exact_fallback", c_u64),
                 ("fp64_ties", c_u64), ("max_abs_error", c_double)]
@@ -106,6 +111,14 @@ _SIGS = {
     "vdx_gpu_mlp_infer": (c_int, [c_ptr, c_ptr, c_u64, c_ptr]),
     "vdx_gpu_hnsw_trace_device": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr, c_u32, c_ptr,
                                           c_ptr, c_ptr, c_ptr]),
+    "vdx_gpu_hnsw_batch_create": (c_int, [c_ptr, c_u64, c_u32, c_u32, ctypes.POINTER(SearchOpts), c_ptr]),
+    "vdx_gpu_hnsw_batch_free": (None, [c_ptr]),
+    "vdx_gpu_hnsw_batch_tune": (c_int, [c_ptr, c_ptr, c_u64, c_ptr, c_u32, c_ptr, c_u32, c_ptr, c_u64,
+                                        c_ptr, c_u32]),
+    "vdx_gpu_hnsw_batch_run": (c_int, [c_ptr, c_ptr, c_u64, c_ptr, c_u32, c_ptr, c_u32, c_ptr, c_u64,
+                                       c_ptr, c_int, c_ptr]),
+    "vdx_gpu_hnsw_batch_classes": (c_int, [c_ptr, ctypes.POINTER(BatchClass), c_u32,
+                                           ctypes.POINTER(c_u32)]),
     "vdx_gpu_bf_upload": (c_int, [c_ptr, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
     "vdx_gpu_bf_free": (None, [c_ptr]),
     "vdx_gpu_bf_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u32, c_u32, c_ptr, c_u64,

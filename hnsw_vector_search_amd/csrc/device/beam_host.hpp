@@ -1,0 +1,51 @@
+#pragma once
+
+// Host-side planning and launch of the HNSW beam kernels, shared by the search entry points
+// and the batch planner.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../../include/vdxgpu.h"
+#include "beam_args.hpp"
+#include "gpu_index.hpp"
+
+namespace vdx::gpu {
+
+// Launch shape choices; 0 keeps the default.
+struct BeamConfig {
+    std::uint32_t visited_log2 = 0;   // visited-cache slots per wave, log2
+    std::uint32_t waves = 0;          // wavefronts per workgroup (1..4)
+    std::uint32_t blocks_per_cu = 0;  // resident workgroups per CU to fill (queries in flight)
+};
+
+struct BeamPlan {
+    std::uint32_t ef_cap, visited_log2, waves, grid;
+    int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
+    std::size_t lds_per_wave, lds_bytes;
+};
+
+vdx::dev::BeamShape beam_shape(std::uint64_t dim);
+
+// The kernel mode an options struct asks for (validating a learned model's settings).
+int beam_mode(const vdx_search_opts* opts, bool trace);
+
+// R capacity and kernel for a launch whose largest max(ef, k) is max_efp: the register kernel
+// with the smallest power-of-two chunk count that holds it, else the LDS kernel.
+BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t nq, const BeamConfig& cfg,
+                   int mode);
+
+// Kernel arguments for one launch over prepared device queries.
+vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode,
+                             const vdx_search_opts* opts, const float* d_queries, std::uint64_t nq,
+                             const std::uint32_t* d_ks, std::uint32_t k, const std::uint32_t* d_efs,
+                             std::uint32_t ef, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
+                             std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats);
+
+void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vdx::dev::BeamArgs& a,
+                 hipStream_t stream);
+
+BeamConfig config_from(const vdx_search_opts* opts);
+
+}  // namespace vdx::gpu

@@ -15,6 +15,7 @@
 #include "../host/capi_common.hpp"
 #include "../host/validate.hpp"
 #include "beam_args.hpp"
+#include "beam_host.hpp"
 #include "bf_args.hpp"
 #include "bf_host.hpp"
 #include "fill_copy.hpp"
@@ -35,12 +36,6 @@ constexpr std::uint32_t kMaxBeamEf = 8192;   // R capacity per wave in LDS
 constexpr std::uint32_t kMaxGpuM = 32;       // a layer-0 block fits one wave
 constexpr std::uint32_t kMaxBfK = 2048;
 constexpr std::uint32_t kRingSlots = 64;
-
-vdx::dev::BeamShape beam_shape(std::uint64_t dim) {
-    if (dim == 128) return vdx::dev::BeamShape::D128;
-    if (dim == 100) return vdx::dev::BeamShape::D100;
-    return vdx::dev::BeamShape::Generic;
-}
 
 int bf_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 2); }
 
@@ -63,59 +58,6 @@ void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::u
     }
 }
 
-std::uint32_t default_visited_log2(std::uint32_t ef_cap) {
-    if (ef_cap <= 128) return 11;
-    if (ef_cap <= 512) return 12;
-    return 13;
-}
-
-struct BeamPlan {
-    std::uint32_t ef_cap, visited_log2, waves, grid;
-    int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
-    std::size_t lds_per_wave, lds_bytes;
-};
-
-// R capacity and kernel for a batch whose largest max(ef, k) is max_efp: the register kernel
-// with the smallest power-of-two chunk count that holds it, else the LDS kernel.
-BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t nq,
-                   const vdx_search_opts* opts, int mode) {
-    BeamPlan p{};
-    const std::uint32_t need = (max_efp + 63u) / 64u;
-    if (need * 64u <= vdx::dev::kRegMaxEf) {
-        p.chunks = 1;
-        while (static_cast<std::uint32_t>(p.chunks) < need) {
-            p.chunks *= 2;
-        }
-        p.ef_cap = 64u * static_cast<std::uint32_t>(p.chunks);
-    } else {
-        p.chunks = 0;
-        p.ef_cap = need * 64u;
-    }
-    p.visited_log2 = opts && opts->visited_log2 ? opts->visited_log2 : default_visited_log2(p.ef_cap);
-    require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
-    p.waves = opts && opts->waves_per_group ? opts->waves_per_group : 4;
-    require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
-    p.lds_per_wave = p.chunks > 0 ? vdx::dev::beam_reg_lds_per_wave(p.ef_cap, p.visited_log2)
-                                  : vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
-    const std::size_t limit = h.ctx->lds_per_block;
-    const std::size_t shared = vdx::dev::beam_lds_shared(mode);
-    while (p.waves > 1 && p.lds_per_wave * p.waves + shared > limit) {
-        --p.waves;
-    }
-    p.lds_bytes = p.lds_per_wave * p.waves + shared;
-    require(p.lds_bytes <= limit, "ef too large for the GPU beam's LDS");
-    const void* kernel = p.chunks > 0 ? vdx::dev::hnsw_beam_reg_kernel_ptr(beam_shape(h.dim), mode, p.chunks)
-                                      : vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim), mode);
-    int per_cu = 0;
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, p.waves * 64, p.lds_bytes),
-          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    per_cu = std::max(per_cu, 1);
-    const std::uint64_t resident = static_cast<std::uint64_t>(per_cu) * h.ctx->cu_count;
-    const std::uint64_t needed = (nq + p.waves - 1) / p.waves;
-    p.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, std::min(resident, needed)));
-    return p;
-}
-
 // Queues one beam launch over prepared device queries.
 void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, const std::uint32_t* d_ks,
                   std::uint32_t k, const std::uint32_t* d_efs, std::uint32_t ef,
@@ -123,53 +65,10 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
                   const vdx::dev::BeamArgs* trace = nullptr) {
-    require(nq <= 0xFFFFFFFFull, "too many queries in one call");
-    int mode = vdx::dev::kBeamPlain;
-    if (trace != nullptr) {
-        mode = vdx::dev::kBeamTrace;
-    } else if (opts != nullptr && opts->termination == 1) {
-        mode = vdx::dev::kBeamLearned;
-        require(opts->mlp != nullptr, "learned termination needs a model (opts.mlp)");
-        require(opts->mlp_threshold > 0.f && opts->mlp_threshold < 1.f,
-                "learned termination needs a threshold in (0, 1)");
-    } else if (opts != nullptr && opts->termination > 1) {
-        throw std::invalid_argument("unknown termination mode");
-    }
-    const BeamPlan plan = plan_beam(h, max_efp, nq, opts, mode);
-    vdx::dev::BeamArgs a{};
-    a.vectors = h.vectors.as<float>();
-    a.layer0 = h.layer0.as<std::uint32_t>();
-    a.upper = h.upper.as<std::uint32_t>();
-    a.upper_offset = h.upper_offset.as<std::uint32_t>();
-    a.ld = h.split.ld;
-    a.dim = static_cast<std::uint32_t>(h.dim);
-    a.b0 = static_cast<std::uint32_t>(1 + 2 * h.M);
-    a.bu = static_cast<std::uint32_t>(1 + h.M);
-    a.n = static_cast<std::uint32_t>(h.n);
-    a.max_level = static_cast<std::uint32_t>(h.max_level);
-    a.entry_point = static_cast<std::uint32_t>(h.entry_point);
-    a.l2 = h.metric == 0 ? 1u : 0u;
-    a.nb = h.split.nb;
-    a.n8 = h.split.n8;
-    a.tail = h.split.tail;
-    a.first_tail = h.split.first_tail;
-    a.queries = d_queries;
-    a.qld = static_cast<std::uint32_t>(h.dim);
-    a.nq = static_cast<std::uint32_t>(nq);
-    a.ks = d_ks;
-    a.efs = d_efs;
-    a.k = k;
-    a.ef = ef;
-    a.out = d_out;
-    a.out_stride = out_stride;
-    a.out_count = d_count;
-    a.k_cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(out_stride, plan.ef_cap));
-    a.queue = d_queue;
-    a.ef_cap = plan.ef_cap;
-    a.visited_log2 = plan.visited_log2;
-    a.lds_per_wave = static_cast<std::uint32_t>(plan.lds_per_wave);
-    a.stats = d_stats;
-    a.M2 = static_cast<std::uint32_t>(2 * h.M);
+    const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
+    const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
+    vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
+                                               out_stride, d_count, d_queue, d_stats);
     if (trace != nullptr) {
         a.truth = trace->truth;
         a.trace_feat = trace->trace_feat;
@@ -178,23 +77,8 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
         a.trace_final = trace->trace_final;
         a.trace_cap = trace->trace_cap;
     }
-    if (mode == vdx::dev::kBeamLearned) {
-        const auto* model = static_cast<const vdx_gpu_mlp*>(opts->mlp);
-        a.mlp = model->weights.as<void>();
-        const float t = opts->mlp_threshold;
-        a.stop_logit = std::log(t / (1.0f - t));
-        a.min_steps = opts->min_steps;
-    }
     check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
-    if (plan.chunks > 0) {
-        check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                             beam_shape(h.dim), mode, plan.chunks),
-              "hnsw_beam_reg launch");
-    } else {
-        check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                         beam_shape(h.dim), mode),
-              "hnsw_beam launch");
-    }
+    vdx::gpu::launch_beam(h, plan, mode, a, stream);
 }
 
 // Validates and prepares nq host queries into pinned staging (rows of dim floats).

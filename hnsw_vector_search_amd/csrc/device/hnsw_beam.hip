@@ -415,13 +415,9 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
                 }
                 if constexpr (MODE == kBeamLearned) {
                     if (steps >= a.min_steps && size >= k) {
-                        float x[kFeatures];
-#pragma unroll
-                        for (int i = 0; i < kFeatures; ++i) {
-                            x[i] = mlp_input(*model, i, feat[i]);
-                        }
                         float h1, h2;
-                        const float z = mlp_forward(*model, x, lane, h1, h2);
+                        const float z = mlp_forward_raw(*model, feat, lane, h1, h2,
+                                                        reinterpret_cast<std::uint16_t*>(fd));
                         if (z > a.stop_logit) {
                             ++n_stop;
                             break;  // stop: the model judges further expansions useless

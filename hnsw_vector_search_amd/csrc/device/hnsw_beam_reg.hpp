@@ -96,7 +96,7 @@ __device__ __forceinline__ void first_open(const std::uint64_t (&Rk)[CH], int fr
 }  // namespace reg
 
 template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE>
-__global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
+__device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
     using namespace reg;
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -470,13 +470,9 @@ __global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
                 }
                 if constexpr (MODE == kBeamLearned) {
                     if (steps >= a.min_steps && size >= k) {
-                        float x[kFeatures];
-#pragma unroll
-                        for (int i = 0; i < kFeatures; ++i) {
-                            x[i] = mlp_input(*model, i, feat[i]);
-                        }
                         float h1, h2;
-                        const float z = mlp_forward(*model, x, lane, h1, h2);
+                        const float z = mlp_forward_raw(*model, feat, lane, h1, h2,
+                                                        reinterpret_cast<std::uint16_t*>(fd));
                         if (z > a.stop_logit) {
                             ++n_stop;
                             break;  // stop: the model judges further expansions useless
@@ -514,6 +510,20 @@ __global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
     }
 }
 
+// The kernels. The plain search is held to 128 VGPRs (4 waves per SIMD): the occupancy it gains
+// outweighs the few spilled registers at R capacities above 64 (measured), while the trace and
+// learned modes keep their registers.
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE>
+__global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
+    beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, MODE>(a);
+}
+
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void hnsw_beam_reg_plain_kernel(
+    BeamArgs a) {
+    beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain>(a);
+}
+
 // Kernel pointers of one chunk count, by shape and mode.
 template <int CH>
 const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
@@ -532,8 +542,15 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
     if (mode == kBeamLearned) {
         VDX_REG_KERNEL(kBeamLearned)
     }
-    VDX_REG_KERNEL(kBeamPlain)
 #undef VDX_REG_KERNEL
+    switch (shape) {
+        case BeamShape::D128:
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<4, 0, 0, 4, true, CH>);
+        case BeamShape::D100:
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<3, 0, 4, 4, true, CH>);
+        default:
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<8, 3, 7, 1, false, CH>);
+    }
 }
 
 }  // namespace vdx::dev

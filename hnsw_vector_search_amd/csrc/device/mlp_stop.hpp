@@ -1,8 +1,8 @@
 #pragma once
 
 // The learned stop model: an MLP 16 -> 64 -> 64 -> 1 with ReLU and a sigmoid output, run in
-// BF16 with FP32 accumulation. Inputs, weights and hidden activations are BF16 values; every
-// sum is accumulated in FP32. The same device functions serve inference inside the beam kernel
+// BF16 with FP32 accumulation. Inputs, weights and hidden activations are BF16 values; the
+// layer sums are chains of two-term BF16 dot products accumulated in FP32 (v_dot2_f32_bf16). The same device functions serve inference inside the beam kernel
 // and the forward pass of training, so both see exactly the same arithmetic.
 //
 // Parameter vector (FP32, the C ABI's order; kMlpParams values):
@@ -24,7 +24,9 @@ constexpr int kOffMean = 0, kOffInvStd = 16, kOffW1 = 32, kOffB1 = kOffW1 + 16 *
               kOffW2 = kOffB1 + 64, kOffB2 = kOffW2 + 64 * 64, kOffW3 = kOffB2 + 64,
               kOffB3 = kOffW3 + 64;
 
-// The device copy used for inference and the forward pass: BF16 weights, FP32 biases.
+// The device copy used for inference and the forward pass: BF16 weights, FP32 biases. W1 and
+// W2 are stored as BF16 pairs along the input index, w1p[p * 64 + u] = (W[2p][u] low half,
+// W[2p + 1][u] high half), so one 32-bit word feeds one two-term BF16 dot product.
 struct MlpWeights {
     float mean[16];
     float inv_std[16];
@@ -32,10 +34,13 @@ struct MlpWeights {
     float b2[64];
     float b3;
     float pad[3];
-    std::uint16_t w1[16 * 64];  // bf16 bits, [i][u]
-    std::uint16_t w2[64 * 64];  // bf16 bits, [j][u]
-    std::uint16_t w3[64];       // bf16 bits
+    std::uint32_t w1p[8 * 64];
+    std::uint32_t w2p[32 * 64];
+    std::uint16_t w3[64];  // bf16 bits
 };
+
+// Position of W[i][u] (input i, unit u) in the BF16-pair storage, as a 16-bit index.
+__host__ __device__ constexpr int pair_index(int i, int u) { return ((i >> 1) * 64 + u) * 2 + (i & 1); }
 
 __device__ __forceinline__ float bf16_bits_to_float(std::uint16_t b) {
     return __uint_as_float(static_cast<std::uint32_t>(b) << 16);
@@ -57,22 +62,43 @@ __device__ __forceinline__ float mlp_input(const MlpWeights& w, int i, float v) 
     return bf16_round((v - w.mean[i]) * w.inv_std[i]);
 }
 
-// Forward pass for one sample across a wavefront: lane u is hidden unit u. `x` holds the 16
-// standardised inputs (same in every lane). Returns the logit (same in every lane) and the
-// lane's BF16 activations h1, h2.
-__device__ __forceinline__ float mlp_forward(const MlpWeights& w, const float (&x)[kFeatures],
-                                             int lane, float& h1, float& h2) {
+typedef __bf16 mlp_bf16x2 __attribute__((ext_vector_type(2)));
+
+// acc + lo(a) * lo(b) + hi(a) * hi(b) over BF16 pairs, FP32 accumulation.
+__device__ __forceinline__ float dot2_bf16(std::uint32_t a, std::uint32_t b, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(mlp_bf16x2, a),
+                                           __builtin_bit_cast(mlp_bf16x2, b), acc, false);
+}
+
+// Forward pass for one sample across a wavefront: lane u is hidden unit u. `input_pair(p)`
+// gives the standardised BF16 inputs 2p and 2p + 1 packed as a pair (same in every lane);
+// `hbuf` is 128 bytes of the wave's LDS (16-byte aligned) through which the first layer's
+// activations reach every lane. Returns the logit (same in every lane) and the lane's BF16
+// activations h1, h2.
+template <typename InputPair>
+__device__ __forceinline__ float mlp_forward_core(const MlpWeights& w, InputPair input_pair, int lane,
+                                                  float& h1, float& h2, std::uint16_t* hbuf) {
     float a = w.b1[lane];
 #pragma unroll
-    for (int i = 0; i < kFeatures; ++i) {
-        a = __builtin_fmaf(x[i], bf16_bits_to_float(w.w1[i * 64 + lane]), a);
+    for (int p = 0; p < kFeatures / 2; ++p) {
+        a = dot2_bf16(input_pair(p), w.w1p[p * 64 + lane], a);
     }
     h1 = bf16_round(fmaxf(a, 0.f));
+    hbuf[lane] = static_cast<std::uint16_t>(__float_as_uint(h1) >> 16);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint4* hp = reinterpret_cast<const uint4*>(hbuf);
     float b = w.b2[lane];
-#pragma unroll 8
-    for (int j = 0; j < kHidden; ++j) {
-        b = __builtin_fmaf(__shfl(h1, j, 64), bf16_bits_to_float(w.w2[j * 64 + lane]), b);
+#pragma unroll
+    for (int q = 0; q < kHidden / 8; ++q) {
+        const uint4 v = hp[q];
+        b = dot2_bf16(v.x, w.w2p[(4 * q + 0) * 64 + lane], b);
+        b = dot2_bf16(v.y, w.w2p[(4 * q + 1) * 64 + lane], b);
+        b = dot2_bf16(v.z, w.w2p[(4 * q + 2) * 64 + lane], b);
+        b = dot2_bf16(v.w, w.w2p[(4 * q + 3) * 64 + lane], b);
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     h2 = bf16_round(fmaxf(b, 0.f));
     float z = h2 * bf16_bits_to_float(w.w3[lane]);
 #pragma unroll
@@ -82,6 +108,29 @@ __device__ __forceinline__ float mlp_forward(const MlpWeights& w, const float (&
     // The butterfly leaves each lane a differently associated sum: take lane 0's, so every
     // lane (and the stop decision) sees one value.
     return __shfl(z, 0, 64) + w.b3;
+}
+
+__device__ __forceinline__ std::uint32_t bf16_pair(float lo, float hi) {
+    return (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xFFFF0000u);
+}
+
+// The forward pass from standardised inputs held in registers (training).
+__device__ __forceinline__ float mlp_forward(const MlpWeights& w, const float (&x)[kFeatures],
+                                             int lane, float& h1, float& h2, std::uint16_t* hbuf) {
+    return mlp_forward_core(w, [&](int p) { return bf16_pair(x[2 * p], x[2 * p + 1]); }, lane, h1, h2,
+                            hbuf);
+}
+
+// The forward pass from raw features in LDS (inference inside the search): each input pair is
+// standardised as it is consumed, the same values mlp_input gives.
+__device__ __forceinline__ float mlp_forward_raw(const MlpWeights& w, const float* raw, int lane,
+                                                 float& h1, float& h2, std::uint16_t* hbuf) {
+    return mlp_forward_core(
+        w,
+        [&](int p) {
+            return bf16_pair(mlp_input(w, 2 * p, raw[2 * p]), mlp_input(w, 2 * p + 1, raw[2 * p + 1]));
+        },
+        lane, h1, h2, hbuf);
 }
 
 }  // namespace vdx::dev

@@ -19,13 +19,15 @@ namespace vdx::dev {
 __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpGradArgs a) {
     __shared__ MlpWeights w;
     __shared__ std::uint16_t w2t[64 * 64];  // w2t[j * 64 + u] = W2[u][j]
+    __shared__ __align__(16) std::uint16_t hbuf[4][64];
     for (std::uint32_t i = threadIdx.x; i < sizeof(MlpWeights) / 4; i += blockDim.x) {
         reinterpret_cast<std::uint32_t*>(&w)[i] = reinterpret_cast<const std::uint32_t*>(a.weights)[i];
     }
     __syncthreads();
+    const std::uint16_t* w2h = reinterpret_cast<const std::uint16_t*>(w.w2p);
     for (std::uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-        const std::uint32_t j = i / 64, u = i % 64;
-        w2t[j * 64 + u] = w.w2[u * 64 + j];
+        const int j = static_cast<int>(i / 64), u = static_cast<int>(i % 64);
+        w2t[j * 64 + u] = w2h[pair_index(u, j)];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpGradArgs a) {
             x[i] = mlp_input(w, i, __shfl(raw, i, 64));
         }
         float h1, h2;
-        const float z = mlp_forward(w, x, lane, h1, h2);
+        const float z = mlp_forward(w, x, lane, h1, h2, hbuf[threadIdx.x >> 6]);
         const float p = 1.0f / (1.0f + __expf(-z));
         // binary cross-entropy, from the logit for stability
         loss += fmaxf(z, 0.f) - z * y + __logf(1.0f + __expf(-fabsf(z)));
@@ -100,11 +102,13 @@ __global__ void mlp_adam_kernel(MlpAdamArgs a) {
     a.master[p] = value;
     MlpWeights* w = a.weights;
     if (p < kOffB1) {
-        w->w1[p - kOffW1] = float_to_bf16_bits(value);
+        const int e = p - kOffW1;
+        reinterpret_cast<std::uint16_t*>(w->w1p)[pair_index(e / 64, e % 64)] = float_to_bf16_bits(value);
     } else if (p < kOffW2) {
         w->b1[p - kOffB1] = value;
     } else if (p < kOffB2) {
-        w->w2[p - kOffW2] = float_to_bf16_bits(value);
+        const int e = p - kOffW2;
+        reinterpret_cast<std::uint16_t*>(w->w2p)[pair_index(e / 64, e % 64)] = float_to_bf16_bits(value);
     } else if (p < kOffW3) {
         w->b2[p - kOffB2] = value;
     } else if (p < kOffB3) {
@@ -124,11 +128,13 @@ __global__ void mlp_pack_kernel(const float* params, MlpWeights* w) {
     } else if (p < kOffW1) {
         w->inv_std[p - kOffInvStd] = value;
     } else if (p < kOffB1) {
-        w->w1[p - kOffW1] = float_to_bf16_bits(value);
+        const int e = p - kOffW1;
+        reinterpret_cast<std::uint16_t*>(w->w1p)[pair_index(e / 64, e % 64)] = float_to_bf16_bits(value);
     } else if (p < kOffW2) {
         w->b1[p - kOffB1] = value;
     } else if (p < kOffB2) {
-        w->w2[p - kOffW2] = float_to_bf16_bits(value);
+        const int e = p - kOffW2;
+        reinterpret_cast<std::uint16_t*>(w->w2p)[pair_index(e / 64, e % 64)] = float_to_bf16_bits(value);
     } else if (p < kOffW3) {
         w->b2[p - kOffB2] = value;
     } else if (p < kOffB3) {
@@ -143,6 +149,7 @@ __global__ void mlp_pack_kernel(const float* params, MlpWeights* w) {
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const MlpWeights* weights, const float* features,
                                                        std::uint64_t n, float* logits) {
     __shared__ MlpWeights w;
+    __shared__ __align__(16) std::uint16_t hbuf[4][64];
     for (std::uint32_t i = threadIdx.x; i < sizeof(MlpWeights) / 4; i += blockDim.x) {
         reinterpret_cast<std::uint32_t*>(&w)[i] = reinterpret_cast<const std::uint32_t*>(weights)[i];
     }
@@ -155,7 +162,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const MlpWeights* weight
 #pragma unroll
     for (int i = 0; i < kFeatures; ++i) x[i] = mlp_input(w, i, __shfl(raw, i, 64));
     float h1, h2;
-    const float z = mlp_forward(w, x, lane, h1, h2);
+    const float z = mlp_forward(w, x, lane, h1, h2, hbuf[threadIdx.x >> 6]);
     if (lane == 0) logits[row] = z;
 }
 

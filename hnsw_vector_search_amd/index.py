@@ -307,6 +307,55 @@ class HnswIndex:
                 for r in range(q.shape[0])]
 
 
+class SearchBatch:
+    """Uneven batches on the GPU: every query with its own (ef, k), served by one persistent
+    launch per R-capacity class, optionally replayed as a captured HIP graph
+    (vdx_gpu_hnsw_batch_*). Buffers are device arrays (nat.DeviceArray); queries must be
+    prepared (unit length under cosine). Each query's result equals HnswIndex.search at its
+    (ef, k). Keep the index alive while the batch is in use."""
+
+    def __init__(self, index: HnswIndex, max_nq: int, max_k: int, max_ef: int,
+                 opts: nat.SearchOpts | None = None):
+        self.index = index
+        self.max_k = int(max_k)
+        self._opts = opts  # a learned model referenced by opts must stay alive
+        h = ctypes.c_void_p()
+        nat.check(nat.lib.vdx_gpu_hnsw_batch_create(index.gpu_handle(), int(max_nq), int(max_k),
+                                                    int(max_ef),
+                                                    ctypes.byref(opts) if opts is not None else None,
+                                                    ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            nat.lib.vdx_gpu_hnsw_batch_free(self._h)
+            self._h = None
+
+    @staticmethod
+    def _args(dq, nq, ks, k, efs, ef, out, count):
+        return (dq.ptr, int(nq), None if ks is None else ks.ptr, int(k),
+                None if efs is None else efs.ptr, int(ef), out.ptr, out.shape[1], count.ptr)
+
+    def tune(self, dq, nq, out, count, *, ks=None, k=0, efs=None, ef=0, reps: int = 3) -> list:
+        """Times each class's launch shapes on this sample and keeps the fastest."""
+        nat.check(nat.lib.vdx_gpu_hnsw_batch_tune(self._h, *self._args(dq, nq, ks, k, efs, ef, out, count),
+                                                  int(reps)))
+        return self.classes()
+
+    def run(self, dq, nq, out, count, *, ks=None, k=0, efs=None, ef=0, graph: bool = True,
+            stream=None) -> None:
+        """Enqueues the batch (asynchronous)."""
+        nat.check(nat.lib.vdx_gpu_hnsw_batch_run(self._h, *self._args(dq, nq, ks, k, efs, ef, out, count),
+                                                 1 if graph else 0, stream))
+
+    def classes(self) -> list:
+        buf = (nat.BatchClass * 8)()
+        n = ctypes.c_uint32()
+        nat.check(nat.lib.vdx_gpu_hnsw_batch_classes(self._h, buf, 8, ctypes.byref(n)))
+        return [{name: getattr(buf[i], name) for name, _ in nat.BatchClass._fields_}
+                for i in range(n.value)]
+
+
 class BruteForceIndex:
     """vecdex::BruteForceIndex (include/vecdex/brute_force.hpp:21-43) with a batched GPU search.
 

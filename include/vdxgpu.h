@@ -234,6 +234,44 @@ VDX_API int vdx_gpu_hnsw_trace_device(vdx_gpu_hnsw* index, const float* d_querie
                                       uint32_t max_steps, float* d_features, uint8_t* d_hits,
                                       uint32_t* d_steps, uint32_t* d_final);
 
+/* ---- Uneven batches (no counterpart in vecdex, whose search_batch takes one k and ef,
+ *      src/hnsw.cpp:490-501): queries with their own (ef, k) in one call. A classify kernel
+ *      groups the queries by R capacity class (64, 128, 256, 512, and one LDS-kernel class
+ *      above 512); each class runs as its own persistent launch on its own stream, forked
+ *      from and joined to the caller's stream. Each query's result equals its standalone
+ *      search at its (ef, k). Per-query k above max_k is clamped to max_k. */
+typedef struct vdx_gpu_batch vdx_gpu_batch;
+typedef struct vdx_batch_class {
+    uint32_t ef_cap;          /* R capacity of the class */
+    uint32_t queries;         /* queries of the tuning sample in the class */
+    uint32_t visited_log2;    /* launch shape in use */
+    uint32_t waves_per_group;
+    uint32_t blocks_per_cu;   /* resident workgroups per CU the grid fills (0: as many as fit) */
+    float ms;                 /* the tuned launch time on the sample (0 before tuning) */
+} vdx_batch_class;
+/* Device buffers for batches of up to max_nq queries with k <= max_k and ef <= max_ef; opts
+ * selects termination (a learned model must outlive the batch) and, when its visited_log2 or
+ * waves_per_group are set, fixes those shapes. Free the batch before its index. */
+VDX_API int vdx_gpu_hnsw_batch_create(vdx_gpu_hnsw* index, uint64_t max_nq, uint32_t max_k, uint32_t max_ef,
+                                      const vdx_search_opts* opts, vdx_gpu_batch** out);
+VDX_API void vdx_gpu_hnsw_batch_free(vdx_gpu_batch* batch);
+/* Autotuner: times each class's launch on a sample batch over visited-cache sizes, waves per
+ * workgroup and resident workgroups per CU, keeping the fastest shape per class. Synchronous. */
+VDX_API int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* batch, const float* d_queries, uint64_t nq,
+                                    const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs, uint32_t ef,
+                                    vdx_neighbor* d_out, uint64_t out_stride, uint32_t* d_out_count,
+                                    uint32_t reps);
+/* Enqueues the batch on `stream` (the index context's stream if NULL) and returns. With
+ * use_graph, the launch sequence is captured into a HIP graph on the first call with these
+ * buffers and sizes and replayed by later identical calls. Calls on one batch must not overlap. */
+VDX_API int vdx_gpu_hnsw_batch_run(vdx_gpu_batch* batch, const float* d_queries, uint64_t nq,
+                                   const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs, uint32_t ef,
+                                   vdx_neighbor* d_out, uint64_t out_stride, uint32_t* d_out_count,
+                                   int use_graph, void* stream);
+/* The classes and their launch shapes (count = number of classes; up to `capacity` written). */
+VDX_API int vdx_gpu_hnsw_batch_classes(const vdx_gpu_batch* batch, vdx_batch_class* out, uint32_t capacity,
+                                       uint32_t* count);
+
 /* ---- Brute force: replaces BruteForceIndex::search (include/vecdex/brute_force.hpp:36;
  *      src/brute_force.cpp:68-84), batched over queries. */
 
