@@ -13,9 +13,10 @@ namespace vdx::gpu {
 
 namespace {
 
+// 2^11 slots up to R capacity 512: a larger cache costs more in LDS occupancy (at capacity 512 a
+// 2^12 cache leaves one 4-wave workgroup per CU) than its fewer recomputed distances save.
 std::uint32_t default_visited_log2(std::uint32_t ef_cap) {
-    if (ef_cap <= 128) return 11;
-    if (ef_cap <= 512) return 12;
+    if (ef_cap <= 512) return 11;
     return 13;
 }
 
