@@ -33,6 +33,8 @@ namespace vdx::dev {
 
 namespace {
 
+constexpr std::uint32_t kStageBytes = 64 * 8 + 64 * 4;  // per wave: 64 survivor keys and queries
+
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -223,6 +225,25 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
             vals[(2 + b_) * NT + threadIdx.x] = stage_xs;
         }
     };
+    // Filter survivors are staged per wave in LDS (64 keys with their queries, positions from a
+    // ballot, no atomics) and flushed together: one instruction reserves the buffer slots of
+    // all staged survivors, so the wave waits for one round trip per 64 survivors instead of one
+    // per survivor.
+    std::uint64_t* skey = reinterpret_cast<std::uint64_t*>(lds + 2 * NT * rs + 4 * NT * 4 + wave * kStageBytes);
+    std::uint32_t* squery = reinterpret_cast<std::uint32_t*>(skey + 64);
+    std::uint32_t staged = 0;  // wave-uniform
+    const auto flush = [&]() {
+        wave_lds_fence();
+        if (static_cast<std::uint32_t>(lane) < staged) {
+            const std::uint32_t sq = squery[lane];
+            const std::uint32_t slot = atomicAdd(a.cnt + sq, 1u);
+            if (slot < a.cap) {
+                a.buf[static_cast<std::size_t>(sq) * a.cap + slot] = skey[lane];
+            }
+        }
+        wave_lds_fence();
+        staged = 0;
+    };
     fetch(t0);
     store(0);
     __syncthreads();
@@ -287,11 +308,19 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
                         // fma equals (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM
                         // distance exceeds tau, which the certification relies on.
                         const float d = a.l2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;
-                        if (qok && d <= tau && j < a.rows) {
-                            const std::uint32_t slot = atomicAdd(a.cnt + q, 1u);
-                            if (slot < a.cap) {
-                                a.buf[static_cast<std::size_t>(q) * a.cap + slot] = make_key(d, j * a.row_stride);
+                        const bool pass = qok && d <= tau && j < a.rows;
+                        const std::uint64_t m = ballot(pass);
+                        if (m != 0) {
+                            const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m));
+                            if (staged + np > 64) {
+                                flush();
                             }
+                            if (pass) {
+                                const std::uint32_t at = staged + lanes_below(m);
+                                skey[at] = make_key(d, j * a.row_stride);
+                                squery[at] = q;
+                            }
+                            staged += np;
                         }
                     }
                 }
@@ -301,6 +330,9 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
             store(cb ^ 1);
         }
         __syncthreads();
+    }
+    if (a.mode != 0 && staged > 0) {
+        flush();
     }
 }
 
@@ -609,7 +641,7 @@ hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
 std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp) {
     const std::size_t eb = prec == kPrecF32 ? 4 : (prec == kPrecBF16 ? 2 : 1);
     const std::size_t nt = prec == kPrecF32 ? 32 : 64;
-    return 2 * nt * (kp * eb + 16) + 4 * nt * sizeof(float);
+    return 2 * nt * (kp * eb + 16) + 4 * nt * sizeof(float) + 4 * kStageBytes;
 }
 
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec,
