@@ -130,8 +130,8 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
 // m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) (base rows) of its 16 accumulators i. Operand
 // fragments: lane l holds row/column l % 32 and the K slice
 // [kstep * K + (l / 32) * K / 2, ... + K / 2).
-template <int PREC>
-__global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
+template <int PREC, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
     constexpr int EB = PREC == kPrecF32 ? 4 : (PREC == kPrecBF16 ? 2 : 1);  // bytes per element
     constexpr int NT = PREC == kPrecF32 ? 32 : 64;                           // rows per tile
     constexpr int NH = NT / 32;                                              // MFMA row tiles
@@ -141,7 +141,8 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
     const int wave = threadIdx.x >> 6;
     const int col = lane & 31;
     const int half = lane >> 5;
-    const std::uint32_t q = blockIdx.x * 128 + wave * 32 + col;  // this lane's query
+    constexpr std::uint32_t kThreads = WAVES * 64;
+    const std::uint32_t q = blockIdx.x * (32 * WAVES) + wave * 32 + col;  // this lane's query
     const std::uint32_t kp = a.kp;
     const std::uint32_t row_bytes = kp * EB;
     const std::uint32_t rs = row_bytes + 16;  // padded LDS row stride
@@ -182,12 +183,12 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
     // Cooperative tile loads: 16-byte pieces, piece p is row p / chunks, chunk p % chunks.
     const std::uint32_t chunks = row_bytes / 16;
     const std::uint32_t pieces = NT * chunks;
-    // NT * row_bytes <= 16 KB: at most 4 pieces per thread
+    // NT * row_bytes <= 16 KB: at most 1024 / kThreads pieces per thread
     uint4 s0 = make_uint4(0, 0, 0, 0), s1 = s0, s2 = s0, s3 = s0;
     float stage_xn = 0.f, stage_xs = 1.f;
 #define VDX_FETCH_PIECE(I, DST)                                                                   \
     {                                                                                             \
-        const std::uint32_t p = threadIdx.x + 256u * (I);                                         \
+        const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
         if (p < pieces) {                                                                         \
             const std::uint32_t j = t_ * NT + p / chunks;                                         \
             const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
     }
 #define VDX_STORE_PIECE(I, SRC)                                                                   \
     {                                                                                             \
-        const std::uint32_t p = threadIdx.x + 256u * (I);                                         \
+        const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
         if (p < pieces) {                                                                         \
             *reinterpret_cast<uint4*>(lds + b_ * NT * rs + (p / chunks) * rs + (p % chunks) * 16) = SRC; \
         }                                                                                         \
@@ -206,8 +207,10 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
     const auto fetch = [&](std::uint32_t t_) {
         VDX_FETCH_PIECE(0, s0)
         VDX_FETCH_PIECE(1, s1)
-        VDX_FETCH_PIECE(2, s2)
-        VDX_FETCH_PIECE(3, s3)
+        if constexpr (WAVES == 4) {
+            VDX_FETCH_PIECE(2, s2)
+            VDX_FETCH_PIECE(3, s3)
+        }
         if (threadIdx.x < NT) {
             const std::uint32_t j = t_ * NT + threadIdx.x;
             const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
@@ -218,8 +221,10 @@ __global__ __launch_bounds__(256) void bf_gemm_kernel(GemmArgs a) {
     const auto store = [&](int b_) {
         VDX_STORE_PIECE(0, s0)
         VDX_STORE_PIECE(1, s1)
-        VDX_STORE_PIECE(2, s2)
-        VDX_STORE_PIECE(3, s3)
+        if constexpr (WAVES == 4) {
+            VDX_STORE_PIECE(2, s2)
+            VDX_STORE_PIECE(3, s3)
+        }
         if (threadIdx.x < NT) {
             vals[b_ * NT + threadIdx.x] = stage_xn;
             vals[(2 + b_) * NT + threadIdx.x] = stage_xs;
@@ -638,23 +643,31 @@ hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
     return hipGetLastError();
 }
 
-std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp) {
+std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
     const std::size_t eb = prec == kPrecF32 ? 4 : (prec == kPrecBF16 ? 2 : 1);
     const std::size_t nt = prec == kPrecF32 ? 32 : 64;
-    return 2 * nt * (kp * eb + 16) + 4 * nt * sizeof(float) + 4 * kStageBytes;
+    return 2 * nt * (kp * eb + 16) + 4 * nt * sizeof(float) + static_cast<std::size_t>(waves) * kStageBytes;
 }
 
-hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec,
+hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
                           hipStream_t stream) {
     const dim3 grid(qtiles, slabs);
-    const std::size_t lds = bf_gemm_lds_bytes(prec, args.kp);
-    if (prec == kPrecF32) {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecF32>, grid, dim3(256), lds, stream, args);
-    } else if (prec == kPrecBF16) {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecBF16>, grid, dim3(256), lds, stream, args);
-    } else {
-        hipLaunchKernelGGL(bf_gemm_kernel<kPrecFP8>, grid, dim3(256), lds, stream, args);
+    const std::size_t lds = bf_gemm_lds_bytes(prec, args.kp, waves);
+    const dim3 block(64 * waves);
+#define VDX_GEMM_LAUNCH(P)                                                                        \
+    if (waves == 8) {                                                                             \
+        hipLaunchKernelGGL((bf_gemm_kernel<P, 8>), grid, block, lds, stream, args);               \
+    } else {                                                                                      \
+        hipLaunchKernelGGL((bf_gemm_kernel<P, 4>), grid, block, lds, stream, args);               \
     }
+    if (prec == kPrecF32) {
+        VDX_GEMM_LAUNCH(kPrecF32)
+    } else if (prec == kPrecBF16) {
+        VDX_GEMM_LAUNCH(kPrecBF16)
+    } else {
+        VDX_GEMM_LAUNCH(kPrecFP8)
+    }
+#undef VDX_GEMM_LAUNCH
     return hipGetLastError();
 }
 

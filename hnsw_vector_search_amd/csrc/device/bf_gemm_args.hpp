@@ -94,7 +94,8 @@ hipError_t launch_bf_scatter_results(const Neighbor* part, const std::uint32_t* 
                                      const std::uint32_t* idx, std::uint32_t count,
                                      std::uint32_t width, Neighbor* out, std::uint64_t out_stride,
                                      std::uint32_t* out_count, hipStream_t stream);
-hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec,
+// One workgroup of `waves` (4 or 8) wavefronts per 32 * waves queries and slab of base rows.
+hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
                           hipStream_t stream);
 hipError_t launch_bf_threshold(const ThresholdArgs& args, unsigned nq, hipStream_t stream);
 hipError_t launch_bf_shortlist(const ShortlistArgs& args, unsigned nq, hipStream_t stream);

@@ -16,6 +16,7 @@ namespace {
 constexpr std::uint32_t kMaxMfmaDim = 128;   // query operands held in registers
 constexpr std::uint32_t kMaxShortlist = 1024;  // re-rank sorts in LDS: 8 KB per wave
 constexpr std::uint64_t kQueryChunk = 16384;
+constexpr int kGemmWaves = 8;  // wavefronts (32 queries each) sharing one staged base tile
 
 std::uint32_t operand_kp(int prec, std::uint64_t dim) {
     return prec == dev::kPrecF32 ? static_cast<std::uint32_t>((dim + 3) / 4 * 4)
@@ -146,7 +147,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
     float max_err = 0.f;
     for (std::uint64_t c0 = 0; c0 < nq; c0 += kQueryChunk) {
         const std::uint64_t cq = std::min<std::uint64_t>(kQueryChunk, nq - c0);
-        const std::uint64_t cq_pad = (cq + 127) / 128 * 128;
+        const std::uint64_t cq_pad = (cq + 255) / 256 * 256;
         const float* cq_src = d_queries + c0 * b.dim;
         DeviceBuffer& qq = ws.extra[0];
         DeviceBuffer& qn2 = ws.extra[1];
@@ -187,7 +188,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         g.kp = kp;
         g.ks = ks;
         g.l2 = b.metric == 0 ? 1u : 0u;
-        const unsigned qtiles = static_cast<unsigned>(cq_pad / 128);
+        const unsigned qtiles = static_cast<unsigned>(cq_pad / (32 * kGemmWaves));
         if (sample) {
             dense.reserve(cq * static_cast<std::uint64_t>(m) * 4);
             g.rows = m;
@@ -195,7 +196,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
             g.mode = 0;
             g.dense = dense.as<float>();
             g.dense_ld = m;
-            check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (m + 63) / 64), prec, s), "bf_gemm(sample) launch");
+            check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (m + 63) / 64), prec, kGemmWaves, s), "bf_gemm(sample) launch");
             dev::ThresholdArgs t{};
             t.dense = dense.as<float>();
             t.dense_ld = m;
@@ -213,7 +214,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         g.buf = buf.as<std::uint64_t>();
         g.cap = cap;
         g.cnt = cnt.as<std::uint32_t>();
-        check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (b.n + 63) / 64), prec, s), "bf_gemm(filter) launch");
+        check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (b.n + 63) / 64), prec, kGemmWaves, s), "bf_gemm(filter) launch");
 
         list.reserve(cq * S * 4);
         approx.reserve(cq * S * 4);
