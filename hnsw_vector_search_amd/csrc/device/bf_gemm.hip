@@ -130,7 +130,9 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
 // m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) (base rows) of its 16 accumulators i. Operand
 // fragments: lane l holds row/column l % 32 and the K slice
 // [kstep * K + (l / 32) * K / 2, ... + K / 2).
-template <int PREC, int WAVES>
+// MODE 0: dense distances to a sample of rows; MODE 1: the threshold filter. L2: squared L2,
+// else 1 - <q, x>.
+template <int PREC, int WAVES, int MODE, bool L2>
 __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
     constexpr int EB = PREC == kPrecF32 ? 4 : (PREC == kPrecBF16 ? 2 : 1);  // bytes per element
     constexpr int NT = PREC == kPrecF32 ? 32 : 64;                           // rows per tile
@@ -183,53 +185,58 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
     // Cooperative tile loads: 16-byte pieces, piece p is row p / chunks, chunk p % chunks.
     const std::uint32_t chunks = row_bytes / 16;
     const std::uint32_t pieces = NT * chunks;
-    // NT * row_bytes <= 16 KB: at most 1024 / kThreads pieces per thread
-    uint4 s0 = make_uint4(0, 0, 0, 0), s1 = s0, s2 = s0, s3 = s0;
-    float stage_xn = 0.f, stage_xs = 1.f;
-#define VDX_FETCH_PIECE(I, DST)                                                                   \
+    // NT * row_bytes <= 16 KB: at most 1024 / kThreads pieces per thread. Two register stages
+    // (a, b) hold the tiles fetched one and two steps ahead, so a tile's global loads have two
+    // tiles of compute to land before they are stored to LDS.
+    uint4 sa0 = make_uint4(0, 0, 0, 0), sa1 = sa0, sa2 = sa0, sa3 = sa0;
+    uint4 sb0 = sa0, sb1 = sa0, sb2 = sa0, sb3 = sa0;
+    float xa_n = 0.f, xa_s = 1.f, xb_n = 0.f, xb_s = 1.f;
+#define VDX_FETCH_PIECE(T_, I, DST)                                                               \
     {                                                                                             \
         const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
         if (p < pieces) {                                                                         \
-            const std::uint32_t j = t_ * NT + p / chunks;                                         \
+            const std::uint32_t j = (T_) * NT + p / chunks;                                       \
             const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
             DST = *reinterpret_cast<const uint4*>(static_cast<const unsigned char*>(a.xq) +       \
                                                   static_cast<std::size_t>(xr) * row_bytes +      \
                                                   (p % chunks) * 16);                             \
         }                                                                                         \
     }
-#define VDX_STORE_PIECE(I, SRC)                                                                   \
+#define VDX_STORE_PIECE(B_, I, SRC)                                                               \
     {                                                                                             \
         const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
         if (p < pieces) {                                                                         \
-            *reinterpret_cast<uint4*>(lds + b_ * NT * rs + (p / chunks) * rs + (p % chunks) * 16) = SRC; \
+            *reinterpret_cast<uint4*>(lds + (B_) * NT * rs + (p / chunks) * rs + (p % chunks) * 16) = SRC; \
         }                                                                                         \
     }
-    const auto fetch = [&](std::uint32_t t_) {
-        VDX_FETCH_PIECE(0, s0)
-        VDX_FETCH_PIECE(1, s1)
-        if constexpr (WAVES == 4) {
-            VDX_FETCH_PIECE(2, s2)
-            VDX_FETCH_PIECE(3, s3)
-        }
-        if (threadIdx.x < NT) {
-            const std::uint32_t j = t_ * NT + threadIdx.x;
-            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
-            stage_xn = a.xn[xr];
-            stage_xs = a.xs ? a.xs[xr] : 1.f;
-        }
-    };
-    const auto store = [&](int b_) {
-        VDX_STORE_PIECE(0, s0)
-        VDX_STORE_PIECE(1, s1)
-        if constexpr (WAVES == 4) {
-            VDX_STORE_PIECE(2, s2)
-            VDX_STORE_PIECE(3, s3)
-        }
-        if (threadIdx.x < NT) {
-            vals[b_ * NT + threadIdx.x] = stage_xn;
-            vals[(2 + b_) * NT + threadIdx.x] = stage_xs;
-        }
-    };
+#define VDX_FETCH_STAGE(T_, S0, S1, S2, S3, XN, XS)                                               \
+    {                                                                                             \
+        VDX_FETCH_PIECE(T_, 0, S0)                                                                \
+        VDX_FETCH_PIECE(T_, 1, S1)                                                                \
+        if constexpr (WAVES == 4) {                                                               \
+            VDX_FETCH_PIECE(T_, 2, S2)                                                            \
+            VDX_FETCH_PIECE(T_, 3, S3)                                                            \
+        }                                                                                         \
+        if (threadIdx.x < NT) {                                                                   \
+            const std::uint32_t j = (T_) * NT + threadIdx.x;                                      \
+            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
+            XN = a.xn[xr];                                                                        \
+            XS = a.xs ? a.xs[xr] : 1.f;                                                           \
+        }                                                                                         \
+    }
+#define VDX_STORE_STAGE(B_, S0, S1, S2, S3, XN, XS)                                               \
+    {                                                                                             \
+        VDX_STORE_PIECE(B_, 0, S0)                                                                \
+        VDX_STORE_PIECE(B_, 1, S1)                                                                \
+        if constexpr (WAVES == 4) {                                                               \
+            VDX_STORE_PIECE(B_, 2, S2)                                                            \
+            VDX_STORE_PIECE(B_, 3, S3)                                                            \
+        }                                                                                         \
+        if (threadIdx.x < NT) {                                                                   \
+            vals[(B_) * NT + threadIdx.x] = XN;                                                   \
+            vals[(2 + (B_)) * NT + threadIdx.x] = XS;                                             \
+        }                                                                                         \
+    }
     // Filter survivors are staged per wave in LDS (64 keys with their queries, positions from a
     // ballot, no atomics) and flushed together: one instruction reserves the buffer slots of
     // all staged survivors, so the wave waits for one round trip per 64 survivors instead of one
@@ -249,14 +256,24 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
         wave_lds_fence();
         staged = 0;
     };
-    fetch(t0);
-    store(0);
+    // Tile t0 goes to LDS buffer 0 now and tile t0 + 1 waits in stage b. At step t (buffer
+    // cb = (t - t0) & 1 holds tile t): even steps fetch tile t + 2 into stage a and store stage
+    // b (tile t + 1) to buffer 1; odd steps fetch into b and store a to buffer 0.
+    VDX_FETCH_STAGE(t0, sa0, sa1, sa2, sa3, xa_n, xa_s)
+    VDX_STORE_STAGE(0, sa0, sa1, sa2, sa3, xa_n, xa_s)
+    if (t0 + 1 < t1) {
+        VDX_FETCH_STAGE(t0 + 1, sb0, sb1, sb2, sb3, xb_n, xb_s)
+    }
     __syncthreads();
     for (std::uint32_t t = t0; t < t1; ++t) {
         const int cb = static_cast<int>((t - t0) & 1u);
         const bool more = t + 1 < t1;
-        if (more) {
-            fetch(t + 1);
+        if (t + 2 < t1) {
+            if (cb == 0) {
+                VDX_FETCH_STAGE(t + 2, sa0, sa1, sa2, sa3, xa_n, xa_s)
+            } else {
+                VDX_FETCH_STAGE(t + 2, sb0, sb1, sb2, sb3, xb_n, xb_s)
+            }
         }
         v16f acc[NH];
 #pragma unroll
@@ -283,66 +300,77 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
                 }
             }
         }
-        // Epilogue: rows 8 * g + 4 * half + (0..3) of each row tile, four at a time.
-#pragma unroll
-        for (int h = 0; h < NH; ++h) {
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int r0 = h * 32 + 8 * g + 4 * half;  // first of four consecutive tile rows
-                const float4 xn4 = *reinterpret_cast<const float4*>(vals + cb * NT + r0);
-                float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);
-                if constexpr (PREC == kPrecFP8) {
-                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + cb) * NT + r0);
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w));
-                    const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w));
-                    float dot = acc[h][4 * g + e];
-                    if constexpr (PREC == kPrecFP8) {
-                        dot = dot * qs * xsv;
-                    }
-                    const std::uint32_t j = t * NT + r0 + e;
-                    if (a.mode == 0) {
-                        if (qok && j < a.rows) {
-                            const float d = a.l2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;
-                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;
-                        }
-                    } else {
-                        // The filter tests exactly the distance it stores (2 dot is exact, so the
-                        // fma equals (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM
-                        // distance exceeds tau, which the certification relies on.
-                        const float d = a.l2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;
-                        const bool pass = qok && d <= tau && j < a.rows;
-                        const std::uint64_t m = ballot(pass);
-                        if (m != 0) {
-                            const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m));
-                            if (staged + np > 64) {
-                                flush();
-                            }
-                            if (pass) {
-                                const std::uint32_t at = staged + lanes_below(m);
-                                skey[at] = make_key(d, j * a.row_stride);
-                                squery[at] = q;
-                            }
-                            staged += np;
-                        }
-                    }
-                }
-            }
+        // Epilogue: rows 8 * g + 4 * half + (0..3) of each row tile, four at a time. Padding
+        // queries never pass (tau = -inf); the row bound is tested only in a tile that crosses
+        // the end of the rows.
+#define VDX_EPILOGUE(CHECK_ROWS)                                                                  \
+        _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
+            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
+                const int r0 = h * 32 + 8 * g + 4 * half;                                         \
+                const float4 xn4 = *reinterpret_cast<const float4*>(vals + cb * NT + r0);         \
+                float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);                                     \
+                if constexpr (PREC == kPrecFP8) {                                                 \
+                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + cb) * NT + r0);            \
+                }                                                                                 \
+                _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                   \
+                    const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w)); \
+                    const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w)); \
+                    float dot = acc[h][4 * g + e];                                                \
+                    if constexpr (PREC == kPrecFP8) {                                             \
+                        dot = dot * qs * xsv;                                                     \
+                    }                                                                             \
+                    const std::uint32_t j = t * NT + r0 + e;                                      \
+                    const float d = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;       \
+                    if constexpr (MODE == 0) {                                                    \
+                        if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
+                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;            \
+                        }                                                                         \
+                    } else {                                                                      \
+                        const bool pass = d <= tau && (!(CHECK_ROWS) || j < a.rows);              \
+                        const std::uint64_t m = ballot(pass);                                     \
+                        if (m != 0) {                                                             \
+                            const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m));     \
+                            if (staged + np > 64) {                                               \
+                                flush();                                                          \
+                            }                                                                     \
+                            if (pass) {                                                           \
+                                const std::uint32_t at = staged + lanes_below(m);                 \
+                                skey[at] = make_key(d, j * a.row_stride);                         \
+                                squery[at] = q;                                                   \
+                            }                                                                     \
+                            staged += np;                                                         \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+            }                                                                                     \
         }
+        // The filter tests exactly the distance it stores (2 dot is exact, so the fma equals
+        // (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM distance exceeds tau, which
+        // the certification relies on.
+        if ((t + 1) * NT <= a.rows) {
+            VDX_EPILOGUE(false)
+        } else {
+            VDX_EPILOGUE(true)
+        }
+#undef VDX_EPILOGUE
         if (more) {
-            store(cb ^ 1);
+            if (cb == 0) {
+                VDX_STORE_STAGE(1, sb0, sb1, sb2, sb3, xb_n, xb_s)
+            } else {
+                VDX_STORE_STAGE(0, sa0, sa1, sa2, sa3, xa_n, xa_s)
+            }
         }
         __syncthreads();
     }
-    if (a.mode != 0 && staged > 0) {
+    if (MODE != 0 && staged > 0) {
         flush();
     }
 }
 
 #undef VDX_FETCH_PIECE
 #undef VDX_STORE_PIECE
+#undef VDX_FETCH_STAGE
+#undef VDX_STORE_STAGE
 
 // Per query (one workgroup): the `rank`-th smallest (1-based) of `count` u32 sortable
 // distances (dense sample row), by an MSB-first radix select, 8 bits per pass.
@@ -651,14 +679,23 @@ std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
 
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
                           hipStream_t stream) {
+    if (waves != 8) return hipErrorInvalidValue;
     const dim3 grid(qtiles, slabs);
     const std::size_t lds = bf_gemm_lds_bytes(prec, args.kp, waves);
     const dim3 block(64 * waves);
 #define VDX_GEMM_LAUNCH(P)                                                                        \
-    if (waves == 8) {                                                                             \
-        hipLaunchKernelGGL((bf_gemm_kernel<P, 8>), grid, block, lds, stream, args);               \
+    if (args.mode == 0) {                                                                         \
+        if (args.l2) {                                                                            \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, true>), grid, block, lds, stream, args);  \
+        } else {                                                                                  \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, false>), grid, block, lds, stream, args); \
+        }                                                                                         \
     } else {                                                                                      \
-        hipLaunchKernelGGL((bf_gemm_kernel<P, 4>), grid, block, lds, stream, args);               \
+        if (args.l2) {                                                                            \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, true>), grid, block, lds, stream, args);  \
+        } else {                                                                                  \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, false>), grid, block, lds, stream, args); \
+        }                                                                                         \
     }
     if (prec == kPrecF32) {
         VDX_GEMM_LAUNCH(kPrecF32)
