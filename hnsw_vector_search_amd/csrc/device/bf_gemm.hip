@@ -191,22 +191,30 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
     uint4 sa0 = make_uint4(0, 0, 0, 0), sa1 = sa0, sa2 = sa0, sa3 = sa0;
     uint4 sb0 = sa0, sb1 = sa0, sb2 = sa0, sb3 = sa0;
     float xa_n = 0.f, xa_s = 1.f, xb_n = 0.f, xb_s = 1.f;
+    // Each thread's pieces sit at fixed positions within a tile: byte offsets in the operand
+    // rows and in the LDS buffer are computed once. A tile past the end of the rows repeats the
+    // last row (its distances are never used).
+    std::uint32_t prow[4], pcol[4], plds[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const std::uint32_t p = threadIdx.x + kThreads * i;
+        prow[i] = p / chunks;
+        pcol[i] = (p % chunks) * 16;
+        plds[i] = prow[i] * rs + pcol[i];
+    }
+    const unsigned char* const xq_bytes = static_cast<const unsigned char*>(a.xq);
 #define VDX_FETCH_PIECE(T_, I, DST)                                                               \
     {                                                                                             \
-        const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
-        if (p < pieces) {                                                                         \
-            const std::uint32_t j = (T_) * NT + p / chunks;                                       \
+        if (threadIdx.x + kThreads * (I) < pieces) {                                              \
+            const std::uint32_t j = (T_) * NT + prow[I];                                          \
             const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
-            DST = *reinterpret_cast<const uint4*>(static_cast<const unsigned char*>(a.xq) +       \
-                                                  static_cast<std::size_t>(xr) * row_bytes +      \
-                                                  (p % chunks) * 16);                             \
+            DST = *reinterpret_cast<const uint4*>(xq_bytes + static_cast<std::size_t>(xr) * row_bytes + pcol[I]); \
         }                                                                                         \
     }
 #define VDX_STORE_PIECE(B_, I, SRC)                                                               \
     {                                                                                             \
-        const std::uint32_t p = threadIdx.x + kThreads * (I);                                     \
-        if (p < pieces) {                                                                         \
-            *reinterpret_cast<uint4*>(lds + (B_) * NT * rs + (p / chunks) * rs + (p % chunks) * 16) = SRC; \
+        if (threadIdx.x + kThreads * (I) < pieces) {                                              \
+            *reinterpret_cast<uint4*>(lds + (B_) * NT * rs + plds[I]) = SRC;                     \
         }                                                                                         \
     }
 #define VDX_FETCH_STAGE(T_, S0, S1, S2, S3, XN, XS)                                               \
@@ -221,7 +229,9 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
             const std::uint32_t j = (T_) * NT + threadIdx.x;                                      \
             const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;               \
             XN = a.xn[xr];                                                                        \
-            XS = a.xs ? a.xs[xr] : 1.f;                                                           \
+            if constexpr (PREC == kPrecFP8) {                                                     \
+                XS = a.xs[xr];                                                                    \
+            }                                                                                     \
         }                                                                                         \
     }
 #define VDX_STORE_STAGE(B_, S0, S1, S2, S3, XN, XS)                                               \
@@ -256,50 +266,19 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
         wave_lds_fence();
         staged = 0;
     };
-    // Tile t0 goes to LDS buffer 0 now and tile t0 + 1 waits in stage b. At step t (buffer
-    // cb = (t - t0) & 1 holds tile t): even steps fetch tile t + 2 into stage a and store stage
-    // b (tile t + 1) to buffer 1; odd steps fetch into b and store a to buffer 0.
+    // Double-buffered LDS tiles and two register stages a and b, the loop unrolled by two so
+    // the stages keep fixed roles (no register moves, so no wait right after a prefetch is
+    // issued): an even step computes buffer 0 while tile t + 2 loads into a and stores b (tile
+    // t + 1) to buffer 1; an odd step computes buffer 1, loads into b and stores a to buffer 0.
+    // Loads past the last tile repeat it (never computed).
+    const std::uint32_t last = t1 - 1;
     VDX_FETCH_STAGE(t0, sa0, sa1, sa2, sa3, xa_n, xa_s)
     VDX_STORE_STAGE(0, sa0, sa1, sa2, sa3, xa_n, xa_s)
-    if (t0 + 1 < t1) {
-        VDX_FETCH_STAGE(t0 + 1, sb0, sb1, sb2, sb3, xb_n, xb_s)
-    }
+    VDX_FETCH_STAGE(min(t0 + 1, last), sb0, sb1, sb2, sb3, xb_n, xb_s)
+    // Every load so far (the query fragments above all) completes here, once: inside the loop
+    // the compiler then waits only for the stage a step is about to store, never for vmcnt(0).
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) only
     __syncthreads();
-    for (std::uint32_t t = t0; t < t1; ++t) {
-        const int cb = static_cast<int>((t - t0) & 1u);
-        const bool more = t + 1 < t1;
-        if (t + 2 < t1) {
-            if (cb == 0) {
-                VDX_FETCH_STAGE(t + 2, sa0, sa1, sa2, sa3, xa_n, xa_s)
-            } else {
-                VDX_FETCH_STAGE(t + 2, sb0, sb1, sb2, sb3, xb_n, xb_s)
-            }
-        }
-        v16f acc[NH];
-#pragma unroll
-        for (int h = 0; h < NH; ++h) {
-            acc[h] = v16f{};
-        }
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            if (static_cast<std::uint32_t>(s) < ks) {
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    const unsigned char* rowp = lds + cb * NT * rs + (h * 32 + col) * rs;
-                    if constexpr (PREC == kPrecF32) {
-                        const float2 x = *reinterpret_cast<const float2*>(rowp + (4 * s + 2 * half) * 4);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, bq[s].x, acc[h], 0, 0, 0);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, bq[s].y, acc[h], 0, 0, 0);
-                    } else if constexpr (PREC == kPrecBF16) {
-                        const v8bf x = *reinterpret_cast<const v8bf*>(rowp + (16 * s + 8 * half) * 2);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
-                    } else {
-                        const long x = *reinterpret_cast<const long*>(rowp + 16 * s + 8 * half);
-                        acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(x, bq[s], acc[h], 0, 0, 0);
-                    }
-                }
-            }
-        }
         // Epilogue: rows 8 * g + 4 * half + (0..3) of each row tile, four at a time. Padding
         // queries never pass (tau = -inf); the row bound is tested only in a tile that crosses
         // the end of the rows.
@@ -307,11 +286,13 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
         _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
             _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
                 const int r0 = h * 32 + 8 * g + 4 * half;                                         \
-                const float4 xn4 = *reinterpret_cast<const float4*>(vals + cb * NT + r0);         \
+                const float4 xn4 = *reinterpret_cast<const float4*>(vals + tile_cb * NT + r0);         \
                 float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);                                     \
                 if constexpr (PREC == kPrecFP8) {                                                 \
-                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + cb) * NT + r0);            \
+                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + tile_cb) * NT + r0);            \
                 }                                                                                 \
+                float d4[4];                                                                      \
+                std::uint64_t m4[4];                                                              \
                 _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                   \
                     const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w)); \
                     const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w)); \
@@ -319,49 +300,131 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
                     if constexpr (PREC == kPrecFP8) {                                             \
                         dot = dot * qs * xsv;                                                     \
                     }                                                                             \
-                    const std::uint32_t j = t * NT + r0 + e;                                      \
-                    const float d = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;       \
+                    const std::uint32_t j = tile_t * NT + r0 + e;                                      \
+                    d4[e] = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;               \
                     if constexpr (MODE == 0) {                                                    \
                         if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
-                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;            \
+                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d4[e];        \
                         }                                                                         \
                     } else {                                                                      \
-                        const bool pass = d <= tau && (!(CHECK_ROWS) || j < a.rows);              \
-                        const std::uint64_t m = ballot(pass);                                     \
-                        if (m != 0) {                                                             \
-                            const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m));     \
-                            if (staged + np > 64) {                                               \
-                                flush();                                                          \
+                        /* the lanes whose distance passes, straight from the compare */          \
+                        m4[e] = __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);           \
+                        if (CHECK_ROWS) {                                                         \
+                            m4[e] &= ballot(j < a.rows);                                          \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+                if constexpr (MODE != 0) {                                                        \
+                    if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
+                        _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
+                            const std::uint64_t m = m4[e];                                        \
+                            if (m != 0) {                                                         \
+                                const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m)); \
+                                if (staged + np > 64) {                                           \
+                                    flush();                                                      \
+                                }                                                                 \
+                                if ((m >> lane) & 1u) {                                           \
+                                    const std::uint32_t at = staged + lanes_below(m);             \
+                                    skey[at] = make_key(d4[e], (tile_t * NT + r0 + e) * a.row_stride); \
+                                    squery[at] = q;                                               \
+                                }                                                                 \
+                                staged += np;                                                     \
                             }                                                                     \
-                            if (pass) {                                                           \
-                                const std::uint32_t at = staged + lanes_below(m);                 \
-                                skey[at] = make_key(d, j * a.row_stride);                         \
-                                squery[at] = q;                                                   \
-                            }                                                                     \
-                            staged += np;                                                         \
                         }                                                                         \
                     }                                                                             \
                 }                                                                                 \
             }                                                                                     \
         }
-        // The filter tests exactly the distance it stores (2 dot is exact, so the fma equals
-        // (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM distance exceeds tau, which
-        // the certification relies on.
-        if ((t + 1) * NT <= a.rows) {
-            VDX_EPILOGUE(false)
-        } else {
-            VDX_EPILOGUE(true)
-        }
-#undef VDX_EPILOGUE
-        if (more) {
-            if (cb == 0) {
-                VDX_STORE_STAGE(1, sb0, sb1, sb2, sb3, xb_n, xb_s)
-            } else {
-                VDX_STORE_STAGE(0, sa0, sa1, sa2, sa3, xa_n, xa_s)
+    for (std::uint32_t t = t0; t < t1; t += 2) {
+        {
+            const std::uint32_t tile_t = t;
+            constexpr int tile_cb = 0;
+            VDX_FETCH_STAGE(min(t + 2, last), sa0, sa1, sa2, sa3, xa_n, xa_s)
+            v16f acc[NH];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                acc[h] = v16f{};
             }
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                if (static_cast<std::uint32_t>(s) < ks) {
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) {
+                        const unsigned char* rowp = lds + tile_cb * NT * rs + (h * 32 + col) * rs;
+                        if constexpr (PREC == kPrecF32) {
+                            const float2 x = *reinterpret_cast<const float2*>(rowp + (4 * s + 2 * half) * 4);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, bq[s].x, acc[h], 0, 0, 0);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, bq[s].y, acc[h], 0, 0, 0);
+                        } else if constexpr (PREC == kPrecBF16) {
+                            const v8bf x = *reinterpret_cast<const v8bf*>(rowp + (16 * s + 8 * half) * 2);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
+                        } else {
+                            const long x = *reinterpret_cast<const long*>(rowp + 16 * s + 8 * half);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(x, bq[s], acc[h], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+
+            // The filter tests exactly the distance it stores (2 dot is exact, so the fma equals
+            // (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM distance exceeds tau, which
+            // the certification relies on.
+            if ((tile_t + 1) * NT <= a.rows) {
+                VDX_EPILOGUE(false)
+            } else {
+                VDX_EPILOGUE(true)
+            }
+
+            VDX_STORE_STAGE(1, sb0, sb1, sb2, sb3, xb_n, xb_s)
+            __syncthreads();
         }
-        __syncthreads();
+        if (t + 1 >= t1) {
+            break;
+        }
+        {
+            const std::uint32_t tile_t = t + 1;
+            constexpr int tile_cb = 1;
+            VDX_FETCH_STAGE(min(t + 3, last), sb0, sb1, sb2, sb3, xb_n, xb_s)
+            v16f acc[NH];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                acc[h] = v16f{};
+            }
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                if (static_cast<std::uint32_t>(s) < ks) {
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) {
+                        const unsigned char* rowp = lds + tile_cb * NT * rs + (h * 32 + col) * rs;
+                        if constexpr (PREC == kPrecF32) {
+                            const float2 x = *reinterpret_cast<const float2*>(rowp + (4 * s + 2 * half) * 4);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, bq[s].x, acc[h], 0, 0, 0);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, bq[s].y, acc[h], 0, 0, 0);
+                        } else if constexpr (PREC == kPrecBF16) {
+                            const v8bf x = *reinterpret_cast<const v8bf*>(rowp + (16 * s + 8 * half) * 2);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
+                        } else {
+                            const long x = *reinterpret_cast<const long*>(rowp + 16 * s + 8 * half);
+                            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(x, bq[s], acc[h], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+
+            // The filter tests exactly the distance it stores (2 dot is exact, so the fma equals
+            // (|q|^2 + |x|^2) - 2 dot): a row is left out iff its GEMM distance exceeds tau, which
+            // the certification relies on.
+            if ((tile_t + 1) * NT <= a.rows) {
+                VDX_EPILOGUE(false)
+            } else {
+                VDX_EPILOGUE(true)
+            }
+
+            VDX_STORE_STAGE(0, sa0, sa1, sa2, sa3, xa_n, xa_s)
+            __syncthreads();
+        }
     }
+#undef VDX_EPILOGUE
     if (MODE != 0 && staged > 0) {
         flush();
     }
