@@ -15,8 +15,8 @@
 //    cache forgot), and every lane counts how many inserted keys fall below its own R member and
 //    its own fresh key. Those counts are the final positions: R and the new keys are scattered
 //    to an LDS buffer by position and read back as chunks.
-//  * The link block of the next expansion is prefetched: when an expansion starts, the block of
-//    the next unexpanded member of R is loaded alongside this expansion's rows; after the merge
+//  * The link block of the next expansion is prefetched: the block of the next unexpanded
+//    member of R is loaded together with this expansion's rows; after the merge
 //    ranks, if a fresh key lands ahead of that member, its block is loaded instead. The next
 //    expansion uses the prefetched block only if it names the node it selects.
 //  * Row distances reduce across the 8 lanes of a candidate with DPP moves (same additions,
@@ -281,30 +281,6 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 cnt = uniform(blk[0]);
                 id = static_cast<std::uint32_t>(lane) < cnt ? blk[1 + lane] : 0u;
             }
-            // Prefetch the block of the next open member of R (the next expansion unless a
-            // fresh key lands ahead of it).
-            std::uint64_t spec_key = kSentinel;
-            {
-                int sc;
-                std::uint32_t sl;
-                first_open<CH>(Rk, pc, sc, sl);
-                pref_node = kNone;
-                if (sc >= 0) {
-                    std::uint64_t ks = 0;
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
-                        if (c == sc) {
-                            ks = readlane64(Rk[c], sl);
-                        }
-                    }
-                    spec_key = ks & kKeyMask;
-                    pref_node = key_id(ks);
-                    const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(pref_node) * b0;
-                    pref_cnt = nb2[0];
-                    pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
-                }
-            }
-
             // Fresh neighbours: not in the visited cache.
             const bool valid = static_cast<std::uint32_t>(lane) < cnt;
             bool fresh = false;
@@ -321,6 +297,30 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 fid[lanes_below(fb)] = id;
             }
             wave_lds_fence();
+            // Prefetch the block of the next open member of R (the next expansion unless a
+            // fresh key lands ahead of it). Issued after this block's ids are consumed and just
+            // before the row loads, its latency overlaps theirs.
+            // The load is unconditional (node 0's block when R has no open member) so no
+            // branch merges stale and fresh stage registers.
+            std::uint64_t spec_key = kSentinel;
+            {
+                int sc;
+                std::uint32_t sl;
+                first_open<CH>(Rk, pc, sc, sl);
+                std::uint64_t ks = 0;
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    if (c == sc) {
+                        ks = readlane64(Rk[c], sl);
+                    }
+                }
+                spec_key = sc >= 0 ? (ks & kKeyMask) : kSentinel;
+                pref_node = sc >= 0 ? key_id(ks) : kNone;
+                const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(key_id(ks)) * b0;
+                pref_cnt = nb2[0];
+                pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
+            }
+
             n_dist += nf;
             for (std::uint32_t f0 = 0; f0 < nf; f0 += 8 * RR) {
                 distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid + f0, min(8u * RR, nf - f0), qb, q8, qt,
