@@ -79,6 +79,8 @@ _SIGS = {
     "vdx_host_check_new_vectors": (c_int, [c_char_p, c_u32, c_u64, c_u64, c_ptr, c_u64, c_u64]),
     "vdx_host_normalize_rows": (c_int, [c_ptr, c_u64, c_u64]),
     "vdx_host_distance": (c_float, [c_u32, c_ptr, c_ptr, c_u64]),
+    "vdx_vecs_info": (c_int, [c_char_p, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "vdx_vecs_read": (c_int, [c_char_p, c_ptr, c_u64, c_u64]),
     "vdx_recall_at_k": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_u64, ctypes.POINTER(ctypes.c_double)]),
     "vdx_threshold_recall_at_k": (c_int, [c_ptr, c_u64, c_float, c_u64, c_float,
                                           ctypes.POINTER(ctypes.c_double)]),

@@ -116,6 +116,11 @@ VDX_API int vdx_recall_at_k(const vdx_neighbor* found, uint64_t found_count, con
  * tolerance, and a negative tolerance. */
 VDX_API int vdx_threshold_recall_at_k(const vdx_neighbor* found, uint64_t found_count,
                                       float kth_true, uint64_t k, float tolerance, double* out);
+/* TEXMEX .fvecs / .ivecs (src/io.cpp:30-87): the shape from the first record and the file size,
+ * then every record read with its dimension checked, into dst (rows * dim 4-byte values; float
+ * for .fvecs, int32 for .ivecs). Errors are runtime errors with the path first. */
+VDX_API int vdx_vecs_info(const char* path, uint64_t* rows, uint64_t* dim);
+VDX_API int vdx_vecs_read(const char* path, void* dst, uint64_t rows, uint64_t dim);
 /* Means of the two over nq result rows (no counterpart in the reference, which evaluates one
  * query at a time in bench/hnsw_eval.cpp:513-520): row q holds found_counts[q] results (all
  * found_stride when found_counts is NULL) at found + q * found_stride; truth rows are

@@ -231,3 +231,72 @@ def test_search_after_add_sees_new_vectors():
     index.add_items(extra)
     ids, _ = index.knn_query(extra, k=1)
     assert ids[0, 0] == 100
+
+
+def _upload(arrays):
+    """Uploads numpy FORMAT sections through the C ABI; returns (status, message)."""
+    import ctypes
+    g = arrays
+    view = nat.GraphView(metric=g["metric"], dim=g["dim"], n=g["n"], M=g["M"],
+                         max_level=g["max_level"], entry_point=g["entry_point"],
+                         upper_blocks=g["upper_blocks"], levels=g["levels"].ctypes.data,
+                         layer0=g["layer0"].ctypes.data,
+                         upper=g["upper"].ctypes.data if g["upper"].size else None,
+                         vectors=g["vectors"].ctypes.data)
+    h = ctypes.c_void_p()
+    status = nat.lib.vdx_gpu_hnsw_upload(nat.Context.get(0).handle, ctypes.byref(view), ctypes.byref(h))
+    if status == 0:
+        nat.lib.vdx_gpu_hnsw_free(h)
+    return status, nat.lib.vdx_last_error().decode()
+
+
+def test_upload_applies_check_links_rules():
+    """Links vecdex's loader rejects (src/hnsw_io.cpp:472-504) are rejected at upload too: the
+    kernels rely on ids existing on the block's layer and appearing once per block."""
+    print("case: upload validation, 2000 x 16", flush=True)
+    index = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=64, seed=42)
+    index.add(datasets.integer_matrix(2000, 16, 1))
+    base = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in index.graph_arrays().items()}
+    assert _upload(base)[0] == 0
+    b0 = 1 + 2 * base["M"]
+
+    def mutated(fn):
+        g = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+        fn(g)
+        return _upload(g)
+
+    node = int(np.nonzero(base["layer0"][::b0] >= 2)[0][0])
+    blk = node * b0
+
+    def dup(g):
+        g["layer0"][blk + 2] = g["layer0"][blk + 1]
+    status, msg = mutated(dup)
+    assert status == nat.VDX_INVALID_ARGUMENT and "repeats a node" in msg
+
+    def self_link(g):
+        g["layer0"][blk + 1] = node
+    status, msg = mutated(self_link)
+    assert status == nat.VDX_INVALID_ARGUMENT and "self-link" in msg
+
+    def missing(g):
+        g["layer0"][blk + 1] = g["n"]
+    status, msg = mutated(missing)
+    assert status == nat.VDX_INVALID_ARGUMENT and "missing node" in msg
+
+    def too_many(g):
+        g["layer0"][blk] = 2 * g["M"] + 1
+    status, msg = mutated(too_many)
+    assert status == nat.VDX_INVALID_ARGUMENT and "links on layer 0" in msg
+
+    # an upper-layer link to a node that does not reach that layer
+    levels = base["levels"]
+    top = int(np.argmax(levels))
+    low = int(np.nonzero(levels == 0)[0][0])
+    bu = 1 + base["M"]
+    off = int(levels[:top].sum())
+
+    def below(g):
+        g["upper"][off * bu] = max(1, g["upper"][off * bu])
+        g["upper"][off * bu + 1] = low
+    status, msg = mutated(below)
+    assert status == nat.VDX_INVALID_ARGUMENT and "below the layer" in msg
