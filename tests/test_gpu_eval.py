@@ -33,13 +33,15 @@ def test_eval_sweep(tmp_path, capsys, device_resident):
     assert ev.main(args) == 0
     out = capsys.readouterr()
     lines = out.out.strip().splitlines()
+    lines = lines[next(i for i, ln in enumerate(lines) if ln.startswith("lib,")):]
     assert lines[0] == "lib,dataset,n,dim,M,ef_construction,build_s,memory_mb,k,ef,recall,recall_dist,qps,threads"
     rows = [dict(zip(lines[0].split(","), ln.split(","))) for ln in lines[1:]]
     assert [r["ef"] for r in rows] == ["16", "64"]
     assert all(r["lib"] == "vecdex-mi355x" and r["n"] == "3000" and r["dim"] == "24" for r in rows)
     assert float(rows[1]["recall"]) >= 0.95 and float(rows[1]["recall_dist"]) >= float(rows[1]["recall"])
     fps = dict(re.findall(r"ef (\d+): results fingerprint ([0-9a-f]{16})", out.err))
-    g = vdx.HnswIndex.load(graph_path).graph_arrays()
+    saved = vdx.HnswIndex.load(graph_path)  # graph_arrays() are views into it: keep it alive
+    g = saved.graph_arrays()
     for ef in (16, 64):
         ref, cnt = o.hnsw_search(g, queries, 10, ef)
         assert fps[str(ef)] == f"{ev.fingerprint(ref, cnt):016x}"
