@@ -179,6 +179,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--serving-nq", type=int, default=65536)
+    ap.add_argument("--base10m", action="store_true",
+                    help="also run the 10M x 128 M=32 base (host graph build takes minutes)")
+    ap.add_argument("--n10m", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -259,6 +262,7 @@ def main(argv=None) -> int:
         learned_leg, headline = run_learned(args, vdx, nat, datasets, learned, index, base, queries,
                                             truth, mix, threads, fixed, fixed_best, headline)
     serving_leg = None if args.no_serving else run_serving(args, vdx, nat, datasets, threads)
+    base10m_leg = run_10m(args, vdx, nat, datasets, threads) if args.base10m else None
 
     if args.copy_probe_mb > 0:
         nbytes = args.copy_probe_mb << 20
@@ -337,6 +341,7 @@ def main(argv=None) -> int:
         "sweep": sweep,
         "bruteforce": bf_leg,
         "serving": serving_leg,
+        "base_10m": base10m_leg,
         "gpu_cpu_identical_fraction": identical,
         "wall_ms_per_step": round(wall_ms, 4),
         "ground_truth_s": round(gt_s, 3),
@@ -422,6 +427,83 @@ def run_serving(args, vdx, nat, datasets, threads):
             "speedup_vs_one_launch": round(ms_single / ms_graph, 3),
             "identical_to_one_launch": identical, "tune_s": round(tune_s, 2),
             "classes": classes, "batch_sizes": sizes}
+
+
+def run_10m(args, vdx, nat, datasets, threads):
+    """The 10,000,000 x 128 L2 base (the 1M generator with 4,096 components, graph M=32,
+    efConstruction=200): fixed-ef recall and QPS over 10,000 queries, then 65,536 queries with
+    ef ~ U{16..512} and k ~ U{1, 10, 100} through the tuned class graph against one launch."""
+    t = time.perf_counter()
+    mix = datasets.GaussianMixture(128, 4096, 0.0, 128.0, 4.0, 32.0, mixture_seed=0)
+    base = mix.sample(args.n10m, 1)
+    queries = mix.sample(10_000, 2)
+    log(f"10M: generated {len(base):,} x 128 in {time.perf_counter() - t:.1f} s")
+    CACHE.mkdir(parents=True, exist_ok=True)
+    path = CACHE / f"c3_n{len(base)}_M32_efc200_seed0.vdx"
+    build_s = None
+    if path.exists():
+        index = vdx.HnswIndex.load(path)
+    else:
+        index = vdx.HnswIndex(128, vdx.Metric.L2, M=32, ef_construction=200, seed=0)
+        t = time.perf_counter()
+        step = 1_000_000
+        for r0 in range(0, len(base), step):  # in slices, reporting progress
+            index.add(base[r0:r0 + step], num_threads=threads)
+            log(f"10M: {min(r0 + step, len(base)):,} rows linked, {time.perf_counter() - t:.0f} s")
+        build_s = time.perf_counter() - t
+        log(f"10M: built graph (M=32, efC=200) on {threads} threads in {build_s:.0f} s")
+        index.save(path)
+    t = time.perf_counter()
+    truth, bf = ground_truth(vdx, base, vdx.Metric.L2, queries)
+    log(f"10M: ground truth (BF16 brute force, certified) in {time.perf_counter() - t:.1f} s")
+    del bf
+    run = Runner(nat, index, queries)
+    sweep = []
+    for ef in (16, 32, 48, 64, 96, 128, 256):
+        run.launch(ef)
+        r = recall(run.results(), truth)
+        ms, _ = run.timed(ef, args.steps, args.warmup)
+        sweep.append({"ef": ef, "recall": round(r, 4), "ms": round(ms, 4), "qps": round(10_000 / ms * 1000, 1)})
+        log(f"10M fixed ef {ef}: recall {r:.4f}, {ms:.3f} ms, {sweep[-1]['qps']:,.0f} QPS")
+    at = [x for x in sweep if x["recall"] >= TARGET_RECALL]
+    nq = args.serving_nq
+    rng = np.random.default_rng(6)
+    efs = np.array([16, 32, 64, 128, 256, 512], dtype=np.uint32)[rng.integers(0, 6, nq)]
+    ks = np.array([1, 10, 100], dtype=np.uint32)[rng.integers(0, 3, nq)]
+    sq = mix.sample(nq, 7)
+    ctx = nat.Context.get(0)
+    dq = nat.DeviceArray.from_numpy(sq, ctx)
+    dks = nat.DeviceArray.from_numpy(ks, ctx)
+    defs = nat.DeviceArray.from_numpy(efs, ctx)
+    out = nat.DeviceArray((nq, 100), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(nq, np.uint32, ctx)
+    handle = index.gpu_handle()
+
+    def timed(fn, reps):
+        fn()
+        nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+        nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 2))
+        for _ in range(reps):
+            fn()
+        nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 3))
+        nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+        return nat.elapsed_ms(ctx, 2, 3) / reps
+
+    ms_one = timed(lambda: nat.check(nat.lib.vdx_gpu_hnsw_search_device(
+        handle, dq.ptr, nq, dks.ptr, 0, defs.ptr, 512, 100, out.ptr, 100, cnt.ptr, None, None)), 3)
+    ref = out.numpy()
+    batch = vdx.SearchBatch(index, nq, 100, 512)
+    batch.tune(dq, nq, out, cnt, ks=dks, efs=defs, reps=1)
+    ms_graph = timed(lambda: batch.run(dq, nq, out, cnt, ks=dks, efs=defs, graph=True), 3)
+    identical = bool(out.numpy().tobytes() == ref.tobytes())
+    log(f"10M serving {nq} mixed queries: one launch {ms_one:.1f} ms, tuned graph {ms_graph:.1f} ms "
+        f"({nq / ms_graph * 1000:,.0f} QPS), identical {identical}")
+    del batch
+    return {"workload": f"{len(base):,} x 128 L2 Gaussian mixture (4,096 components), M=32, efC=200",
+            "graph_build_s": None if build_s is None else round(build_s, 1), "fixed_ef_sweep": sweep,
+            "qps_at_recall_0.95": at[0] if at else None,
+            "serving": {"queries": nq, "one_launch_ms": round(ms_one, 2), "tuned_graph_ms": round(ms_graph, 2),
+                        "tuned_graph_qps": round(nq / ms_graph * 1000, 1), "identical_to_one_launch": identical}}
 
 
 def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, mix, threads,
