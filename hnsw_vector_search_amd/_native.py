@@ -73,6 +73,7 @@ _SIGS = {
     "vdx_host_graph_load": (c_int, [c_char_p, c_ptr]),
     "vdx_host_graph_view": (c_int, [c_ptr, ctypes.POINTER(GraphView)]),
     "vdx_host_graph_params": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
+    "vdx_host_graph_changes": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_ptr]),
     "vdx_host_graph_memory_bytes": (c_u64, [c_ptr]),
     "vdx_host_graph_inspect": (c_int, [c_ptr, ctypes.POINTER(GraphReport), c_char_p, c_u64]),
     "vdx_host_graph_free": (None, [c_ptr]),
@@ -100,6 +101,7 @@ _SIGS = {
     "vdx_gpu_event_record": (c_int, [c_ptr, c_u32]),
     "vdx_gpu_event_elapsed": (c_int, [c_ptr, c_u32, c_u32, ctypes.POINTER(c_float)]),
     "vdx_gpu_hnsw_upload": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr]),
+    "vdx_gpu_hnsw_update": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr, c_u64]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),
     "vdx_gpu_hnsw_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_ptr, c_u32, c_ptr, c_u32,
                                     c_ptr, c_u64, c_ptr, c_char_p, ctypes.POINTER(SearchOpts)]),

@@ -134,7 +134,7 @@ struct vdx_gpu_batch {
     hipEvent_t fork = nullptr, join[vdx::dev::kMaxClasses] = {};
     // the captured launch sequence and the call it was captured for
     hipGraphExec_t exec = nullptr;
-    std::uint64_t key[10] = {};
+    std::uint64_t key[11] = {};
 
     ~vdx_gpu_batch() {
         if (device >= 0) {
@@ -287,12 +287,12 @@ int vdx_gpu_hnsw_batch_run(vdx_gpu_batch* b, const float* d_queries, uint64_t nq
             enqueue_all(*b, c, s);
             return;
         }
-        const std::uint64_t key[10] = {reinterpret_cast<std::uint64_t>(d_queries), nq,
+        const std::uint64_t key[11] = {reinterpret_cast<std::uint64_t>(d_queries), nq,
                                        reinterpret_cast<std::uint64_t>(d_ks), k,
                                        reinterpret_cast<std::uint64_t>(d_efs), ef,
                                        reinterpret_cast<std::uint64_t>(d_out), out_stride,
                                        reinterpret_cast<std::uint64_t>(d_out_count),
-                                       reinterpret_cast<std::uint64_t>(s)};
+                                       reinterpret_cast<std::uint64_t>(s), b->index->generation};
         if (b->exec == nullptr || std::memcmp(key, b->key, sizeof(key)) != 0) {
             if (b->exec != nullptr) {
                 check(hipStreamSynchronize(s), "hipStreamSynchronize");

@@ -81,6 +81,55 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     vdx::gpu::launch_beam(h, plan, mode, a, stream);
 }
 
+// The link rules of HnswIndex::check_links (src/hnsw_io.cpp:472-504) for the blocks of one node:
+// counts within capacity, ids naming nodes that exist on the block's layer, no self-links, no
+// id twice in a block. The kernels rely on them: an upper link to a node below the layer would
+// index past that node's blocks, and a repeated id would enter one expansion's fresh batch twice.
+class LinkChecker {
+public:
+    LinkChecker(const vdx_graph_view& g, const char* where) : g_(g), where_(where), stamp_(g.n, 0) {}
+
+    // Node i's layer-0 block and its upper blocks, which start at upper block `off`.
+    void node(std::uint64_t i, std::uint64_t off) {
+        block(g_.layer0 + i * (1 + 2 * g_.M), i, 0, 2 * g_.M);
+        for (std::uint64_t layer = 1; layer <= g_.levels[i]; ++layer) {
+            block(g_.upper + (off + layer - 1) * (1 + g_.M), i, layer, g_.M);
+        }
+    }
+
+private:
+    void block(const std::uint32_t* b, std::uint64_t node, std::uint64_t layer, std::uint64_t cap) {
+        using ull = unsigned long long;
+        if (b[0] > cap) {
+            throw std::invalid_argument(vdx::fmt("%s: node %llu has %u links on layer %llu (max %llu)", where_,
+                                                 static_cast<ull>(node), b[0], static_cast<ull>(layer),
+                                                 static_cast<ull>(cap)));
+        }
+        if (++epoch_ == 0) {
+            std::fill(stamp_.begin(), stamp_.end(), 0u);
+            epoch_ = 1;
+        }
+        for (std::uint32_t j = 0; j < b[0]; ++j) {
+            const std::uint32_t id = b[1 + j];
+            const char* why = id >= g_.n                ? "names a missing node"
+                              : id == node              ? "is a self-link"
+                              : g_.levels[id] < layer   ? "names a node below the layer"
+                              : stamp_[id] == epoch_    ? "repeats a node"
+                                                        : nullptr;
+            if (why != nullptr) {
+                throw std::invalid_argument(vdx::fmt("%s: link %u of node %llu on layer %llu %s", where_, id,
+                                                     static_cast<ull>(node), static_cast<ull>(layer), why));
+            }
+            stamp_[id] = epoch_;
+        }
+    }
+
+    const vdx_graph_view& g_;
+    const char* where_;
+    std::vector<std::uint32_t> stamp_;
+    std::uint32_t epoch_ = 0;
+};
+
 // Validates and prepares nq host queries into pinned staging (rows of dim floats).
 void prepare_queries(const char* where, vdx::Metric metric, std::uint64_t dim, std::uint64_t size,
                      const float* queries, std::uint64_t nq, std::uint64_t query_dim,
@@ -237,46 +286,13 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         require(g->n == 0 || g->levels[g->entry_point] == g->max_level,
                 "vdx_gpu_hnsw_upload: the entry point is not on the top layer");
         // Every block the kernel may follow, with the checks of HnswIndex::check_links
-        // (src/hnsw_io.cpp:472-504): counts within capacity, ids naming nodes that exist on the
-        // block's layer, no self-links, no id twice in a block. The kernels rely on these: an
-        // upper link to a node below the layer would index past that node's blocks, and a
-        // repeated id would enter one expansion's fresh batch twice.
+        // (src/hnsw_io.cpp:472-504).
         {
-            std::vector<std::uint32_t> stamp(g->n, 0);
-            std::uint32_t epoch = 0;
-            const auto check_block = [&](const std::uint32_t* b, std::uint64_t node, std::uint64_t layer,
-                                         std::uint64_t cap) {
-                if (b[0] > cap) {
-                    throw std::invalid_argument(vdx::fmt(
-                        "vdx_gpu_hnsw_upload: node %llu has %u links on layer %llu (max %llu)",
-                        static_cast<unsigned long long>(node), b[0], static_cast<unsigned long long>(layer),
-                        static_cast<unsigned long long>(cap)));
-                }
-                if (++epoch == 0) {
-                    std::fill(stamp.begin(), stamp.end(), 0u);
-                    epoch = 1;
-                }
-                for (std::uint32_t j = 0; j < b[0]; ++j) {
-                    const std::uint32_t id = b[1 + j];
-                    const char* why = id >= g->n               ? "names a missing node"
-                                      : id == node             ? "is a self-link"
-                                      : g->levels[id] < layer  ? "names a node below the layer"
-                                      : stamp[id] == epoch     ? "repeats a node"
-                                                               : nullptr;
-                    if (why != nullptr) {
-                        throw std::invalid_argument(vdx::fmt(
-                            "vdx_gpu_hnsw_upload: link %u of node %llu on layer %llu %s", id,
-                            static_cast<unsigned long long>(node), static_cast<unsigned long long>(layer), why));
-                    }
-                    stamp[id] = epoch;
-                }
-            };
+            LinkChecker check_links(*g, "vdx_gpu_hnsw_upload");
             std::uint64_t off = 0;
             for (std::uint64_t i = 0; i < g->n; ++i) {
-                check_block(g->layer0 + i * (1 + 2 * g->M), i, 0, 2 * g->M);
-                for (std::uint64_t layer = 1; layer <= g->levels[i]; ++layer, ++off) {
-                    check_block(g->upper + off * (1 + g->M), i, layer, g->M);
-                }
+                check_links.node(i, off);
+                off += g->levels[i];
             }
         }
 
@@ -289,6 +305,7 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         h->M = g->M;
         h->max_level = g->n ? g->max_level : 0;
         h->entry_point = g->n ? g->entry_point : 0;
+        h->upper_blocks = g->upper_blocks;
         h->split = vdx::gpu::split_dim(g->dim);
         hipStream_t s = ctx->stream;
         upload_rows(h->vectors, g->vectors, g->n, g->dim, h->split.ld, s);
@@ -311,6 +328,130 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         h->ring.resize(kRingSlots * 256);
         check(hipStreamSynchronize(s), "hipStreamSynchronize(upload)");
         *out = h.release();
+    });
+}
+
+int vdx_gpu_hnsw_update(vdx_gpu_hnsw* h, const vdx_graph_view* g, const uint32_t* nodes, uint64_t count) {
+    return guarded([&] {
+        require(h != nullptr && g != nullptr, "vdx_gpu_hnsw_update: null argument");
+        require(count == 0 || nodes != nullptr, "vdx_gpu_hnsw_update: null node list");
+        require(g->metric == h->metric && g->dim == h->dim && g->M == h->M,
+                "vdx_gpu_hnsw_update: the graph's metric, dimension or M differs from the device copy's");
+        require(g->n >= h->n, "vdx_gpu_hnsw_update: the graph has fewer nodes than the device copy");
+        require(g->n < 0x80000000ull, "vdx_gpu_hnsw_update: more than 2^31 - 1 nodes");
+        require(g->n == 0 || (g->levels && g->layer0 && g->vectors), "vdx_gpu_hnsw_update: null section");
+        require(g->upper_blocks == 0 || g->upper != nullptr, "vdx_gpu_hnsw_update: null upper section");
+        require(g->n == 0 || g->entry_point < g->n, "vdx_gpu_hnsw_update: entry point out of range");
+        if (g->n == 0) return;
+        const std::uint64_t old_n = h->n;
+        // Upper offsets: old nodes keep theirs (blocks are appended in id order), new ones follow.
+        std::vector<std::uint32_t> offsets(g->n);
+        std::uint64_t acc = 0;
+        for (std::uint64_t i = 0; i < g->n; ++i) {
+            offsets[i] = static_cast<std::uint32_t>(acc);
+            require(g->levels[i] <= g->max_level, "vdx_gpu_hnsw_update: a level is above max_level");
+            acc += g->levels[i];
+        }
+        require(acc == g->upper_blocks, "vdx_gpu_hnsw_update: levels do not add up to upper_blocks");
+        require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_update: too many upper blocks");
+        require(old_n == 0 || offsets[old_n - 1] + g->levels[old_n - 1] == h->upper_blocks,
+                "vdx_gpu_hnsw_update: the old nodes' upper blocks moved");
+        require(g->levels[g->entry_point] == g->max_level,
+                "vdx_gpu_hnsw_update: the entry point is not on the top layer");
+        // The changed old nodes (ascending, below old_n); every new node counts as changed.
+        std::vector<std::uint32_t> changed;
+        for (std::uint64_t i = 0; i < count; ++i) {
+            require(nodes[i] < g->n, "vdx_gpu_hnsw_update: a listed node does not exist");
+            require(i == 0 || nodes[i] > nodes[i - 1], "vdx_gpu_hnsw_update: the node list is not ascending");
+            if (nodes[i] < old_n) changed.push_back(nodes[i]);
+        }
+        {
+            LinkChecker check_links(*g, "vdx_gpu_hnsw_update");
+            for (const std::uint32_t i : changed) check_links.node(i, offsets[i]);
+            for (std::uint64_t i = old_n; i < g->n; ++i) check_links.node(i, offsets[i]);
+        }
+
+        DeviceGuard guard(h->ctx->device);
+        hipStream_t s = h->ctx->stream;
+        ++h->generation;  // from here on buffers may move, even if a step below fails
+        const std::uint64_t b0 = 1 + 2 * g->M, bu = 1 + g->M, ld = h->split.ld;
+        // Grows a buffer to hold `need` bytes with a quarter of headroom, keeping its contents.
+        const auto ensure = [&](DeviceBuffer& buf, std::uint64_t need) {
+            if (need <= buf.bytes()) return;
+            DeviceBuffer grown(need + need / 4);
+            if (buf.bytes() > 0) {
+                check(hipMemcpyAsync(grown.as<void>(), buf.as<void>(), buf.bytes(), hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync(grow)");
+            }
+            check(hipStreamSynchronize(s), "hipStreamSynchronize(grow)");
+            buf.swap(grown);
+        };
+        ensure(h->vectors, g->n * ld * 4);
+        ensure(h->layer0, g->n * b0 * 4);
+        ensure(h->upper, std::max<std::uint64_t>(1, g->upper_blocks) * bu * 4);
+        ensure(h->upper_offset, g->n * 4);
+        const std::uint64_t add = g->n - old_n;
+        if (add > 0) {
+            float* vdst = h->vectors.as<float>() + old_n * ld;
+            if (ld != g->dim) {
+                check(hipMemsetAsync(vdst, 0, add * ld * 4, s), "hipMemsetAsync");
+            }
+            check(hipMemcpy2DAsync(vdst, ld * 4, g->vectors + old_n * g->dim, g->dim * 4, g->dim * 4, add,
+                                   hipMemcpyHostToDevice, s),
+                  "hipMemcpy2DAsync(vectors)");
+            check(hipMemcpyAsync(h->layer0.as<std::uint32_t>() + old_n * b0, g->layer0 + old_n * b0, add * b0 * 4,
+                                 hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(layer0)");
+            check(hipMemcpyAsync(h->upper_offset.as<std::uint32_t>() + old_n, offsets.data() + old_n, add * 4,
+                                 hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(offsets)");
+            const std::uint64_t new_blocks = g->upper_blocks - h->upper_blocks;
+            if (new_blocks > 0) {
+                check(hipMemcpyAsync(h->upper.as<std::uint32_t>() + h->upper_blocks * bu,
+                                     g->upper + h->upper_blocks * bu, new_blocks * bu * 4, hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync(upper)");
+            }
+        }
+        if (!changed.empty()) {
+            // The changed old nodes' blocks, gathered on the host and scattered on the device.
+            std::vector<std::uint32_t> ublocks;
+            for (const std::uint32_t i : changed) {
+                for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(offsets[i] + l);
+            }
+            const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
+            vdx::gpu::PinnedBuffer staging;
+            staging.reserve(words * 4);
+            std::uint32_t* st = staging.as<std::uint32_t>();
+            std::memcpy(st, changed.data(), changed.size() * 4);
+            std::uint32_t* l0 = st + changed.size();
+            for (std::size_t r = 0; r < changed.size(); ++r) {
+                std::memcpy(l0 + r * b0, g->layer0 + static_cast<std::uint64_t>(changed[r]) * b0, b0 * 4);
+            }
+            std::uint32_t* uids = l0 + changed.size() * b0;
+            std::memcpy(uids, ublocks.data(), ublocks.size() * 4);
+            std::uint32_t* ub = uids + ublocks.size();
+            for (std::size_t r = 0; r < ublocks.size(); ++r) {
+                std::memcpy(ub + r * bu, g->upper + static_cast<std::uint64_t>(ublocks[r]) * bu, bu * 4);
+            }
+            DeviceBuffer dst(words * 4);
+            check(hipMemcpyAsync(dst.as<void>(), st, words * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(changes)");
+            const std::uint32_t* d = dst.as<std::uint32_t>();
+            check(vdx::dev::launch_scatter_rows(h->layer0.as<std::uint32_t>(), d + changed.size(), d, changed.size(),
+                                                static_cast<std::uint32_t>(b0), s),
+                  "scatter(layer0)");
+            check(vdx::dev::launch_scatter_rows(h->upper.as<std::uint32_t>(),
+                                                d + changed.size() * (1 + b0) + ublocks.size(),
+                                                d + changed.size() * (1 + b0), ublocks.size(),
+                                                static_cast<std::uint32_t>(bu), s),
+                  "scatter(upper)");
+            check(hipStreamSynchronize(s), "hipStreamSynchronize(update)");
+        } else {
+            check(hipStreamSynchronize(s), "hipStreamSynchronize(update)");
+        }
+        h->n = g->n;
+        h->upper_blocks = g->upper_blocks;
+        h->max_level = g->max_level;
+        h->entry_point = g->entry_point;
     });
 }
 

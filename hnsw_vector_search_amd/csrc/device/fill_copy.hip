@@ -33,6 +33,19 @@ __global__ __launch_bounds__(256) void copy_bytes_kernel(unsigned char* dst, con
     }
 }
 
+// dst[ids[r] * row_len + c] = src[r * row_len + c]: rows of u32 scattered by index.
+__global__ __launch_bounds__(256) void scatter_rows_kernel(std::uint32_t* dst, const std::uint32_t* src,
+                                                           const std::uint32_t* ids, std::uint64_t rows,
+                                                           std::uint32_t row_len) {
+    const std::uint64_t total = rows * row_len;
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += stride) {
+        const std::uint64_t r = i / row_len;
+        dst[static_cast<std::uint64_t>(ids[r]) * row_len + (i - r * row_len)] = src[i];
+    }
+}
+
 namespace {
 unsigned blocks_for(std::uint64_t items) {
     const std::uint64_t b = (items + 255) / 256;
@@ -43,6 +56,14 @@ unsigned blocks_for(std::uint64_t items) {
 hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(count)), dim3(256), 0, s, dst, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, const std::uint32_t* ids,
+                               std::uint64_t rows, std::uint32_t row_len, hipStream_t s) {
+    if (rows == 0 || row_len == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_rows_kernel, dim3(blocks_for(rows * row_len)), dim3(256), 0, s, dst, src, ids,
+                       rows, row_len);
     return hipGetLastError();
 }
 

@@ -9,6 +9,9 @@
 namespace vdx::dev {
 
 hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_t count, hipStream_t s);
+// Rows of row_len u32 from src scattered to dst rows ids[0..rows) (ids valid row indices of dst).
+hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, const std::uint32_t* ids,
+                               std::uint64_t rows, std::uint32_t row_len, hipStream_t s);
 // dst and src 16-byte aligned (device allocations are).
 hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
 

@@ -12,6 +12,10 @@ struct vdx_gpu_hnsw {
     vdx_gpu_ctx* ctx = nullptr;
     std::uint32_t metric = 0;
     std::uint64_t dim = 0, n = 0, M = 0, max_level = 0, entry_point = 0;
+    std::uint64_t upper_blocks = 0;  // U of the uploaded graph (its upper section's blocks)
+    // Counts vdx_gpu_hnsw_update calls: launches captured before one name buffers and sizes it
+    // may have replaced.
+    std::uint64_t generation = 0;
     vdx::gpu::DimSplit split{};
     vdx::gpu::DeviceBuffer vectors, layer0, upper, upper_offset;
     // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "../host/common.hpp"
@@ -44,6 +45,10 @@ public:
     }
     void resize(std::size_t bytes);
     void release() noexcept;
+    void swap(DeviceBuffer& o) noexcept {
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+    }
     template <typename T>
     T* as() const noexcept {
         return static_cast<T*>(ptr_);

@@ -101,6 +101,19 @@ int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, ui
     });
 }
 
+int vdx_host_graph_changes(const vdx_host_graph* g, uint64_t since_version, uint32_t* nodes,
+                           uint64_t capacity, uint64_t* count) {
+    return guarded([&] {
+        require(g != nullptr && count != nullptr, "vdx_host_graph_changes: null argument");
+        const std::vector<vdx::Id> changed = g->graph->changed_since(since_version);
+        *count = changed.size();
+        if (nodes != nullptr) {
+            require(capacity >= changed.size(), "vdx_host_graph_changes: capacity is below the count");
+            std::memcpy(nodes, changed.data(), changed.size() * sizeof(uint32_t));
+        }
+    });
+}
+
 uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g) {
     return g == nullptr ? 0 : g->graph->memory_bytes();
 }

@@ -274,6 +274,7 @@ std::unique_ptr<HnswGraph> HnswGraph::load(const std::string& path) {
 
     std::uint64_t blocks = 0;
     g->upper_offset_.resize(nodes);
+    g->touched_.assign(nodes, 0u);
     for (std::size_t node = 0; node < nodes; ++node) {
         if (g->levels_[node] > max_level) {
             fail(path, fmt("node %zu has level %u, above the max level %llu", node,

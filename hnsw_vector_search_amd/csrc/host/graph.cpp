@@ -162,8 +162,19 @@ void HnswGraph::store(const float* vector) {
     }
     levels_.push_back(static_cast<std::uint8_t>(level));
     upper_offset_.push_back(static_cast<std::uint32_t>(first_block));
+    touched_.push_back(static_cast<std::uint32_t>(version_));
     layer0_.resize(layer0_.size() + 1 + 2 * params_.M, Id{0});
     upper_.resize(upper_.size() + level * (1 + params_.M), Id{0});
+}
+
+std::vector<Id> HnswGraph::changed_since(std::uint64_t version) const {
+    std::vector<Id> out;
+    for (std::size_t i = 0; i < touched_.size(); ++i) {
+        if (touched_[i] > version) {
+            out.push_back(static_cast<Id>(i));
+        }
+    }
+    return out;
 }
 
 void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads) {
@@ -174,6 +185,7 @@ void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads
     vectors_.reserve(total * dim_);
     levels_.reserve(total);
     upper_offset_.reserve(total);
+    touched_.reserve(total);
     layer0_.reserve(total * (1 + 2 * params_.M));
     const std::size_t threads = std::min(resolve_threads(num_threads), rows);
     if (threads <= 1) {
@@ -341,6 +353,7 @@ void HnswGraph::link_back(Id node, std::size_t layer, const std::vector<Neighbor
             lock = std::unique_lock<std::mutex>(build_locks_->links[neighbor.id]);
         }
         Id* b = block(neighbor.id, layer);
+        touched_[neighbor.id] = static_cast<std::uint32_t>(version_);
         const std::size_t count = b[0];
         if (count < max_links(layer)) {
             b[1 + count] = node;

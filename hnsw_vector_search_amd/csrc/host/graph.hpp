@@ -74,6 +74,9 @@ public:
 
     // Bumped by every add(); a device copy remembers the version it was made from.
     std::uint64_t version() const noexcept { return version_; }
+    // Nodes whose link blocks (any layer) were written by an add() after `version`: the nodes
+    // it stored and the existing nodes it linked back to, in id order.
+    std::vector<Id> changed_since(std::uint64_t version) const;
 
     struct Scratch;
     class ScratchPool;
@@ -119,6 +122,7 @@ private:
     std::vector<Id> upper_;
     std::vector<std::uint32_t> upper_offset_;
     std::vector<float> vectors_;
+    std::vector<std::uint32_t> touched_;  // per node: the version that last wrote its links
     Id entry_point_ = 0;
     std::size_t max_level_ = 0;
     std::uint64_t version_ = 0;

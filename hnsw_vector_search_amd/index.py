@@ -160,6 +160,15 @@ class _HostGraph:
                                                 ctypes.byref(version)))
         return efc.value, seed.value, version.value
 
+    def changes(self, since: int) -> np.ndarray:
+        """Ascending ids of the nodes whose link blocks the adds after version `since` wrote."""
+        count = ctypes.c_uint64()
+        nat.check(nat.lib.vdx_host_graph_changes(self.handle, since, None, 0, ctypes.byref(count)))
+        out = np.zeros(count.value, dtype=np.uint32)
+        nat.check(nat.lib.vdx_host_graph_changes(self.handle, since, nat.ptr(out), out.size,
+                                                 ctypes.byref(count)))
+        return out
+
 
 class HnswIndex:
     """vecdex::HnswIndex with search/search_batch on the GPU.
@@ -245,19 +254,24 @@ class HnswIndex:
 
     # ---- GPU copy ---------------------------------------------------------------------
     def gpu_handle(self):
-        """The HBM copy of the graph, (re)uploaded when the host graph changed."""
+        """The HBM copy of the graph: uploaded on first use, then brought up to date after each
+        add by appending the new nodes and rewriting the blocks the add linked back into
+        (vdx_gpu_hnsw_update), so the handle stays the same for the index's lifetime."""
         _, _, version = self._graph.params()
         with self._gpu_lock:
-            if self._gpu is None or self._gpu_version != version:
+            if self._gpu is None:
                 ctx = nat.Context.get(self._device)
-                if self._gpu is not None:
-                    nat.lib.vdx_gpu_hnsw_free(self._gpu)
-                    self._gpu = None
                 h = ctypes.c_void_p()
                 view = self._view()
                 nat.check(nat.lib.vdx_gpu_hnsw_upload(ctx.handle, ctypes.byref(view),
                                                       ctypes.byref(h)))
                 self._gpu, self._gpu_version = h, version
+            elif self._gpu_version != version:
+                changed = self._graph.changes(self._gpu_version)
+                view = self._view()
+                nat.check(nat.lib.vdx_gpu_hnsw_update(self._gpu, ctypes.byref(view),
+                                                      nat.ptr(changed), changed.size))
+                self._gpu_version = version
             return self._gpu
 
     def __del__(self):

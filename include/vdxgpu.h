@@ -85,6 +85,11 @@ VDX_API int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out);
 VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
                           uint64_t* version);
 VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
+/* Nodes whose link blocks an add() after `since_version` wrote (the nodes it stored and those it
+ * linked back to), ascending; with nodes NULL only the count. No counterpart in vecdex: it
+ * drives the incremental device update (vdx_gpu_hnsw_update). */
+VDX_API int vdx_host_graph_changes(const vdx_host_graph* g, uint64_t since_version, uint32_t* nodes,
+                                   uint64_t capacity, uint64_t* count);
 
 /* HnswIndex::inspect (include/vecdex/hnsw.hpp:102; src/hnsw.cpp:527-594). */
 typedef struct vdx_graph_report {
@@ -163,6 +168,13 @@ VDX_API int vdx_gpu_event_elapsed(vdx_gpu_ctx* ctx, uint32_t from, uint32_t to, 
 typedef struct vdx_gpu_hnsw vdx_gpu_hnsw;
 
 VDX_API int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* graph, vdx_gpu_hnsw** out);
+/* Brings the device copy up to a graph that extends the uploaded one (HnswIndex::add appends
+ * nodes and rewrites the link blocks of the nodes it links back to; no counterpart in vecdex,
+ * whose index has no device copy): new nodes' rows, blocks and offsets are appended and the
+ * blocks of the listed changed nodes (vdx_host_graph_changes) rewritten, all checked with the
+ * upload's link rules. Device buffers grow with headroom. Not concurrent with searches on it. */
+VDX_API int vdx_gpu_hnsw_update(vdx_gpu_hnsw* index, const vdx_graph_view* graph, const uint32_t* nodes,
+                                uint64_t count);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 
 /* Knobs of one search launch. Zero-initialise for defaults (autotuned). */
