@@ -214,6 +214,13 @@ class HnswIndex:
         arr = _as_matrix(vectors)
         nat.check(nat.lib.vdx_host_graph_add(self._graph.handle, nat.ptr(arr), arr.shape[0],
                                              arr.shape[1], num_threads))
+        self._refresh_gpu()
+
+    def _refresh_gpu(self) -> None:
+        """Brings an existing HBM copy up to the host graph at once (an index never searched on
+        the GPU uploads on its first search)."""
+        if self._gpu is not None:
+            self.gpu_handle()
 
     def inspect(self) -> GraphReport:
         r = nat.GraphReport()
@@ -534,6 +541,8 @@ class Index:
             if rows.shape[0]:
                 nat.check(nat.lib.vdx_host_graph_add(self._core._graph.handle, nat.ptr(rows),
                                                      rows.shape[0], rows.shape[1], threads))
+                # the device copy follows under the same write lock, so no search sees it stale
+                self._core._refresh_gpu()
         finally:
             self._lock.release_write()
         ids[:] = np.arange(first, first + rows.shape[0], dtype=np.int64)
