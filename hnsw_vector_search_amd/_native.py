@@ -41,6 +41,21 @@ class GraphView(ctypes.Structure):
                 ("vectors", c_ptr)]
 
 
+SYNC_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(GraphView), ctypes.POINTER(c_u32), c_u64)
+SEARCH_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(c_u32), c_u64, c_u32, c_ptr,
+                             ctypes.POINTER(c_u32))
+
+
+class BatchSearcher(ctypes.Structure):
+    _fields_ = [("ctx", c_ptr), ("sync", SYNC_FN), ("search", SEARCH_FN)]
+
+
+class BuildPlan(ctypes.Structure):
+    """vdx_build_plan: host_prefix, batch_min, batch_max, batch_fraction."""
+    _fields_ = [("host_prefix", c_u64), ("batch_min", c_u64), ("batch_max", c_u64),
+                ("batch_fraction", c_double)]
+
+
 class GraphReport(ctypes.Structure):
     _fields_ = [("reachable", c_u64), ("max_level", c_u64), ("violation_count", c_u64),
                 ("mean_degree0", c_double)]
@@ -74,6 +89,8 @@ _SIGS = {
     "vdx_host_graph_view": (c_int, [c_ptr, ctypes.POINTER(GraphView)]),
     "vdx_host_graph_params": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
     "vdx_host_graph_changes": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_ptr]),
+    "vdx_host_graph_add_batched": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BatchSearcher),
+                                           ctypes.POINTER(BuildPlan), c_u64]),
     "vdx_host_graph_memory_bytes": (c_u64, [c_ptr]),
     "vdx_host_graph_inspect": (c_int, [c_ptr, ctypes.POINTER(GraphReport), c_char_p, c_u64]),
     "vdx_host_graph_free": (None, [c_ptr]),
@@ -102,6 +119,8 @@ _SIGS = {
     "vdx_gpu_event_elapsed": (c_int, [c_ptr, c_u32, c_u32, ctypes.POINTER(c_float)]),
     "vdx_gpu_hnsw_upload": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr]),
     "vdx_gpu_hnsw_update": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr, c_u64]),
+    "vdx_gpu_hnsw_build_add": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BuildPlan),
+                                       c_u64]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),
     "vdx_gpu_hnsw_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_ptr, c_u32, c_ptr, c_u32,
                                     c_ptr, c_u64, c_ptr, c_char_p, ctypes.POINTER(SearchOpts)]),

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "../host/batched_build.hpp"
 #include "../host/capi_common.hpp"
 #include "../host/validate.hpp"
 #include "beam_args.hpp"
@@ -87,7 +88,14 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
 // index past that node's blocks, and a repeated id would enter one expansion's fresh batch twice.
 class LinkChecker {
 public:
-    LinkChecker(const vdx_graph_view& g, const char* where) : g_(g), where_(where), stamp_(g.n, 0) {}
+    // `marks` (one per node, grown here) and `epoch` persist between checkers of one index.
+    LinkChecker(const vdx_graph_view& g, const char* where, std::vector<std::uint32_t>& marks,
+                std::uint32_t& epoch)
+        : g_(g), where_(where), marks_(marks), epoch_(epoch) {
+        if (marks_.size() < g.n) {
+            marks_.resize(g.n, 0u);
+        }
+    }
 
     // Node i's layer-0 block and its upper blocks, which start at upper block `off`.
     void node(std::uint64_t i, std::uint64_t off) {
@@ -106,7 +114,7 @@ private:
                                                  static_cast<ull>(cap)));
         }
         if (++epoch_ == 0) {
-            std::fill(stamp_.begin(), stamp_.end(), 0u);
+            std::fill(marks_.begin(), marks_.end(), 0u);
             epoch_ = 1;
         }
         for (std::uint32_t j = 0; j < b[0]; ++j) {
@@ -114,21 +122,204 @@ private:
             const char* why = id >= g_.n                ? "names a missing node"
                               : id == node              ? "is a self-link"
                               : g_.levels[id] < layer   ? "names a node below the layer"
-                              : stamp_[id] == epoch_    ? "repeats a node"
+                              : marks_[id] == epoch_    ? "repeats a node"
                                                         : nullptr;
             if (why != nullptr) {
                 throw std::invalid_argument(vdx::fmt("%s: link %u of node %llu on layer %llu %s", where_, id,
                                                      static_cast<ull>(node), static_cast<ull>(layer), why));
             }
-            stamp_[id] = epoch_;
+            marks_[id] = epoch_;
         }
     }
 
     const vdx_graph_view& g_;
     const char* where_;
-    std::vector<std::uint32_t> stamp_;
-    std::uint32_t epoch_ = 0;
+    std::vector<std::uint32_t>& marks_;
+    std::uint32_t& epoch_;
 };
+
+// Brings the device copy up to a graph that extends it (vdx_gpu_hnsw_update): the old nodes keep
+// their levels (so their upper blocks keep their positions); new nodes are appended and the
+// listed old nodes' blocks rewritten, all under the upload's link rules. Host work is
+// proportional to what changed (plus one comparison of the old levels).
+void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t* nodes, std::uint64_t count) {
+    require(h != nullptr && g != nullptr, "vdx_gpu_hnsw_update: null argument");
+    require(count == 0 || nodes != nullptr, "vdx_gpu_hnsw_update: null node list");
+    require(g->metric == h->metric && g->dim == h->dim && g->M == h->M,
+            "vdx_gpu_hnsw_update: the graph's metric, dimension or M differs from the device copy's");
+    require(g->n >= h->n, "vdx_gpu_hnsw_update: the graph has fewer nodes than the device copy");
+    require(g->n < 0x80000000ull, "vdx_gpu_hnsw_update: more than 2^31 - 1 nodes");
+    require(g->n == 0 || (g->levels && g->layer0 && g->vectors), "vdx_gpu_hnsw_update: null section");
+    require(g->upper_blocks == 0 || g->upper != nullptr, "vdx_gpu_hnsw_update: null upper section");
+    require(g->n == 0 || g->entry_point < g->n, "vdx_gpu_hnsw_update: entry point out of range");
+    if (g->n == 0) return;
+    const std::uint64_t old_n = h->n;
+    require(std::memcmp(g->levels, h->levels.data(), old_n) == 0,
+            "vdx_gpu_hnsw_update: a level of a node in the device copy changed");
+    // New nodes' upper blocks follow the old ones in id order. A new node's level may exceed
+    // max_level while a build has stored it but not linked it yet: the kernels never read
+    // levels, and the link rules below keep every followed link on its node's layers.
+    std::vector<std::uint32_t> new_offsets(g->n - old_n);
+    std::uint64_t acc = h->upper_blocks;
+    for (std::uint64_t i = old_n; i < g->n; ++i) {
+        new_offsets[i - old_n] = static_cast<std::uint32_t>(acc);
+        acc += g->levels[i];
+    }
+    require(acc == g->upper_blocks, "vdx_gpu_hnsw_update: levels do not add up to upper_blocks");
+    require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_update: too many upper blocks");
+    require(g->levels[g->entry_point] == g->max_level, "vdx_gpu_hnsw_update: the entry point is not on the top layer");
+    // The changed old nodes (ascending, below old_n); every new node counts as changed.
+    std::vector<std::uint32_t> changed;
+    for (std::uint64_t i = 0; i < count; ++i) {
+        require(nodes[i] < g->n, "vdx_gpu_hnsw_update: a listed node does not exist");
+        require(i == 0 || nodes[i] > nodes[i - 1], "vdx_gpu_hnsw_update: the node list is not ascending");
+        if (nodes[i] < old_n) changed.push_back(nodes[i]);
+    }
+    {
+        LinkChecker check_links(*g, "vdx_gpu_hnsw_update", h->link_marks, h->link_epoch);
+        for (const std::uint32_t i : changed) check_links.node(i, h->offsets[i]);
+        for (std::uint64_t i = old_n; i < g->n; ++i) check_links.node(i, new_offsets[i - old_n]);
+    }
+
+    DeviceGuard guard(h->ctx->device);
+    hipStream_t s = h->ctx->stream;
+    ++h->generation;  // from here on buffers may move, even if a step below fails
+    const std::uint64_t b0 = 1 + 2 * g->M, bu = 1 + g->M, ld = h->split.ld;
+    // Grows a buffer to hold `need` bytes with a quarter of headroom, keeping its contents.
+    const auto ensure = [&](DeviceBuffer& buf, std::uint64_t need) {
+        if (need <= buf.bytes()) return;
+        DeviceBuffer grown(need + need / 4);
+        if (buf.bytes() > 0) {
+            check(hipMemcpyAsync(grown.as<void>(), buf.as<void>(), buf.bytes(), hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync(grow)");
+        }
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(grow)");
+        buf.swap(grown);
+    };
+    ensure(h->vectors, g->n * ld * 4);
+    ensure(h->layer0, g->n * b0 * 4);
+    ensure(h->upper, std::max<std::uint64_t>(1, g->upper_blocks) * bu * 4);
+    ensure(h->upper_offset, g->n * 4);
+    const std::uint64_t add = g->n - old_n;
+    if (add > 0) {
+        float* vdst = h->vectors.as<float>() + old_n * ld;
+        if (ld != g->dim) {
+            check(hipMemsetAsync(vdst, 0, add * ld * 4, s), "hipMemsetAsync");
+        }
+        check(hipMemcpy2DAsync(vdst, ld * 4, g->vectors + old_n * g->dim, g->dim * 4, g->dim * 4, add,
+                               hipMemcpyHostToDevice, s),
+              "hipMemcpy2DAsync(vectors)");
+        check(hipMemcpyAsync(h->layer0.as<std::uint32_t>() + old_n * b0, g->layer0 + old_n * b0, add * b0 * 4,
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(layer0)");
+        check(hipMemcpyAsync(h->upper_offset.as<std::uint32_t>() + old_n, new_offsets.data(), add * 4,
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(offsets)");
+        const std::uint64_t new_blocks = g->upper_blocks - h->upper_blocks;
+        if (new_blocks > 0) {
+            check(hipMemcpyAsync(h->upper.as<std::uint32_t>() + h->upper_blocks * bu, g->upper + h->upper_blocks * bu,
+                                 new_blocks * bu * 4, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(upper)");
+        }
+    }
+    if (!changed.empty()) {
+        // The changed old nodes' blocks, gathered on the host and scattered on the device.
+        std::vector<std::uint32_t> ublocks;
+        for (const std::uint32_t i : changed) {
+            for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(h->offsets[i] + l);
+        }
+        const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
+        h->staging.reserve(words * 6);  // with headroom: updates of a batched build grow
+        std::uint32_t* st = h->staging.as<std::uint32_t>();
+        std::memcpy(st, changed.data(), changed.size() * 4);
+        std::uint32_t* l0 = st + changed.size();
+        for (std::size_t r = 0; r < changed.size(); ++r) {
+            std::memcpy(l0 + r * b0, g->layer0 + static_cast<std::uint64_t>(changed[r]) * b0, b0 * 4);
+        }
+        std::uint32_t* uids = l0 + changed.size() * b0;
+        std::memcpy(uids, ublocks.data(), ublocks.size() * 4);
+        std::uint32_t* ub = uids + ublocks.size();
+        for (std::size_t r = 0; r < ublocks.size(); ++r) {
+            std::memcpy(ub + r * bu, g->upper + static_cast<std::uint64_t>(ublocks[r]) * bu, bu * 4);
+        }
+        h->changes.reserve(words * 6);
+        check(hipMemcpyAsync(h->changes.as<void>(), st, words * 4, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(changes)");
+        const std::uint32_t* d = h->changes.as<std::uint32_t>();
+        check(vdx::dev::launch_scatter_rows(h->layer0.as<std::uint32_t>(), d + changed.size(), d, changed.size(),
+                                            static_cast<std::uint32_t>(b0), s),
+              "scatter(layer0)");
+        check(vdx::dev::launch_scatter_rows(h->upper.as<std::uint32_t>(), d + changed.size() * (1 + b0) + ublocks.size(),
+                                            d + changed.size() * (1 + b0), ublocks.size(),
+                                            static_cast<std::uint32_t>(bu), s),
+              "scatter(upper)");
+    }
+    check(hipStreamSynchronize(s), "hipStreamSynchronize(update)");
+    h->levels.insert(h->levels.end(), g->levels + old_n, g->levels + g->n);
+    h->offsets.insert(h->offsets.end(), new_offsets.begin(), new_offsets.end());
+    h->n = g->n;
+    h->upper_blocks = g->upper_blocks;
+    h->max_level = g->max_level;
+    h->entry_point = g->entry_point;
+}
+
+// The GPU as the searcher of a batched build: the device copy kept current by update_index, each
+// batch's insertion searches one beam launch over the nodes' own vectors gathered from HBM.
+struct BuildSearch {
+    vdx_gpu_hnsw* h = nullptr;
+    DeviceBuffer ids, queries, out, count, queue;
+    vdx::gpu::PinnedBuffer h_ids, h_out, h_count;
+};
+
+int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* changed, std::uint64_t count) {
+    return guarded([&] { update_index(static_cast<BuildSearch*>(ctx)->h, view, changed, count); });
+}
+
+int build_search(void* ctx, const std::uint32_t* nodes, std::uint64_t count, std::uint32_t ef, vdx_neighbor* out,
+                 std::uint32_t* counts) {
+    return guarded([&] {
+        BuildSearch& b = *static_cast<BuildSearch*>(ctx);
+        vdx_gpu_hnsw& h = *b.h;
+        if (ef > kMaxBeamEf) {
+            throw std::invalid_argument(vdx::fmt("HnswIndex::add: ef_construction %u is above the GPU path's limit of %u",
+                                                 ef, kMaxBeamEf));
+        }
+        require(count < 0x80000000ull, "HnswIndex::add: batch too large");
+        for (std::uint64_t i = 0; i < count; ++i) {
+            require(nodes[i] < h.n, "HnswIndex::add: a batch node is not in the device copy");
+        }
+        if (count == 0) return;
+        DeviceGuard guard(h.ctx->device);
+        hipStream_t s = h.ctx->stream;
+        // sizes with headroom: batches grow
+        const auto grow = [](std::uint64_t bytes) { return bytes + bytes / 2; };
+        b.h_ids.reserve(grow(count * 4));
+        b.h_out.reserve(grow(count * ef * sizeof(vdx_neighbor)));
+        b.h_count.reserve(grow(count * 4));
+        b.ids.reserve(grow(count * 4));
+        b.queries.reserve(grow(count * h.dim * 4));
+        b.out.reserve(grow(count * ef * sizeof(vdx_neighbor)));
+        b.count.reserve(grow(count * 4));
+        b.queue.reserve(64);
+        std::memcpy(b.h_ids.as<void>(), nodes, count * 4);
+        check(hipMemcpyAsync(b.ids.as<void>(), b.h_ids.as<void>(), count * 4, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(batch ids)");
+        check(vdx::dev::launch_gather_rows(b.queries.as<float>(), static_cast<std::uint32_t>(h.dim), h.vectors.as<float>(),
+                                           h.split.ld, b.ids.as<std::uint32_t>(), count,
+                                           static_cast<std::uint32_t>(h.dim), s),
+              "gather(batch vectors)");
+        enqueue_beam(h, b.queries.as<float>(), count, nullptr, ef, nullptr, ef, ef, b.out.as<vdx::dev::Neighbor>(), ef,
+                     b.count.as<std::uint32_t>(), b.queue.as<std::uint32_t>(), nullptr, nullptr, s);
+        check(hipMemcpyAsync(b.h_out.as<void>(), b.out.as<void>(), count * ef * sizeof(vdx_neighbor),
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(batch results)");
+        check(hipMemcpyAsync(b.h_count.as<void>(), b.count.as<void>(), count * 4, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(batch counts)");
+        check(hipStreamSynchronize(s), "batch search");
+        std::memcpy(out, b.h_out.as<void>(), count * ef * sizeof(vdx_neighbor));
+        std::memcpy(counts, b.h_count.as<void>(), count * 4);
+    });
+}
 
 // Validates and prepares nq host queries into pinned staging (rows of dim floats).
 void prepare_queries(const char* where, vdx::Metric metric, std::uint64_t dim, std::uint64_t size,
@@ -287,8 +478,9 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
                 "vdx_gpu_hnsw_upload: the entry point is not on the top layer");
         // Every block the kernel may follow, with the checks of HnswIndex::check_links
         // (src/hnsw_io.cpp:472-504).
+        auto h = std::make_unique<vdx_gpu_hnsw>();
         {
-            LinkChecker check_links(*g, "vdx_gpu_hnsw_upload");
+            LinkChecker check_links(*g, "vdx_gpu_hnsw_upload", h->link_marks, h->link_epoch);
             std::uint64_t off = 0;
             for (std::uint64_t i = 0; i < g->n; ++i) {
                 check_links.node(i, off);
@@ -297,7 +489,6 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         }
 
         DeviceGuard guard(ctx->device);
-        auto h = std::make_unique<vdx_gpu_hnsw>();
         h->ctx = ctx;
         h->metric = g->metric;
         h->dim = g->dim;
@@ -306,6 +497,8 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
         h->max_level = g->n ? g->max_level : 0;
         h->entry_point = g->n ? g->entry_point : 0;
         h->upper_blocks = g->upper_blocks;
+        h->levels.assign(g->levels, g->levels + g->n);
+        h->offsets.assign(offsets.begin(), offsets.begin() + g->n);
         h->split = vdx::gpu::split_dim(g->dim);
         hipStream_t s = ctx->stream;
         upload_rows(h->vectors, g->vectors, g->n, g->dim, h->split.ld, s);
@@ -332,126 +525,17 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
 }
 
 int vdx_gpu_hnsw_update(vdx_gpu_hnsw* h, const vdx_graph_view* g, const uint32_t* nodes, uint64_t count) {
-    return guarded([&] {
-        require(h != nullptr && g != nullptr, "vdx_gpu_hnsw_update: null argument");
-        require(count == 0 || nodes != nullptr, "vdx_gpu_hnsw_update: null node list");
-        require(g->metric == h->metric && g->dim == h->dim && g->M == h->M,
-                "vdx_gpu_hnsw_update: the graph's metric, dimension or M differs from the device copy's");
-        require(g->n >= h->n, "vdx_gpu_hnsw_update: the graph has fewer nodes than the device copy");
-        require(g->n < 0x80000000ull, "vdx_gpu_hnsw_update: more than 2^31 - 1 nodes");
-        require(g->n == 0 || (g->levels && g->layer0 && g->vectors), "vdx_gpu_hnsw_update: null section");
-        require(g->upper_blocks == 0 || g->upper != nullptr, "vdx_gpu_hnsw_update: null upper section");
-        require(g->n == 0 || g->entry_point < g->n, "vdx_gpu_hnsw_update: entry point out of range");
-        if (g->n == 0) return;
-        const std::uint64_t old_n = h->n;
-        // Upper offsets: old nodes keep theirs (blocks are appended in id order), new ones follow.
-        std::vector<std::uint32_t> offsets(g->n);
-        std::uint64_t acc = 0;
-        for (std::uint64_t i = 0; i < g->n; ++i) {
-            offsets[i] = static_cast<std::uint32_t>(acc);
-            require(g->levels[i] <= g->max_level, "vdx_gpu_hnsw_update: a level is above max_level");
-            acc += g->levels[i];
-        }
-        require(acc == g->upper_blocks, "vdx_gpu_hnsw_update: levels do not add up to upper_blocks");
-        require(acc <= 0xFFFFFFFFull, "vdx_gpu_hnsw_update: too many upper blocks");
-        require(old_n == 0 || offsets[old_n - 1] + g->levels[old_n - 1] == h->upper_blocks,
-                "vdx_gpu_hnsw_update: the old nodes' upper blocks moved");
-        require(g->levels[g->entry_point] == g->max_level,
-                "vdx_gpu_hnsw_update: the entry point is not on the top layer");
-        // The changed old nodes (ascending, below old_n); every new node counts as changed.
-        std::vector<std::uint32_t> changed;
-        for (std::uint64_t i = 0; i < count; ++i) {
-            require(nodes[i] < g->n, "vdx_gpu_hnsw_update: a listed node does not exist");
-            require(i == 0 || nodes[i] > nodes[i - 1], "vdx_gpu_hnsw_update: the node list is not ascending");
-            if (nodes[i] < old_n) changed.push_back(nodes[i]);
-        }
-        {
-            LinkChecker check_links(*g, "vdx_gpu_hnsw_update");
-            for (const std::uint32_t i : changed) check_links.node(i, offsets[i]);
-            for (std::uint64_t i = old_n; i < g->n; ++i) check_links.node(i, offsets[i]);
-        }
+    return guarded([&] { update_index(h, g, nodes, count); });
+}
 
-        DeviceGuard guard(h->ctx->device);
-        hipStream_t s = h->ctx->stream;
-        ++h->generation;  // from here on buffers may move, even if a step below fails
-        const std::uint64_t b0 = 1 + 2 * g->M, bu = 1 + g->M, ld = h->split.ld;
-        // Grows a buffer to hold `need` bytes with a quarter of headroom, keeping its contents.
-        const auto ensure = [&](DeviceBuffer& buf, std::uint64_t need) {
-            if (need <= buf.bytes()) return;
-            DeviceBuffer grown(need + need / 4);
-            if (buf.bytes() > 0) {
-                check(hipMemcpyAsync(grown.as<void>(), buf.as<void>(), buf.bytes(), hipMemcpyDeviceToDevice, s),
-                      "hipMemcpyAsync(grow)");
-            }
-            check(hipStreamSynchronize(s), "hipStreamSynchronize(grow)");
-            buf.swap(grown);
-        };
-        ensure(h->vectors, g->n * ld * 4);
-        ensure(h->layer0, g->n * b0 * 4);
-        ensure(h->upper, std::max<std::uint64_t>(1, g->upper_blocks) * bu * 4);
-        ensure(h->upper_offset, g->n * 4);
-        const std::uint64_t add = g->n - old_n;
-        if (add > 0) {
-            float* vdst = h->vectors.as<float>() + old_n * ld;
-            if (ld != g->dim) {
-                check(hipMemsetAsync(vdst, 0, add * ld * 4, s), "hipMemsetAsync");
-            }
-            check(hipMemcpy2DAsync(vdst, ld * 4, g->vectors + old_n * g->dim, g->dim * 4, g->dim * 4, add,
-                                   hipMemcpyHostToDevice, s),
-                  "hipMemcpy2DAsync(vectors)");
-            check(hipMemcpyAsync(h->layer0.as<std::uint32_t>() + old_n * b0, g->layer0 + old_n * b0, add * b0 * 4,
-                                 hipMemcpyHostToDevice, s),
-                  "hipMemcpyAsync(layer0)");
-            check(hipMemcpyAsync(h->upper_offset.as<std::uint32_t>() + old_n, offsets.data() + old_n, add * 4,
-                                 hipMemcpyHostToDevice, s),
-                  "hipMemcpyAsync(offsets)");
-            const std::uint64_t new_blocks = g->upper_blocks - h->upper_blocks;
-            if (new_blocks > 0) {
-                check(hipMemcpyAsync(h->upper.as<std::uint32_t>() + h->upper_blocks * bu,
-                                     g->upper + h->upper_blocks * bu, new_blocks * bu * 4, hipMemcpyHostToDevice, s),
-                      "hipMemcpyAsync(upper)");
-            }
-        }
-        if (!changed.empty()) {
-            // The changed old nodes' blocks, gathered on the host and scattered on the device.
-            std::vector<std::uint32_t> ublocks;
-            for (const std::uint32_t i : changed) {
-                for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(offsets[i] + l);
-            }
-            const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
-            vdx::gpu::PinnedBuffer staging;
-            staging.reserve(words * 4);
-            std::uint32_t* st = staging.as<std::uint32_t>();
-            std::memcpy(st, changed.data(), changed.size() * 4);
-            std::uint32_t* l0 = st + changed.size();
-            for (std::size_t r = 0; r < changed.size(); ++r) {
-                std::memcpy(l0 + r * b0, g->layer0 + static_cast<std::uint64_t>(changed[r]) * b0, b0 * 4);
-            }
-            std::uint32_t* uids = l0 + changed.size() * b0;
-            std::memcpy(uids, ublocks.data(), ublocks.size() * 4);
-            std::uint32_t* ub = uids + ublocks.size();
-            for (std::size_t r = 0; r < ublocks.size(); ++r) {
-                std::memcpy(ub + r * bu, g->upper + static_cast<std::uint64_t>(ublocks[r]) * bu, bu * 4);
-            }
-            DeviceBuffer dst(words * 4);
-            check(hipMemcpyAsync(dst.as<void>(), st, words * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(changes)");
-            const std::uint32_t* d = dst.as<std::uint32_t>();
-            check(vdx::dev::launch_scatter_rows(h->layer0.as<std::uint32_t>(), d + changed.size(), d, changed.size(),
-                                                static_cast<std::uint32_t>(b0), s),
-                  "scatter(layer0)");
-            check(vdx::dev::launch_scatter_rows(h->upper.as<std::uint32_t>(),
-                                                d + changed.size() * (1 + b0) + ublocks.size(),
-                                                d + changed.size() * (1 + b0), ublocks.size(),
-                                                static_cast<std::uint32_t>(bu), s),
-                  "scatter(upper)");
-            check(hipStreamSynchronize(s), "hipStreamSynchronize(update)");
-        } else {
-            check(hipStreamSynchronize(s), "hipStreamSynchronize(update)");
-        }
-        h->n = g->n;
-        h->upper_blocks = g->upper_blocks;
-        h->max_level = g->max_level;
-        h->entry_point = g->entry_point;
+int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const float* data, uint64_t rows,
+                           uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan, uint64_t synced_version) {
+    return guarded([&] {
+        require(index != nullptr, "vdx_gpu_hnsw_build_add: index is null");
+        BuildSearch b;
+        b.h = index;
+        const vdx_batch_searcher searcher{&b, &build_sync, &build_search};
+        vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version);
     });
 }
 

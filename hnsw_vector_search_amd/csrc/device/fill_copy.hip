@@ -46,6 +46,19 @@ __global__ __launch_bounds__(256) void scatter_rows_kernel(std::uint32_t* dst, c
     }
 }
 
+__global__ __launch_bounds__(256) void gather_rows_kernel(float* dst, std::uint32_t dst_ld, const float* src,
+                                                          std::uint32_t src_ld, const std::uint32_t* ids,
+                                                          std::uint64_t rows, std::uint32_t width) {
+    const std::uint64_t total = rows * width;
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += stride) {
+        const std::uint64_t r = i / width;
+        const std::uint64_t c = i - r * width;
+        dst[r * dst_ld + c] = src[static_cast<std::uint64_t>(ids[r]) * src_ld + c];
+    }
+}
+
 namespace {
 unsigned blocks_for(std::uint64_t items) {
     const std::uint64_t b = (items + 255) / 256;
@@ -64,6 +77,14 @@ hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, con
     if (rows == 0 || row_len == 0) return hipSuccess;
     hipLaunchKernelGGL(scatter_rows_kernel, dim3(blocks_for(rows * row_len)), dim3(256), 0, s, dst, src, ids,
                        rows, row_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(float* dst, std::uint32_t dst_ld, const float* src, std::uint32_t src_ld,
+                              const std::uint32_t* ids, std::uint64_t rows, std::uint32_t width, hipStream_t s) {
+    if (rows == 0 || width == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(rows * width)), dim3(256), 0, s, dst, dst_ld, src, src_ld,
+                       ids, rows, width);
     return hipGetLastError();
 }
 

@@ -12,6 +12,10 @@ hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_
 // Rows of row_len u32 from src scattered to dst rows ids[0..rows) (ids valid row indices of dst).
 hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, const std::uint32_t* ids,
                                std::uint64_t rows, std::uint32_t row_len, hipStream_t s);
+// dst row r (width floats, row stride dst_ld) = src row ids[r] (row stride src_ld); ids valid
+// row indices of src.
+hipError_t launch_gather_rows(float* dst, std::uint32_t dst_ld, const float* src, std::uint32_t src_ld,
+                              const std::uint32_t* ids, std::uint64_t rows, std::uint32_t width, hipStream_t s);
 // dst and src 16-byte aligned (device allocations are).
 hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
 

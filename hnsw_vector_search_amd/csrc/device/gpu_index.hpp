@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "gpu_state.hpp"
 
@@ -16,6 +17,15 @@ struct vdx_gpu_hnsw {
     // Counts vdx_gpu_hnsw_update calls: launches captured before one name buffers and sizes it
     // may have replaced.
     std::uint64_t generation = 0;
+    // Host copies of what updates check against: each node's level and first upper block, and
+    // the link checker's per-node marks (kept so an update costs what it changes, not n).
+    std::vector<std::uint8_t> levels;
+    std::vector<std::uint32_t> offsets;
+    std::vector<std::uint32_t> link_marks;
+    std::uint32_t link_epoch = 0;
+    // staging of an update's changed blocks (pinned) and their device copy, kept between updates
+    vdx::gpu::PinnedBuffer staging;
+    vdx::gpu::DeviceBuffer changes;
     vdx::gpu::DimSplit split{};
     vdx::gpu::DeviceBuffer vectors, layer0, upper, upper_offset;
     // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in

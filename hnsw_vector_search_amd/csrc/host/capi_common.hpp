@@ -40,6 +40,18 @@ int guarded(Fn&& fn) noexcept {
     }
 }
 
+// The inverse of guarded(): rethrows a status (with the message it recorded) as its exception.
+[[noreturn]] inline void throw_status(int status) {
+    const std::string msg = vdx_last_error();
+    switch (status) {
+        case VDX_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+        case VDX_LENGTH_ERROR: throw std::length_error(msg);
+        case VDX_DEVICE_ERROR: throw device_error(msg);
+        case VDX_OUT_OF_MEMORY: throw std::bad_alloc();
+        default: throw std::runtime_error(msg);
+    }
+}
+
 inline void require(bool ok, const char* what) {
     if (!ok) {
         throw std::invalid_argument(what);

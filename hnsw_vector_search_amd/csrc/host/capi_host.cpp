@@ -3,6 +3,7 @@
 #include <cstring>
 #include <memory>
 
+#include "batched_build.hpp"
 #include "capi_common.hpp"
 #include "cpu_distance.hpp"
 #include "graph.hpp"
@@ -23,6 +24,86 @@ struct vdx_host_graph {
 
 using vdx::guarded;
 using vdx::require;
+
+namespace vdx {
+namespace {
+
+// The graph's section view (FORMAT order), as vdx_host_graph_view fills it.
+void fill_view(const HnswGraph& h, vdx_graph_view* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->metric = static_cast<uint32_t>(h.metric());
+    out->dim = h.dim();
+    out->n = h.size();
+    out->M = h.params().M;
+    out->max_level = h.size() == 0 ? 0 : h.max_level();
+    out->entry_point = h.size() == 0 ? 0 : h.entry_point();
+    out->upper_blocks = h.upper_blocks();
+    out->levels = h.levels().data();
+    out->layer0 = h.layer0().data();
+    out->upper = h.upper().data();
+    out->vectors = h.vectors().data();
+}
+
+// A BatchSearcher over C callbacks.
+class HookSearcher final : public BatchSearcher {
+public:
+    HookSearcher(const vdx_batch_searcher& hooks, bool errors_set) : hooks_(hooks), errors_set_(errors_set) {}
+    void sync(const HnswGraph& g, const std::vector<Id>& changed) override {
+        vdx_graph_view view;
+        fill_view(g, &view);
+        check(hooks_.sync(hooks_.ctx, &view, changed.data(), changed.size()), "sync");
+    }
+    void search(const std::vector<Id>& nodes, std::size_t ef, std::vector<Neighbor>& out,
+                std::vector<std::uint32_t>& count) override {
+        out.resize(nodes.size() * ef);
+        count.assign(nodes.size(), 0u);
+        static_assert(sizeof(Neighbor) == sizeof(vdx_neighbor));
+        check(hooks_.search(hooks_.ctx, nodes.data(), nodes.size(), static_cast<std::uint32_t>(ef),
+                            reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
+              "search");
+        for (const std::uint32_t c : count) {
+            if (c > ef) {
+                throw std::runtime_error("HnswIndex::add: the batch searcher returned more than ef results");
+            }
+        }
+    }
+
+private:
+    void check(int rc, const char* which) const {
+        if (rc == VDX_OK) return;
+        if (errors_set_) throw_status(rc);
+        throw std::runtime_error(fmt("HnswIndex::add: the batch searcher's %s callback returned status %d", which, rc));
+    }
+
+    vdx_batch_searcher hooks_;
+    bool errors_set_;
+};
+
+}  // namespace
+
+std::uint64_t host_graph_add_batched(vdx_host_graph* g, const float* data, std::uint64_t rows, std::uint64_t dim,
+                                     std::uint64_t num_threads, const vdx_batch_searcher& searcher,
+                                     bool errors_set, const vdx_build_plan* plan, std::uint64_t synced_version) {
+    require(g != nullptr, "HnswIndex::add: graph is null");
+    require(rows == 0 || data != nullptr, "HnswIndex::add: data is null");
+    require(searcher.sync != nullptr && searcher.search != nullptr, "HnswIndex::add: searcher callback is null");
+    if (dim != g->graph->dim()) {
+        throw std::invalid_argument(fmt("HnswIndex::add: got dimension %llu, index has %zu",
+                                        static_cast<unsigned long long>(dim), g->graph->dim()));
+    }
+    HookSearcher hooks(searcher, errors_set);
+    BatchPlan p;
+    if (plan != nullptr) {
+        p.host_prefix = plan->host_prefix;
+        p.batch_min = plan->batch_min;
+        p.batch_max = plan->batch_max;
+        p.batch_fraction = plan->batch_fraction;
+    }
+    g->graph->add_batched(data, rows, num_threads, hooks, p, synced_version);
+    return g->graph->version();
+}
+
+}  // namespace vdx
 
 extern "C" {
 
@@ -72,22 +153,19 @@ int vdx_host_graph_load(const char* path, vdx_host_graph** out) {
     });
 }
 
+int vdx_host_graph_add_batched(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t dim,
+                               uint64_t num_threads, const vdx_batch_searcher* searcher, const vdx_build_plan* plan,
+                               uint64_t synced_version) {
+    return guarded([&] {
+        require(searcher != nullptr, "vdx_host_graph_add_batched: searcher is null");
+        vdx::host_graph_add_batched(g, data, rows, dim, num_threads, *searcher, false, plan, synced_version);
+    });
+}
+
 int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out) {
     return guarded([&] {
         require(g != nullptr && out != nullptr, "vdx_host_graph_view: null argument");
-        const vdx::HnswGraph& h = *g->graph;
-        std::memset(out, 0, sizeof(*out));
-        out->metric = static_cast<uint32_t>(h.metric());
-        out->dim = h.dim();
-        out->n = h.size();
-        out->M = h.params().M;
-        out->max_level = h.size() == 0 ? 0 : h.max_level();
-        out->entry_point = h.size() == 0 ? 0 : h.entry_point();
-        out->upper_blocks = h.upper_blocks();
-        out->levels = h.levels().data();
-        out->layer0 = h.layer0().data();
-        out->upper = h.upper().data();
-        out->vectors = h.vectors().data();
+        vdx::fill_view(*g->graph, out);
     });
 }
 

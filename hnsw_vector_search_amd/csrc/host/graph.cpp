@@ -211,6 +211,96 @@ void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads
     build_locks_.reset();
 }
 
+void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num_threads,
+                            BatchSearcher& searcher, const BatchPlan& plan, std::uint64_t synced_version) {
+    check_new_vectors("HnswIndex::add", IndexShape{metric_, dim_, size()}, dim_, data, rows);
+    if (plan.batch_min == 0 || plan.batch_max < plan.batch_min || !(plan.batch_fraction >= 0.0)) {
+        throw std::invalid_argument("HnswIndex::add: batch sizes must satisfy 1 <= batch_min <= batch_max "
+                                    "and batch_fraction >= 0");
+    }
+    ++version_;
+    const std::size_t first = size();
+    const std::size_t total = first + rows;
+    vectors_.reserve(total * dim_);
+    levels_.reserve(total);
+    upper_offset_.reserve(total);
+    touched_.reserve(total);
+    layer0_.reserve(total * (1 + 2 * params_.M));
+    // As in add(): every row is stored (levels drawn in serial order) before any is linked.
+    for (std::size_t r = 0; r < rows; ++r) {
+        store(data + r * dim_);
+    }
+    if (rows == 0) {
+        return;
+    }
+    const std::size_t threads = std::max<std::size_t>(1, std::min(resolve_threads(num_threads), rows));
+    build_locks_ = std::make_unique<BuildLocks>(total);
+    struct Release {
+        HnswGraph* g;
+        ~Release() { g->build_locks_.reset(); }
+    } release{this};
+    std::size_t next = first;
+    if (next == 0) {
+        insert<Locking::None>(0);
+        next = 1;
+    }
+    const std::size_t prefix_end = std::min(total, std::max(next, plan.host_prefix));
+    if (prefix_end > next) {
+        parallel_for(prefix_end - next, threads,
+                     [this, next](std::size_t i) { insert<Locking::PerNode>(static_cast<Id>(next + i)); });
+        next = prefix_end;
+    }
+    // Stamps written after a sync carry a later version than the one synced.
+    std::uint64_t synced = synced_version;
+    const auto sync = [&] {
+        searcher.sync(*this, changed_since(synced));
+        synced = version_;
+        ++version_;
+    };
+    const std::size_t ef = std::max(params_.ef_construction, params_.M);
+    std::vector<Id> high, low;
+    std::vector<Neighbor> found;
+    std::vector<std::uint32_t> count;
+    while (next < total) {
+        const auto scaled = static_cast<std::size_t>(plan.batch_fraction * static_cast<double>(next));
+        const std::size_t batch = std::clamp(scaled, plan.batch_min, plan.batch_max);
+        const std::size_t end = std::min(total, next + batch);
+        high.clear();
+        low.clear();
+        for (std::size_t id = next; id < end; ++id) {
+            (levels_[id] > 0 ? high : low).push_back(static_cast<Id>(id));
+        }
+        if (!high.empty()) {
+            parallel_for(high.size(), std::min(threads, high.size()),
+                         [this, &high](std::size_t i) { insert<Locking::PerNode>(high[i]); });
+        }
+        sync();
+        if (!low.empty()) {
+            searcher.search(low, ef, found, count);
+            parallel_for(low.size(), std::min(threads, low.size()), [this, &low, &found, &count, ef](std::size_t i) {
+                link_searched(low[i], found.data() + i * ef, count[i]);
+            });
+        }
+        next = end;
+    }
+    sync();
+}
+
+// The linking half of insert for a level-0 node whose layer-0 search results are `found`.
+void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count) {
+    std::vector<Neighbor> chosen = select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
+    {
+        const std::lock_guard<std::mutex> lock(build_locks_->links[node]);
+        touched_[node] = static_cast<std::uint32_t>(version_);
+        Id* b = block(node, 0);
+        b[0] = static_cast<Id>(chosen.size());
+        for (std::size_t i = 0; i < chosen.size(); ++i) {
+            b[1 + i] = chosen[i].id;
+        }
+    }
+    link_back<Locking::PerNode>(node, 0, chosen);
+}
+
 template <HnswGraph::Locking L>
 const Id* HnswGraph::read_links(Id node, std::size_t layer, std::vector<Id>& copy,
                                 std::size_t& count) const {
@@ -328,6 +418,7 @@ void HnswGraph::insert(Id node) {
         if constexpr (L == Locking::PerNode) {
             lock = std::unique_lock<std::mutex>(build_locks_->links[node]);
         }
+        touched_[node] = static_cast<std::uint32_t>(version_);
         for (std::size_t layer = 0; layer <= top; ++layer) {
             Id* b = block(node, layer);
             b[0] = static_cast<Id>(chosen[layer].size());

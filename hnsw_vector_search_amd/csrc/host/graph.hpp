@@ -41,6 +41,33 @@ struct GraphReport {
     double mean_degree0 = 0.0;
 };
 
+class HnswGraph;
+
+// Runs the layer-0 insertion searches of a batch of nodes on a copy of the graph held
+// elsewhere (the GPU): for each node, the greedy descent from the entry point through the upper
+// layers and search_layer on layer 0 with breadth ef (src/hnsw.cpp:315-335), from the node's
+// own stored vector, over the graph as of the last sync.
+class BatchSearcher {
+public:
+    virtual ~BatchSearcher() = default;
+    // Makes the copy current: `changed` lists, ascending, the nodes stored or whose link
+    // blocks were written since the previous sync (all of them for the first).
+    virtual void sync(const HnswGraph& g, const std::vector<Id>& changed) = 0;
+    // out[i * ef ..] receives the count[i] results of nodes[i], in any order.
+    virtual void search(const std::vector<Id>& nodes, std::size_t ef, std::vector<Neighbor>& out,
+                        std::vector<std::uint32_t>& count) = 0;
+};
+
+// How add_batched splits the rows: the first `host_prefix` nodes of the graph are inserted on
+// the host; after that, batches of batch_fraction x (nodes linked so far), clamped to
+// [batch_min, batch_max].
+struct BatchPlan {
+    std::size_t host_prefix = 16384;
+    std::size_t batch_min = 256;
+    std::size_t batch_max = 65536;
+    double batch_fraction = 0.02;
+};
+
 class HnswGraph {
 public:
     HnswGraph(std::size_t dim, Metric metric, GraphParams params);
@@ -72,6 +99,16 @@ public:
     void save(const std::string& path) const;
     static std::unique_ptr<HnswGraph> load(const std::string& path);
 
+    // add() with the layer-0 searches of level-0 nodes done in batches by `searcher`: per batch,
+    // the batch's nodes with upper layers are inserted on the host (all layers), the searcher's
+    // copy is synced, the level-0 nodes' candidates come from one batched search over it, and
+    // each is linked as insert links it (neighbour selection, link-back with pruning). Nodes of
+    // one batch do not see each other as candidates. `synced_version` is the version the
+    // searcher's copy was last synced at; the copy is current when this returns. A batch of one
+    // node with one thread builds exactly what add() with one thread builds.
+    void add_batched(const float* data, std::size_t rows, std::size_t num_threads, BatchSearcher& searcher,
+                     const BatchPlan& plan, std::uint64_t synced_version);
+
     // Bumped by every add(); a device copy remembers the version it was made from.
     std::uint64_t version() const noexcept { return version_; }
     // Nodes whose link blocks (any layer) were written by an add() after `version`: the nodes
@@ -99,6 +136,7 @@ private:
     void insert(Id node);
     template <Locking L>
     void link_back(Id node, std::size_t layer, const std::vector<Neighbor>& chosen);
+    void link_searched(Id node, const Neighbor* found, std::size_t count);
     std::vector<Neighbor> select_neighbors(std::vector<Neighbor> candidates, std::size_t m) const;
     template <Locking L>
     const Id* read_links(Id node, std::size_t layer, std::vector<Id>& copy,

@@ -208,10 +208,22 @@ class HnswIndex:
         return self.size()
 
     # ---- build (host) -----------------------------------------------------------------
-    def add(self, vectors, num_threads: int = 1) -> None:
+    def add(self, vectors, num_threads: int = 1, *, gpu_assisted: bool = False,
+            plan: nat.BuildPlan | None = None) -> None:
         """Stores and links every row (HnswIndex::add, src/hnsw.cpp:259-291). A 1-D input is
-        one vector."""
+        one vector. gpu_assisted: the layer-0 insertion searches of level-0 nodes run on the GPU
+        in batches (vdx_gpu_hnsw_build_add, batch sizes from `plan`); nodes of one batch are not
+        candidates of each other, so the graph differs from the host build's (a batch of one
+        node on one thread builds the same graph)."""
         arr = _as_matrix(vectors)
+        if gpu_assisted:
+            handle = self.gpu_handle()
+            with self._gpu_lock:
+                nat.check(nat.lib.vdx_gpu_hnsw_build_add(
+                    handle, self._graph.handle, nat.ptr(arr), arr.shape[0], arr.shape[1], num_threads,
+                    ctypes.byref(plan) if plan is not None else None, self._gpu_version))
+                self._gpu_version = self._graph.params()[2]
+            return
         nat.check(nat.lib.vdx_host_graph_add(self._graph.handle, nat.ptr(arr), arr.shape[0],
                                              arr.shape[1], num_threads))
         self._refresh_gpu()

@@ -85,6 +85,34 @@ VDX_API int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out);
 VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
                           uint64_t* version);
 VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
+/* HnswIndex::add (src/hnsw.cpp:247-291) with the layer-0 insertion searches of level-0 nodes
+ * run in batches by a searcher over its own copy of the graph (no counterpart in vecdex, which
+ * searches node by node). Per batch: the batch's nodes with upper layers are inserted on the
+ * host, sync() brings the searcher's copy up to the graph (`changed`: ascending ids of the nodes
+ * stored or rewritten since the last sync), search() returns each level-0 node's search_layer
+ * results at breadth ef = max(ef_construction, M) after the greedy descent (src/hnsw.cpp:315-335),
+ * any order, and each node is linked as insert links it. Nodes of one batch are not candidates
+ * of each other. Callbacks run on the calling thread and return a vdx status. */
+typedef struct vdx_batch_searcher {
+    void* ctx;
+    int (*sync)(void* ctx, const vdx_graph_view* graph, const uint32_t* changed, uint64_t count);
+    int (*search)(void* ctx, const uint32_t* nodes, uint64_t count, uint32_t ef, vdx_neighbor* out,
+                  uint32_t* counts);
+} vdx_batch_searcher;
+/* Batch sizes: the first host_prefix nodes of the graph are inserted on the host; then batches
+ * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 256,
+ * 65536, 0.02. batch_min = batch_max = 1 with host_prefix 0 and one thread builds exactly
+ * what vdx_host_graph_add with one thread builds (given exact searches). */
+typedef struct vdx_build_plan {
+    uint64_t host_prefix, batch_min, batch_max;
+    double batch_fraction;
+} vdx_build_plan;
+/* `synced_version`: the graph version the searcher's copy reflects (vdx_host_graph_params);
+ * the first sync lists everything changed since. */
+VDX_API int vdx_host_graph_add_batched(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t dim,
+                                       uint64_t num_threads, const vdx_batch_searcher* searcher,
+                                       const vdx_build_plan* plan, uint64_t synced_version);
+
 /* Nodes whose link blocks an add() after `since_version` wrote (the nodes it stored and those it
  * linked back to), ascending; with nodes NULL only the count. No counterpart in vecdex: it
  * drives the incremental device update (vdx_gpu_hnsw_update). */
@@ -175,6 +203,12 @@ VDX_API int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* graph, v
  * upload's link rules. Device buffers grow with headroom. Not concurrent with searches on it. */
 VDX_API int vdx_gpu_hnsw_update(vdx_gpu_hnsw* index, const vdx_graph_view* graph, const uint32_t* nodes,
                                 uint64_t count);
+/* vdx_host_graph_add_batched with the GPU as the searcher: `index` is the device copy of `graph`
+ * as of graph version `synced_version`; it is kept current (vdx_gpu_hnsw_update per batch) and
+ * each batch's searches are one beam launch over the nodes' vectors already in HBM. */
+VDX_API int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const float* data, uint64_t rows,
+                                   uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan,
+                                   uint64_t synced_version);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 
 /* Knobs of one search launch. Zero-initialise for defaults (autotuned). */

@@ -1,0 +1,90 @@
+"""The batched add (vdx_host_graph_add_batched): level-0 insertion searches done in batches by a
+searcher over a synced copy of the graph. Here the searcher is the oracle's restatement of
+vecdex's search (the CPU stand-in for the GPU searcher of vdx_gpu_hnsw_build_add), which pins
+the batching logic on the CPU: a batch of one node on one thread must build vecdex's serial graph
+(golden fingerprints of tests/hnsw_parallel_test.cpp:146-169), larger batches a valid graph whose
+recall matches the host build's."""
+
+import numpy as np
+import pytest
+
+import hnsw_vector_search_amd as vdx
+from hnsw_vector_search_amd import _native as nat
+from hnsw_vector_search_amd import datasets
+from oracle import oracle as o
+from batch_searchers import OracleSearcher, add_batched, sections
+
+GREEDY_FP = 6133164406513967666   # tests/hnsw_parallel_test.cpp:164
+TOP10_FP = 15666754926265556722   # tests/hnsw_parallel_test.cpp:165
+
+
+def fingerprint(graph, queries, k, ef):
+    out, cnt = o.hnsw_search(graph, queries, k, ef)
+    return o.fingerprint([(out[i]["id"][:cnt[i]], out[i]["distance"][:cnt[i]]) for i in range(len(cnt))])
+
+
+def test_batch_of_one_builds_the_serial_graph():
+    base = datasets.integer_matrix(1000, 16, 1)
+    queries = datasets.integer_matrix(200, 16, 2)
+    serial = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=64, seed=42)
+    serial.add(base)
+    batched = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=64, seed=42)
+    s = OracleSearcher()
+    assert add_batched(batched, base, 1, nat.BuildPlan(0, 1, 1, 0.0), s) == 0
+    a, b = sections(serial), sections(batched)
+    for key in ("levels", "layer0", "upper", "vectors", "max_level", "entry_point", "upper_blocks"):
+        assert np.array_equal(a[key], b[key]), key
+    assert fingerprint(b, queries, 1, 1) == GREEDY_FP
+    assert fingerprint(b, queries, 10, 10) == TOP10_FP
+    # every level-0 node after the first was searched once, in id order
+    searched = np.concatenate(s.batches)
+    assert np.array_equal(searched, np.nonzero(a["levels"] == 0)[0][np.nonzero(a["levels"] == 0)[0] > 0])
+    # the first sync lists every stored node; later ones what the previous batch wrote
+    assert np.array_equal(s.changed[0], np.arange(1000, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
+def test_batches_build_a_valid_graph_with_the_host_build_recall(metric):
+    mix = datasets.c0_mixture()
+    base, queries = mix.sample(12_000, 1) + 1, mix.sample(200, 2) + 1
+    host = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
+    host.add(base, num_threads=4)
+    batched = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
+    s = OracleSearcher()
+    assert add_batched(batched, base, 4, nat.BuildPlan(1000, 128, 2048, 0.05), s) == 0
+    # vecdex's pruning leaves a few nodes without in-links in the host build too
+    report = batched.inspect()
+    assert report.violations == [] and report.reachable >= host.inspect().reachable - 12
+    sizes = [len(b) for b in s.batches]
+    assert max(sizes) <= 2048 and len(sizes) > 10
+    prepared = o.normalize(queries) if metric == vdx.Metric.Cosine else queries
+    truth, _ = o.bruteforce_f64(sections(host)["vectors"], prepared, 10, metric=metric)
+    recalls = []
+    for index in (host, batched):
+        out, _ = o.hnsw_search(sections(index), prepared, 10, 32)
+        recalls.append(o.recall_at_k(out["id"], truth, 10))
+    assert recalls[1] >= recalls[0] - 0.01, recalls
+    # the last sync left the searcher's copy equal to the graph
+    assert np.array_equal(s.graph["layer0"], sections(batched)["layer0"])
+
+
+def test_batched_add_extends_a_built_graph_and_reports_errors():
+    base = datasets.integer_matrix(3000, 16, 5)
+    index = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=40, seed=1)
+    index.add(base[:2000], num_threads=2)
+    v0 = index._graph.params()[2]
+    s = OracleSearcher()
+    assert add_batched(index, base[2000:], 2, nat.BuildPlan(0, 100, 400, 0.1), s) == 0
+    assert index.size() == 3000 and index.inspect().violations == []
+    # the first sync lists the 1000 stored rows and the old nodes the first batch's upper-layer
+    # nodes linked back into
+    first = set(s.changed[0].tolist())
+    assert set(range(2000, 3000)) <= first and len(first) < 3000
+    assert index._graph.params()[2] > v0
+    bad = OracleSearcher(fail_search=7)
+    assert add_batched(index, base[:10], 1, nat.BuildPlan(0, 4, 4, 0.0), bad) == nat.VDX_RUNTIME_ERROR
+    assert "search callback returned status 7" in nat.lib.vdx_last_error().decode()
+    status = add_batched(index, base[:10], 1, nat.BuildPlan(0, 0, 4, 0.0), OracleSearcher())
+    assert status == nat.VDX_INVALID_ARGUMENT and "batch_min" in nat.lib.vdx_last_error().decode()
+    wrong = np.zeros((3, 8), np.float32)
+    assert add_batched(index, wrong, 1, nat.BuildPlan(), OracleSearcher()) == nat.VDX_INVALID_ARGUMENT
