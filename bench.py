@@ -182,6 +182,9 @@ def main(argv=None) -> int:
     ap.add_argument("--base10m", action="store_true",
                     help="also run the 10M x 128 M=32 base (host graph build takes minutes)")
     ap.add_argument("--n10m", type=int, default=10_000_000)
+    ap.add_argument("--no-gpu-build", action="store_true", help="skip the GPU-assisted build leg")
+    ap.add_argument("--host-build-10m", action="store_true",
+                    help="build the 10M graph on the host alone (default: GPU-assisted)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -249,6 +252,8 @@ def main(argv=None) -> int:
                "ms": round(ms, 4), "qps": round(args.nq / ms * 1000, 1)}
         sweep.append(row)
         log(f"fixed ef {ef}: recall {row['recall']:.4f}, {ms:.3f} ms/pass, {row['qps']:,.0f} QPS")
+    gpu_build_leg = None if args.no_gpu_build else run_gpu_build(vdx, nat, base, queries, truth, threads,
+                                                                  build_s, sweep)
     config_points = [s for s in sweep if s["ef"] in EF_SWEEP and s["recall"] >= TARGET_RECALL]
     fixed = min(config_points, key=lambda s: s["ef"]) if config_points else sweep[-1]
     fine_points = [s for s in sweep if s["recall"] >= TARGET_RECALL]
@@ -352,6 +357,7 @@ def main(argv=None) -> int:
         "sweep": sweep,
         "bruteforce": bf_leg,
         "serving": serving_leg,
+        "gpu_build": gpu_build_leg,
         "base_10m": base10m_leg,
         "host_entry_point": {"ms": round(host_ms, 4), "qps": round(nq / host_ms * 1000, 1),
                              "note": "queries from and results to host memory (PCIe inclusive)"},
@@ -460,11 +466,12 @@ def run_10m(args, vdx, nat, datasets, threads):
         index = vdx.HnswIndex(128, vdx.Metric.L2, M=32, ef_construction=200, seed=0)
         t = time.perf_counter()
         step = 1_000_000
+        how = "host" if args.host_build_10m else "GPU-assisted"
         for r0 in range(0, len(base), step):  # in slices, reporting progress
-            index.add(base[r0:r0 + step], num_threads=threads)
-            log(f"10M: {min(r0 + step, len(base)):,} rows linked, {time.perf_counter() - t:.0f} s")
+            index.add(base[r0:r0 + step], num_threads=threads, gpu_assisted=not args.host_build_10m)
+            log(f"10M: {min(r0 + step, len(base)):,} rows linked ({how}), {time.perf_counter() - t:.0f} s")
         build_s = time.perf_counter() - t
-        log(f"10M: built graph (M=32, efC=200) on {threads} threads in {build_s:.0f} s")
+        log(f"10M: built graph (M=32, efC=200, {how}) on {threads} threads in {build_s:.0f} s")
         if "VDX_BENCH_CACHE" in os.environ:  # an 8 GB file: kept only where asked for
             index.save(path)
     t = time.perf_counter()
@@ -514,10 +521,39 @@ def run_10m(args, vdx, nat, datasets, threads):
         f"({nq / ms_graph * 1000:,.0f} QPS), identical {identical}")
     del batch
     return {"workload": f"{len(base):,} x 128 L2 Gaussian mixture (4,096 components), M=32, efC=200",
-            "graph_build_s": None if build_s is None else round(build_s, 1), "fixed_ef_sweep": sweep,
+            "graph_build_s": None if build_s is None else round(build_s, 1),
+            "graph_build": "host" if args.host_build_10m else "GPU-assisted", "fixed_ef_sweep": sweep,
             "qps_at_recall_0.95": at[0] if at else None,
             "serving": {"queries": nq, "one_launch_ms": round(ms_one, 2), "tuned_graph_ms": round(ms_graph, 2),
                         "tuned_graph_qps": round(nq / ms_graph * 1000, 1), "identical_to_one_launch": identical}}
+
+
+def run_gpu_build(vdx, nat, base, queries, truth, threads, host_build_s, sweep):
+    """The same 1M graph (M=16, efC=200, seed 0) built GPU-assisted: level-0 insertion searches
+    in batches on the GPU, selection and linking on the host threads; build time, its split, and
+    recall/QPS of the resulting graph against the host-built one's at the same ef."""
+    t = time.perf_counter()
+    g = vdx.HnswIndex(base.shape[1], vdx.Metric.L2, M=16, ef_construction=200, seed=0)
+    g.add(base, num_threads=threads, gpu_assisted=True)
+    secs = time.perf_counter() - t
+    st = g.last_build_stats
+    log(f"GPU-assisted build: {secs:.1f} s on {threads} threads (host build {host_build_s} s); "
+        + ", ".join(f"{k} {v:.2f}" if isinstance(v, float) else f"{k} {v}" for k, v in st.items()))
+    run = Runner(nat, g, queries)
+    rows = []
+    host_rows = {r["ef"]: r for r in sweep}
+    for ef in (36, 64):
+        run.launch(ef)
+        r = recall(run.results(), truth)
+        ms, _ = run.timed(ef, 10, 2)
+        host = host_rows.get(ef)
+        rows.append({"ef": ef, "recall": round(r, 4), "qps": round(len(queries) / ms * 1000, 1),
+                     "host_graph_recall": host["recall"] if host else None})
+        log(f"GPU-built graph, ef {ef}: recall {r:.4f} (host-built {rows[-1]['host_graph_recall']}), "
+            f"{rows[-1]['qps']:,.0f} QPS")
+    return {"build_s": round(secs, 2), "host_build_s": None if host_build_s is None else round(host_build_s, 2),
+            "threads": threads, "stats": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
+            "search": rows}
 
 
 def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, mix, threads,

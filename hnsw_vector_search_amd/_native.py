@@ -56,6 +56,12 @@ class BuildPlan(ctypes.Structure):
                 ("batch_fraction", c_double)]
 
 
+class BuildStats(ctypes.Structure):
+    _fields_ = [("batches", c_u64), ("searched", c_u64), ("host_inserted", c_u64),
+                ("host_insert_s", c_double), ("sync_s", c_double), ("search_s", c_double),
+                ("link_s", c_double), ("total_s", c_double)]
+
+
 class GraphReport(ctypes.Structure):
     _fields_ = [("reachable", c_u64), ("max_level", c_u64), ("violation_count", c_u64),
                 ("mean_degree0", c_double)]
@@ -90,7 +96,7 @@ _SIGS = {
     "vdx_host_graph_params": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
     "vdx_host_graph_changes": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_ptr]),
     "vdx_host_graph_add_batched": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BatchSearcher),
-                                           ctypes.POINTER(BuildPlan), c_u64]),
+                                           ctypes.POINTER(BuildPlan), c_u64, ctypes.POINTER(BuildStats)]),
     "vdx_host_graph_memory_bytes": (c_u64, [c_ptr]),
     "vdx_host_graph_inspect": (c_int, [c_ptr, ctypes.POINTER(GraphReport), c_char_p, c_u64]),
     "vdx_host_graph_free": (None, [c_ptr]),
@@ -120,7 +126,7 @@ _SIGS = {
     "vdx_gpu_hnsw_upload": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr]),
     "vdx_gpu_hnsw_update": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr, c_u64]),
     "vdx_gpu_hnsw_build_add": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BuildPlan),
-                                       c_u64]),
+                                       c_u64, ctypes.POINTER(BuildStats)]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),
     "vdx_gpu_hnsw_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_ptr, c_u32, c_ptr, c_u32,
                                     c_ptr, c_u64, c_ptr, c_char_p, ctypes.POINTER(SearchOpts)]),

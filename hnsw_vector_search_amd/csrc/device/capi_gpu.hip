@@ -40,23 +40,40 @@ constexpr std::uint32_t kRingSlots = 64;
 
 int bf_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 2); }
 
-// Copies `rows` rows of `dim` floats into a device buffer of rows of `ld` floats (zero padded).
-void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::uint64_t dim,
-                 std::uint64_t ld, hipStream_t stream) {
-    dst.resize(std::max<std::uint64_t>(1, rows) * ld * sizeof(float));
+// Copies `rows` host rows of `dim` floats to device rows of `ld` floats (zero padded). Rows
+// that need padding are packed in pinned chunks: a strided copy from pageable memory would go
+// row by row.
+void copy_rows(float* dst, const float* src, std::uint64_t rows, std::uint64_t dim, std::uint64_t ld,
+               hipStream_t stream) {
     if (rows == 0) {
         return;
     }
     if (ld == dim) {
-        check(hipMemcpyAsync(dst.as<float>(), src, rows * dim * sizeof(float), hipMemcpyHostToDevice,
-                             stream),
+        check(hipMemcpyAsync(dst, src, rows * dim * sizeof(float), hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync(vectors)");
-    } else {
-        check(hipMemsetAsync(dst.as<float>(), 0, rows * ld * sizeof(float), stream), "hipMemsetAsync");
-        check(hipMemcpy2DAsync(dst.as<float>(), ld * sizeof(float), src, dim * sizeof(float),
-                               dim * sizeof(float), rows, hipMemcpyHostToDevice, stream),
-              "hipMemcpy2DAsync(vectors)");
+        return;
     }
+    const std::uint64_t chunk = std::max<std::uint64_t>(1, (std::uint64_t{64} << 20) / (ld * sizeof(float)));
+    vdx::gpu::PinnedBuffer staging;
+    staging.reserve(std::min(rows, chunk) * ld * sizeof(float));
+    float* st = staging.as<float>();
+    for (std::uint64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const std::uint64_t m = std::min(chunk, rows - r0);
+        for (std::uint64_t r = 0; r < m; ++r) {
+            std::memcpy(st + r * ld, src + (r0 + r) * dim, dim * sizeof(float));
+            std::memset(st + r * ld + dim, 0, (ld - dim) * sizeof(float));
+        }
+        check(hipMemcpyAsync(dst + r0 * ld, st, m * ld * sizeof(float), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(vectors)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize(vectors)");  // before the chunk is reused
+    }
+}
+
+// Copies `rows` rows of `dim` floats into a device buffer of rows of `ld` floats (zero padded).
+void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::uint64_t dim,
+                 std::uint64_t ld, hipStream_t stream) {
+    dst.resize(std::max<std::uint64_t>(1, rows) * ld * sizeof(float));
+    copy_rows(dst.as<float>(), src, rows, dim, ld, stream);
 }
 
 // Queues one beam launch over prepared device queries.
@@ -142,7 +159,10 @@ private:
 // their levels (so their upper blocks keep their positions); new nodes are appended and the
 // listed old nodes' blocks rewritten, all under the upload's link rules. Host work is
 // proportional to what changed (plus one comparison of the old levels).
-void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t* nodes, std::uint64_t count) {
+// `trusted`: the graph comes from this library's own build (whose link invariants hold by
+// construction and are tested), so the per-link checks are skipped.
+void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t* nodes, std::uint64_t count,
+                  bool trusted = false) {
     require(h != nullptr && g != nullptr, "vdx_gpu_hnsw_update: null argument");
     require(count == 0 || nodes != nullptr, "vdx_gpu_hnsw_update: null node list");
     require(g->metric == h->metric && g->dim == h->dim && g->M == h->M,
@@ -175,7 +195,7 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
         require(i == 0 || nodes[i] > nodes[i - 1], "vdx_gpu_hnsw_update: the node list is not ascending");
         if (nodes[i] < old_n) changed.push_back(nodes[i]);
     }
-    {
+    if (!trusted) {
         LinkChecker check_links(*g, "vdx_gpu_hnsw_update", h->link_marks, h->link_epoch);
         for (const std::uint32_t i : changed) check_links.node(i, h->offsets[i]);
         for (std::uint64_t i = old_n; i < g->n; ++i) check_links.node(i, new_offsets[i - old_n]);
@@ -202,13 +222,7 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
     ensure(h->upper_offset, g->n * 4);
     const std::uint64_t add = g->n - old_n;
     if (add > 0) {
-        float* vdst = h->vectors.as<float>() + old_n * ld;
-        if (ld != g->dim) {
-            check(hipMemsetAsync(vdst, 0, add * ld * 4, s), "hipMemsetAsync");
-        }
-        check(hipMemcpy2DAsync(vdst, ld * 4, g->vectors + old_n * g->dim, g->dim * 4, g->dim * 4, add,
-                               hipMemcpyHostToDevice, s),
-              "hipMemcpy2DAsync(vectors)");
+        copy_rows(h->vectors.as<float>() + old_n * ld, g->vectors + old_n * g->dim, add, g->dim, ld, s);
         check(hipMemcpyAsync(h->layer0.as<std::uint32_t>() + old_n * b0, g->layer0 + old_n * b0, add * b0 * 4,
                              hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(layer0)");
@@ -222,14 +236,25 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
                   "hipMemcpyAsync(upper)");
         }
     }
-    if (!changed.empty()) {
-        // The changed old nodes' blocks, gathered on the host and scattered on the device.
-        std::vector<std::uint32_t> ublocks;
-        for (const std::uint32_t i : changed) {
-            for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(h->offsets[i] + l);
-        }
-        const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
-        h->staging.reserve(words * 6);  // with headroom: updates of a batched build grow
+    // When a quarter or more of the old nodes changed, their layer-0 section goes over as one
+    // contiguous copy (cheaper than gathering scattered blocks); upper blocks are always scattered.
+    const bool dense = !changed.empty() && changed.size() * 4 >= old_n;
+    if (dense) {
+        check(hipMemcpyAsync(h->layer0.as<void>(), g->layer0, old_n * b0 * 4, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(layer0)");
+    }
+    std::vector<std::uint32_t> ublocks;
+    for (const std::uint32_t i : changed) {
+        for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(h->offsets[i] + l);
+    }
+    if (dense) {
+        changed.clear();
+    }
+    const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
+    if (words > 0) {
+        // The changed blocks with their ids, gathered on the host into pinned staging (headroom:
+        // the updates of a batched build grow), one copy over, scattered on the device.
+        h->staging.reserve(words * 6);
         std::uint32_t* st = h->staging.as<std::uint32_t>();
         std::memcpy(st, changed.data(), changed.size() * 4);
         std::uint32_t* l0 = st + changed.size();
@@ -272,7 +297,7 @@ struct BuildSearch {
 };
 
 int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* changed, std::uint64_t count) {
-    return guarded([&] { update_index(static_cast<BuildSearch*>(ctx)->h, view, changed, count); });
+    return guarded([&] { update_index(static_cast<BuildSearch*>(ctx)->h, view, changed, count, true); });
 }
 
 int build_search(void* ctx, const std::uint32_t* nodes, std::uint64_t count, std::uint32_t ef, vdx_neighbor* out,
@@ -529,13 +554,14 @@ int vdx_gpu_hnsw_update(vdx_gpu_hnsw* h, const vdx_graph_view* g, const uint32_t
 }
 
 int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const float* data, uint64_t rows,
-                           uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan, uint64_t synced_version) {
+                           uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan, uint64_t synced_version,
+                           vdx_build_stats* stats) {
     return guarded([&] {
         require(index != nullptr, "vdx_gpu_hnsw_build_add: index is null");
         BuildSearch b;
         b.h = index;
         const vdx_batch_searcher searcher{&b, &build_sync, &build_search};
-        vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version);
+        vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version, stats);
     });
 }
 

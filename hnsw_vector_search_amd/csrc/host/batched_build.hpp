@@ -15,6 +15,7 @@ namespace vdx {
 // its status. Returns the graph's version afterwards (the searcher's copy is current).
 std::uint64_t host_graph_add_batched(vdx_host_graph* g, const float* data, std::uint64_t rows, std::uint64_t dim,
                                      std::uint64_t num_threads, const vdx_batch_searcher& searcher,
-                                     bool errors_set, const vdx_build_plan* plan, std::uint64_t synced_version);
+                                     bool errors_set, const vdx_build_plan* plan, std::uint64_t synced_version,
+                                     vdx_build_stats* stats);
 
 }  // namespace vdx

@@ -83,7 +83,8 @@ private:
 
 std::uint64_t host_graph_add_batched(vdx_host_graph* g, const float* data, std::uint64_t rows, std::uint64_t dim,
                                      std::uint64_t num_threads, const vdx_batch_searcher& searcher,
-                                     bool errors_set, const vdx_build_plan* plan, std::uint64_t synced_version) {
+                                     bool errors_set, const vdx_build_plan* plan, std::uint64_t synced_version,
+                                     vdx_build_stats* stats) {
     require(g != nullptr, "HnswIndex::add: graph is null");
     require(rows == 0 || data != nullptr, "HnswIndex::add: data is null");
     require(searcher.sync != nullptr && searcher.search != nullptr, "HnswIndex::add: searcher callback is null");
@@ -99,7 +100,12 @@ std::uint64_t host_graph_add_batched(vdx_host_graph* g, const float* data, std::
         p.batch_max = plan->batch_max;
         p.batch_fraction = plan->batch_fraction;
     }
-    g->graph->add_batched(data, rows, num_threads, hooks, p, synced_version);
+    BuildStats st;
+    g->graph->add_batched(data, rows, num_threads, hooks, p, synced_version, &st);
+    if (stats != nullptr) {
+        *stats = vdx_build_stats{st.batches, st.searched, st.host_inserted, st.host_insert_s,
+                                 st.sync_s,  st.search_s, st.link_s,        st.total_s};
+    }
     return g->graph->version();
 }
 
@@ -155,10 +161,10 @@ int vdx_host_graph_load(const char* path, vdx_host_graph** out) {
 
 int vdx_host_graph_add_batched(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t dim,
                                uint64_t num_threads, const vdx_batch_searcher* searcher, const vdx_build_plan* plan,
-                               uint64_t synced_version) {
+                               uint64_t synced_version, vdx_build_stats* stats) {
     return guarded([&] {
         require(searcher != nullptr, "vdx_host_graph_add_batched: searcher is null");
-        vdx::host_graph_add_batched(g, data, rows, dim, num_threads, *searcher, false, plan, synced_version);
+        vdx::host_graph_add_batched(g, data, rows, dim, num_threads, *searcher, false, plan, synced_version, stats);
     });
 }
 

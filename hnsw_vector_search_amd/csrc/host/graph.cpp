@@ -6,6 +6,7 @@
 #include "graph.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cassert>
 #include <cmath>
 
@@ -212,7 +213,20 @@ void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads
 }
 
 void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num_threads,
-                            BatchSearcher& searcher, const BatchPlan& plan, std::uint64_t synced_version) {
+                            BatchSearcher& searcher, const BatchPlan& plan, std::uint64_t synced_version,
+                            BuildStats* stats) {
+    using clock = std::chrono::steady_clock;
+    BuildStats local;
+    BuildStats& st = stats != nullptr ? *stats : local;
+    st = BuildStats{};
+    const auto start = clock::now();
+    auto lap = clock::now();
+    const auto since = [&lap] {
+        const auto now = clock::now();
+        const double s = std::chrono::duration<double>(now - lap).count();
+        lap = now;
+        return s;
+    };
     check_new_vectors("HnswIndex::add", IndexShape{metric_, dim_, size()}, dim_, data, rows);
     if (plan.batch_min == 0 || plan.batch_max < plan.batch_min || !(plan.batch_fraction >= 0.0)) {
         throw std::invalid_argument("HnswIndex::add: batch sizes must satisfy 1 <= batch_min <= batch_max "
@@ -245,17 +259,21 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
         next = 1;
     }
     const std::size_t prefix_end = std::min(total, std::max(next, plan.host_prefix));
+    since();
     if (prefix_end > next) {
         parallel_for(prefix_end - next, threads,
                      [this, next](std::size_t i) { insert<Locking::PerNode>(static_cast<Id>(next + i)); });
+        st.host_inserted += prefix_end - next;
         next = prefix_end;
     }
+    st.host_insert_s += since();
     // Stamps written after a sync carry a later version than the one synced.
     std::uint64_t synced = synced_version;
     const auto sync = [&] {
         searcher.sync(*this, changed_since(synced));
         synced = version_;
         ++version_;
+        st.sync_s += since();
     };
     const std::size_t ef = std::max(params_.ef_construction, params_.M);
     std::vector<Id> high, low;
@@ -273,17 +291,24 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
         if (!high.empty()) {
             parallel_for(high.size(), std::min(threads, high.size()),
                          [this, &high](std::size_t i) { insert<Locking::PerNode>(high[i]); });
+            st.host_inserted += high.size();
         }
+        st.host_insert_s += since();
         sync();
         if (!low.empty()) {
             searcher.search(low, ef, found, count);
+            st.search_s += since();
+            st.searched += low.size();
             parallel_for(low.size(), std::min(threads, low.size()), [this, &low, &found, &count, ef](std::size_t i) {
                 link_searched(low[i], found.data() + i * ef, count[i]);
             });
+            st.link_s += since();
         }
+        ++st.batches;
         next = end;
     }
     sync();
+    st.total_s = std::chrono::duration<double>(clock::now() - start).count();
 }
 
 // The linking half of insert for a level-0 node whose layer-0 search results are `found`.

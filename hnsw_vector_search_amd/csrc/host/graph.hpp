@@ -63,9 +63,14 @@ public:
 // [batch_min, batch_max].
 struct BatchPlan {
     std::size_t host_prefix = 16384;
-    std::size_t batch_min = 256;
-    std::size_t batch_max = 65536;
-    double batch_fraction = 0.02;
+    std::size_t batch_min = 1024;
+    std::size_t batch_max = 131072;
+    double batch_fraction = 0.05;
+};
+
+struct BuildStats {
+    std::size_t batches = 0, searched = 0, host_inserted = 0;
+    double host_insert_s = 0, sync_s = 0, search_s = 0, link_s = 0, total_s = 0;
 };
 
 class HnswGraph {
@@ -107,7 +112,7 @@ public:
     // searcher's copy was last synced at; the copy is current when this returns. A batch of one
     // node with one thread builds exactly what add() with one thread builds.
     void add_batched(const float* data, std::size_t rows, std::size_t num_threads, BatchSearcher& searcher,
-                     const BatchPlan& plan, std::uint64_t synced_version);
+                     const BatchPlan& plan, std::uint64_t synced_version, BuildStats* stats = nullptr);
 
     // Bumped by every add(); a device copy remembers the version it was made from.
     std::uint64_t version() const noexcept { return version_; }

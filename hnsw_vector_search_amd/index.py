@@ -185,6 +185,7 @@ class HnswIndex:
         self._gpu = None
         self._gpu_version = None
         self._gpu_lock = threading.Lock()
+        self.last_build_stats = None  # vdx_build_stats of the last gpu_assisted add
 
     # ---- properties -------------------------------------------------------------------
     def _view(self) -> nat.GraphView:
@@ -218,11 +219,14 @@ class HnswIndex:
         arr = _as_matrix(vectors)
         if gpu_assisted:
             handle = self.gpu_handle()
+            stats = nat.BuildStats()
             with self._gpu_lock:
                 nat.check(nat.lib.vdx_gpu_hnsw_build_add(
                     handle, self._graph.handle, nat.ptr(arr), arr.shape[0], arr.shape[1], num_threads,
-                    ctypes.byref(plan) if plan is not None else None, self._gpu_version))
+                    ctypes.byref(plan) if plan is not None else None, self._gpu_version,
+                    ctypes.byref(stats)))
                 self._gpu_version = self._graph.params()[2]
+            self.last_build_stats = {name: getattr(stats, name) for name, _ in stats._fields_}
             return
         nat.check(nat.lib.vdx_host_graph_add(self._graph.handle, nat.ptr(arr), arr.shape[0],
                                              arr.shape[1], num_threads))

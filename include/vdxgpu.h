@@ -100,18 +100,24 @@ typedef struct vdx_batch_searcher {
                   uint32_t* counts);
 } vdx_batch_searcher;
 /* Batch sizes: the first host_prefix nodes of the graph are inserted on the host; then batches
- * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 256,
- * 65536, 0.02. batch_min = batch_max = 1 with host_prefix 0 and one thread builds exactly
+ * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 1024,
+ * 131072, 0.05. batch_min = batch_max = 1 with host_prefix 0 and one thread builds exactly
  * what vdx_host_graph_add with one thread builds (given exact searches). */
 typedef struct vdx_build_plan {
     uint64_t host_prefix, batch_min, batch_max;
     double batch_fraction;
 } vdx_build_plan;
+/* Where a batched add spent its time (seconds, wall clock) and how it was split. */
+typedef struct vdx_build_stats {
+    uint64_t batches, searched, host_inserted;  /* batches; nodes searched by the searcher; on the host */
+    double host_insert_s, sync_s, search_s, link_s, total_s;
+} vdx_build_stats;
 /* `synced_version`: the graph version the searcher's copy reflects (vdx_host_graph_params);
- * the first sync lists everything changed since. */
+ * the first sync lists everything changed since. `stats` may be NULL. */
 VDX_API int vdx_host_graph_add_batched(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t dim,
                                        uint64_t num_threads, const vdx_batch_searcher* searcher,
-                                       const vdx_build_plan* plan, uint64_t synced_version);
+                                       const vdx_build_plan* plan, uint64_t synced_version,
+                                       vdx_build_stats* stats);
 
 /* Nodes whose link blocks an add() after `since_version` wrote (the nodes it stored and those it
  * linked back to), ascending; with nodes NULL only the count. No counterpart in vecdex: it
@@ -208,7 +214,7 @@ VDX_API int vdx_gpu_hnsw_update(vdx_gpu_hnsw* index, const vdx_graph_view* graph
  * each batch's searches are one beam launch over the nodes' vectors already in HBM. */
 VDX_API int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const float* data, uint64_t rows,
                                    uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan,
-                                   uint64_t synced_version);
+                                   uint64_t synced_version, vdx_build_stats* stats);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 
 /* Knobs of one search launch. Zero-initialise for defaults (autotuned). */

@@ -79,7 +79,7 @@ def add_batched(index, data, threads, plan, searcher):
     version = index._graph.params()[2]
     return nat.lib.vdx_host_graph_add_batched(index._graph.handle, nat.ptr(data), data.shape[0],
                                               data.shape[1], threads, ctypes.byref(searcher.c),
-                                              ctypes.byref(plan), version)
+                                              ctypes.byref(plan), version, None)
 
 
 def sections(index):
