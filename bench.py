@@ -183,6 +183,7 @@ def main(argv=None) -> int:
                     help="also run the 10M x 128 M=32 base (host graph build takes minutes)")
     ap.add_argument("--n10m", type=int, default=10_000_000)
     ap.add_argument("--no-gpu-build", action="store_true", help="skip the GPU-assisted build leg")
+    ap.add_argument("--no-load", action="store_true", help="skip the FORMAT v1/v2 load leg")
     ap.add_argument("--host-build-10m", action="store_true",
                     help="build the 10M graph on the host alone (default: GPU-assisted)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -254,6 +255,7 @@ def main(argv=None) -> int:
         log(f"fixed ef {ef}: recall {row['recall']:.4f}, {ms:.3f} ms/pass, {row['qps']:,.0f} QPS")
     gpu_build_leg = None if args.no_gpu_build else run_gpu_build(vdx, nat, base, queries, truth, threads,
                                                                   build_s, sweep)
+    load_leg = None if args.no_load else run_load(vdx, index, queries)
     config_points = [s for s in sweep if s["ef"] in EF_SWEEP and s["recall"] >= TARGET_RECALL]
     fixed = min(config_points, key=lambda s: s["ef"]) if config_points else sweep[-1]
     fine_points = [s for s in sweep if s["recall"] >= TARGET_RECALL]
@@ -358,6 +360,7 @@ def main(argv=None) -> int:
         "bruteforce": bf_leg,
         "serving": serving_leg,
         "gpu_build": gpu_build_leg,
+        "load": load_leg,
         "base_10m": base10m_leg,
         "host_entry_point": {"ms": round(host_ms, 4), "qps": round(nq / host_ms * 1000, 1),
                              "note": "queries from and results to host memory (PCIe inclusive)"},
@@ -526,6 +529,32 @@ def run_10m(args, vdx, nat, datasets, threads):
             "qps_at_recall_0.95": at[0] if at else None,
             "serving": {"queries": nq, "one_launch_ms": round(ms_one, 2), "tuned_graph_ms": round(ms_graph, 2),
                         "tuned_graph_qps": round(nq / ms_graph * 1000, 1), "identical_to_one_launch": identical}}
+
+
+def run_load(vdx, index, queries):
+    """Index file to a searchable HBM copy, 1M graph: FORMAT v1 through the host graph (load,
+    then upload) against FORMAT v2 streamed straight to HBM. The files were just written, so the
+    page cache is warm: this times parsing, checks and copies, not the disk."""
+    CACHE.mkdir(parents=True, exist_ok=True)
+    p1, p2 = CACHE / "load_test.vdx", CACHE / "load_test.vdx2"
+    index.save(p1)
+    index.save(p2, version=2)
+    out = {"bytes_v1": p1.stat().st_size, "bytes_v2": p2.stat().st_size}
+    ref, _ = index.search_arrays(queries[:1000], K, 64)
+    for name, fn in (("v1_host_then_upload", lambda: vdx.HnswIndex.load(p1)),
+                     ("v2_host_then_upload", lambda: vdx.HnswIndex.load(p2)),
+                     ("v2_device", lambda: vdx.HnswDeviceIndex.load(p2))):
+        t = time.perf_counter()
+        loaded = fn()
+        loaded.gpu_handle()
+        secs = time.perf_counter() - t
+        got, _ = loaded.search_arrays(queries[:1000], K, 64)
+        out[name] = {"s": round(secs, 3), "identical_results": bool(got.tobytes() == ref.tobytes())}
+        log(f"load {name}: {secs:.2f} s, results identical {out[name]['identical_results']}")
+        del loaded
+    p1.unlink()
+    p2.unlink()
+    return out
 
 
 def run_gpu_build(vdx, nat, base, queries, truth, threads, host_build_s, sweep):

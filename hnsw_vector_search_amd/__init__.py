@@ -6,7 +6,8 @@ The classes load libvdxgpu.so on first use, so `python -m hnsw_vector_search_amd
 before the library exists or while it is stale."""
 
 __version__ = "0.1.0"
-_INDEX = ("BruteForceIndex", "GraphReport", "HnswIndex", "Index", "Metric", "Neighbor", "SearchBatch")
+_INDEX = ("BruteForceIndex", "GraphReport", "HnswDeviceIndex", "HnswIndex", "Index", "Metric", "Neighbor",
+          "SearchBatch")
 _RECALL = ("recall_at_k", "threshold_recall_at_k", "mean_recall_at_k", "mean_threshold_recall_at_k")
 __all__ = [*_INDEX, *_RECALL, "__version__"]
 

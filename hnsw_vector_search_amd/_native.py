@@ -95,6 +95,7 @@ _SIGS = {
     "vdx_host_graph_view": (c_int, [c_ptr, ctypes.POINTER(GraphView)]),
     "vdx_host_graph_params": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
     "vdx_host_graph_changes": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_ptr]),
+    "vdx_host_graph_save_v2": (c_int, [c_ptr, c_char_p]),
     "vdx_host_graph_add_batched": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BatchSearcher),
                                            ctypes.POINTER(BuildPlan), c_u64, ctypes.POINTER(BuildStats)]),
     "vdx_host_graph_memory_bytes": (c_u64, [c_ptr]),
@@ -125,6 +126,8 @@ _SIGS = {
     "vdx_gpu_event_elapsed": (c_int, [c_ptr, c_u32, c_u32, ctypes.POINTER(c_float)]),
     "vdx_gpu_hnsw_upload": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr]),
     "vdx_gpu_hnsw_update": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr, c_u64]),
+    "vdx_gpu_hnsw_load": (c_int, [c_ptr, c_char_p, c_ptr]),
+    "vdx_gpu_hnsw_info": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "vdx_gpu_hnsw_build_add": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BuildPlan),
                                        c_u64, ctypes.POINTER(BuildStats)]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),

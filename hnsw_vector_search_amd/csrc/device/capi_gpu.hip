@@ -14,6 +14,7 @@
 
 #include "../host/batched_build.hpp"
 #include "../host/capi_common.hpp"
+#include "../host/format_v2.hpp"
 #include "../host/validate.hpp"
 #include "beam_args.hpp"
 #include "beam_host.hpp"
@@ -562,6 +563,113 @@ int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const flo
         b.h = index;
         const vdx_batch_searcher searcher{&b, &build_sync, &build_search};
         vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version, stats);
+    });
+}
+
+int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out) {
+    return guarded([&] {
+        require(ctx != nullptr && path != nullptr && out != nullptr, "vdx_gpu_hnsw_load: null argument");
+        vdx::V2Map map(path);
+        const vdx::V2Header& hd = map.header();
+        if (hd.dim > kMaxGpuDim) {
+            throw std::invalid_argument(vdx::fmt("vdx_gpu_hnsw_load: dimension %llu is above the GPU path's limit of %u",
+                                                 static_cast<unsigned long long>(hd.dim), kMaxGpuDim));
+        }
+        if (hd.M > kMaxGpuM) {
+            throw std::invalid_argument(vdx::fmt("vdx_gpu_hnsw_load: M = %llu is outside the GPU path's range [2, %u]",
+                                                 static_cast<unsigned long long>(hd.M), kMaxGpuM));
+        }
+        vdx::V2Checker verify(map);
+        verify.levels_and_offsets();
+
+        DeviceGuard guard(ctx->device);
+        auto h = std::make_unique<vdx_gpu_hnsw>();
+        h->ctx = ctx;
+        h->metric = hd.metric;
+        h->dim = hd.dim;
+        h->n = hd.n;
+        h->M = hd.M;
+        h->max_level = hd.max_level;
+        h->entry_point = hd.entry_point;
+        h->upper_blocks = hd.upper_blocks;
+        h->split = vdx::gpu::split_dim(hd.dim);
+        require(h->split.ld == hd.ld, "vdx_gpu_hnsw_load: row length differs from the device layout");
+        const std::uint64_t b0 = 1 + 2 * hd.M, bu = 1 + hd.M;
+        h->vectors.resize(std::max<std::uint64_t>(1, hd.n) * hd.ld * 4);
+        h->layer0.resize(std::max<std::uint64_t>(1, hd.n * b0) * 4);
+        h->upper.resize(std::max<std::uint64_t>(1, hd.upper_blocks * bu) * 4);
+        h->upper_offset.resize(std::max<std::uint64_t>(1, hd.n) * 4);
+        h->ring.resize(kRingSlots * 256);
+        hipStream_t s = ctx->stream;
+        if (hd.n > 0) {
+            check(hipMemcpyAsync(h->upper_offset.as<void>(), verify.offsets().data(), hd.n * 4, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(offsets)");
+        }
+        // Two pinned chunks in turn: the file's pages are copied into one (and checked there,
+        // cache-hot) while the other's copy to HBM is in flight.
+        constexpr std::uint64_t kChunkBytes = std::uint64_t{64} << 20;
+        vdx::gpu::PinnedBuffer pinned[2];
+        hipEvent_t done[2] = {nullptr, nullptr};
+        struct Drain {  // no copy may still read a pinned chunk when it is freed
+            hipStream_t s;
+            hipEvent_t* ev;
+            ~Drain() {
+                (void)hipStreamSynchronize(s);
+                for (int i = 0; i < 2; ++i) {
+                    if (ev[i] != nullptr) (void)hipEventDestroy(ev[i]);
+                }
+            }
+        } drain{s, done};
+        for (int i = 0; i < 2; ++i) {
+            check(hipEventCreateWithFlags(&done[i], hipEventDisableTiming), "hipEventCreate");
+        }
+        std::uint64_t chunk_no = 0;
+        bool in_flight[2] = {false, false};
+        const auto stream_section = [&](int sec, std::uint64_t unit_bytes, std::uint64_t units, char* dst,
+                                        const auto& check_chunk) {
+            const std::uint64_t per = std::max<std::uint64_t>(1, kChunkBytes / unit_bytes);
+            for (std::uint64_t u0 = 0; u0 < units; u0 += per, ++chunk_no) {
+                const std::uint64_t m = std::min(per, units - u0);
+                const std::uint64_t bytes = m * unit_bytes;
+                const int slot = static_cast<int>(chunk_no & 1);
+                if (in_flight[slot]) {
+                    check(hipEventSynchronize(done[slot]), "hipEventSynchronize");
+                }
+                pinned[slot].reserve(std::min(units, per) * unit_bytes);
+                std::memcpy(pinned[slot].as<void>(), map.section(sec) + u0 * unit_bytes, bytes);
+                check_chunk(pinned[slot].as<void>(), u0, m);
+                check(hipMemcpyAsync(dst + u0 * unit_bytes, pinned[slot].as<void>(), bytes, hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync(section)");
+                check(hipEventRecord(done[slot], s), "hipEventRecord");
+                in_flight[slot] = true;
+            }
+        };
+        stream_section(vdx::kV2Layer0, b0 * 4, hd.n, h->layer0.as<char>(), [&](void* p, std::uint64_t first, std::uint64_t m) {
+            verify.layer0(static_cast<const std::uint32_t*>(p), first, m);
+        });
+        stream_section(vdx::kV2Upper, bu * 4, hd.upper_blocks, h->upper.as<char>(),
+                       [&](void* p, std::uint64_t first, std::uint64_t m) {
+                           verify.upper(static_cast<const std::uint32_t*>(p), first, m);
+                       });
+        stream_section(vdx::kV2Vectors, hd.ld * 4, hd.n, h->vectors.as<char>(),
+                       [&](void* p, std::uint64_t first, std::uint64_t m) {
+                           verify.vectors(static_cast<const float*>(p), first, m);
+                       });
+        verify.finish();
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(load)");
+        h->levels = verify.levels();
+        h->offsets = verify.offsets();
+        *out = h.release();
+    });
+}
+
+int vdx_gpu_hnsw_info(const vdx_gpu_hnsw* h, uint32_t* metric, uint64_t* dim, uint64_t* n, uint64_t* M) {
+    return guarded([&] {
+        require(h != nullptr, "vdx_gpu_hnsw_info: index is null");
+        if (metric) *metric = h->metric;
+        if (dim) *dim = h->dim;
+        if (n) *n = h->n;
+        if (M) *M = h->M;
     });
 }
 

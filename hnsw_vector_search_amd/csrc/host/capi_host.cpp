@@ -150,6 +150,13 @@ int vdx_host_graph_save(const vdx_host_graph* g, const char* path) {
     });
 }
 
+int vdx_host_graph_save_v2(const vdx_host_graph* g, const char* path) {
+    return guarded([&] {
+        require(g != nullptr && path != nullptr, "vdx_host_graph_save_v2: null argument");
+        g->graph->save_v2(path);
+    });
+}
+
 int vdx_host_graph_load(const char* path, vdx_host_graph** out) {
     return guarded([&] {
         require(out != nullptr && path != nullptr, "vdx_host_graph_load: null argument");

@@ -14,6 +14,7 @@
 #include <random>
 #include <system_error>
 
+#include "file_io.hpp"
 #include "graph.hpp"
 #include "validate.hpp"
 #include "xxh64.hpp"
@@ -29,9 +30,7 @@ constexpr std::uint64_t kHeaderBytes = 80;
 constexpr std::uint64_t kChecksumBytes = 8;
 constexpr std::size_t kChunk = std::size_t{1} << 20;
 
-[[noreturn]] void fail(const std::string& path, const std::string& what) {
-    throw std::runtime_error(path + ": " + what);
-}
+[[noreturn]] void fail(const std::string& path, const std::string& what) { fail_path(path, what); }
 
 // Streams bytes to a file and hashes them on the way.
 class HashingWriter {
@@ -86,54 +85,11 @@ private:
     Xxh64 hash_;
 };
 
-std::uint64_t mul_checked(std::uint64_t a, std::uint64_t b, const std::string& path) {
-    if (b != 0 && a > std::numeric_limits<std::uint64_t>::max() / b) {
-        fail(path, "the header's sizes overflow 64 bits");
-    }
-    return a * b;
-}
-
-std::uint64_t add_checked(std::uint64_t a, std::uint64_t b, const std::string& path) {
-    if (a > std::numeric_limits<std::uint64_t>::max() - b) {
-        fail(path, "the header's sizes overflow 64 bits");
-    }
-    return a + b;
-}
-
-void sync_path(const std::string& path, bool directory) {
-    const int fd = ::open(path.c_str(), directory ? (O_RDONLY | O_DIRECTORY) : O_RDONLY);
-    if (fd < 0) {
-        throw std::system_error(errno, std::generic_category());
-    }
-    const int rc = ::fsync(fd);
-    const int err = errno;
-    ::close(fd);
-    if (rc != 0) {
-        throw std::system_error(err, std::generic_category());
-    }
-}
-
 }  // namespace
 
 void HnswGraph::save(const std::string& path) const {
     const bool empty = size() == 0;
-    std::random_device random;
-    const std::uint64_t suffix = (std::uint64_t{random()} << 32U) | random();
-    const std::string temp = path + fmt(".tmp-%016llx", static_cast<unsigned long long>(suffix));
-    struct Cleanup {
-        std::string path;
-        ~Cleanup() {
-            if (!path.empty()) {
-                std::error_code ignored;
-                fs::remove(path, ignored);
-            }
-        }
-    } cleanup{temp};
-    {
-        std::ofstream out(temp, std::ios::binary | std::ios::trunc);
-        if (!out) {
-            fail(path, "cannot create the temporary file " + temp);
-        }
+    save_atomically(path, [&](std::ofstream& out) {
         HashingWriter w(out);
         w.write(kMagic, 8);
         w.u32(kVersion);
@@ -151,29 +107,7 @@ void HnswGraph::save(const std::string& path) const {
         w.write(upper_.data(), upper_.size() * sizeof(Id));
         w.write(vectors_.data(), vectors_.size() * sizeof(float));
         w.u64(w.digest());
-        out.close();
-        if (!out) {
-            fail(path, "cannot write the temporary file " + temp);
-        }
-    }
-    try {
-        sync_path(temp, false);
-    } catch (const std::system_error& e) {
-        fail(path, "cannot flush the temporary file " + temp + " to storage: " + e.what());
-    }
-    std::error_code ec;
-    fs::rename(temp, path, ec);
-    if (ec) {
-        fail(path, "cannot replace it with " + temp + ": " + ec.message());
-    }
-    cleanup.path.clear();
-    const fs::path parent = fs::path(path).parent_path();
-    const std::string dir = parent.empty() ? std::string(".") : parent.string();
-    try {
-        sync_path(dir, true);
-    } catch (const std::system_error& e) {
-        fail(path, "saved, but cannot flush the directory " + dir + " to storage: " + e.what());
-    }
+    });
 }
 
 std::unique_ptr<HnswGraph> HnswGraph::load(const std::string& path) {
@@ -211,9 +145,12 @@ std::unique_ptr<HnswGraph> HnswGraph::load(const std::string& path) {
     if (std::memcmp(magic, kMagic, 8) != 0) {
         fail(path, "not a vecdex HNSW index (wrong magic bytes)");
     }
+    if (version == 2) {
+        in.close();
+        return load_v2(path);
+    }
     if (version != kVersion) {
-        fail(path, fmt("format version %u is not supported (this build reads version %u)", version,
-                       kVersion));
+        fail(path, fmt("format version %u is not supported (this build reads versions 1 and 2)", version));
     }
     if (!valid_metric(metric)) {
         fail(path, fmt("unknown metric code %u", metric));

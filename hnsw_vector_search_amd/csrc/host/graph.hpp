@@ -102,7 +102,11 @@ public:
     std::size_t memory_bytes() const noexcept;
 
     void save(const std::string& path) const;
+    // FORMAT v1 or v2 (dispatching on the version field).
     static std::unique_ptr<HnswGraph> load(const std::string& path);
+    // FORMAT v2: sections aligned and laid out as the device copy holds them.
+    void save_v2(const std::string& path) const;
+    static std::unique_ptr<HnswGraph> load_v2(const std::string& path);
 
     // add() with the layer-0 searches of level-0 nodes done in batches by `searcher`: per batch,
     // the batch's nodes with upper layers are inserted on the host (all layers), the searcher's

@@ -170,7 +170,52 @@ class _HostGraph:
         return out
 
 
-class HnswIndex:
+class _GpuSearch:
+    """The search methods shared by the host-graph index and the device-only index; the class
+    provides gpu_handle()."""
+
+    def search_arrays(self, queries: np.ndarray, k, ef, *, where: str = "HnswIndex::search_batch",
+                      opts: nat.SearchOpts | None = None):
+        """GPU search of prepared-or-raw host queries; k and ef may be ints or per-query
+        arrays. Returns (neighbors[nq, kmax] structured array, counts[nq])."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q.reshape(1, -1)
+        nq = q.shape[0]
+        ks = None if np.isscalar(k) else np.ascontiguousarray(k, dtype=np.uint32)
+        efs = None if np.isscalar(ef) else np.ascontiguousarray(ef, dtype=np.uint32)
+        kmax = int(k) if ks is None else int(ks.max(initial=1))
+        out = np.zeros((nq, max(kmax, 1)), dtype=nat.NEIGHBOR_DTYPE)
+        count = np.zeros(nq, dtype=np.uint32)
+        if nq == 0:
+            return out, count
+        handle = self.gpu_handle()
+        nat.check(nat.lib.vdx_gpu_hnsw_search(
+            handle, nat.ptr(q), nq, q.shape[1], nat.ptr(ks), 0 if ks is not None else int(k),
+            nat.ptr(efs), 0 if efs is not None else int(ef), nat.ptr(out), out.shape[1],
+            nat.ptr(count), where.encode(), ctypes.byref(opts) if opts is not None else None))
+        return out, count
+
+    def search(self, query: Sequence[float], k: int, ef: int) -> list:
+        """Up to k approximate nearest neighbors, closest first (HnswIndex::search)."""
+        q = np.asarray(query, dtype=np.float32).reshape(1, -1)
+        if k == 0:
+            raise ValueError("HnswIndex::search: k must be positive")
+        out, count = self.search_arrays(q, k, ef, where="HnswIndex::search")
+        return [Neighbor(float(d), int(i)) for d, i in out[0][: count[0]]]
+
+    def search_batch(self, queries, k: int, ef: int, num_threads: int = 0) -> list:
+        """search() for every row (HnswIndex::search_batch); num_threads is accepted and has no
+        effect on results."""
+        q = _as_matrix(queries)
+        if k == 0 and q.shape[0]:
+            raise ValueError("HnswIndex::search_batch: k must be positive")
+        out, count = self.search_arrays(q, k, ef, where="HnswIndex::search_batch")
+        return [[Neighbor(float(d), int(i)) for d, i in out[r][: count[r]]]
+                for r in range(q.shape[0])]
+
+
+class HnswIndex(_GpuSearch):
     """vecdex::HnswIndex with search/search_batch on the GPU.
 
     HnswIndex(dim, metric=Metric.L2, M=16, ef_construction=200, seed=42)
@@ -251,8 +296,13 @@ class HnswIndex:
     def memory_bytes(self) -> int:
         return nat.lib.vdx_host_graph_memory_bytes(self._graph.handle)
 
-    def save(self, path) -> None:
-        nat.check(nat.lib.vdx_host_graph_save(self._graph.handle, os.fsencode(os.fspath(path))))
+    def save(self, path, version: int = 1) -> None:
+        """HnswIndex::save (FORMAT v1), or version=2: this library's aligned, mappable format,
+        which HnswDeviceIndex.load streams straight to HBM. load() reads both."""
+        if version not in (1, 2):
+            raise ValueError(f"HnswIndex::save: format version {version} is not 1 or 2")
+        fn = nat.lib.vdx_host_graph_save if version == 1 else nat.lib.vdx_host_graph_save_v2
+        nat.check(fn(self._graph.handle, os.fsencode(os.fspath(path))))
 
     @classmethod
     def load(cls, path, *, device: int = 0) -> "HnswIndex":
@@ -302,46 +352,49 @@ class HnswIndex:
             nat.lib.vdx_gpu_hnsw_free(self._gpu)
             self._gpu = None
 
-    # ---- search (GPU) -----------------------------------------------------------------
-    def search_arrays(self, queries: np.ndarray, k, ef, *, where: str = "HnswIndex::search_batch",
-                      opts: nat.SearchOpts | None = None):
-        """GPU search of prepared-or-raw host queries; k and ef may be ints or per-query
-        arrays. Returns (neighbors[nq, kmax] structured array, counts[nq])."""
-        q = np.ascontiguousarray(queries, dtype=np.float32)
-        if q.ndim == 1:
-            q = q.reshape(1, -1)
-        nq = q.shape[0]
-        ks = None if np.isscalar(k) else np.ascontiguousarray(k, dtype=np.uint32)
-        efs = None if np.isscalar(ef) else np.ascontiguousarray(ef, dtype=np.uint32)
-        kmax = int(k) if ks is None else int(ks.max(initial=1))
-        out = np.zeros((nq, max(kmax, 1)), dtype=nat.NEIGHBOR_DTYPE)
-        count = np.zeros(nq, dtype=np.uint32)
-        if nq == 0:
-            return out, count
-        handle = self.gpu_handle()
-        nat.check(nat.lib.vdx_gpu_hnsw_search(
-            handle, nat.ptr(q), nq, q.shape[1], nat.ptr(ks), 0 if ks is not None else int(k),
-            nat.ptr(efs), 0 if efs is not None else int(ef), nat.ptr(out), out.shape[1],
-            nat.ptr(count), where.encode(), ctypes.byref(opts) if opts is not None else None))
-        return out, count
+class HnswDeviceIndex(_GpuSearch):
+    """A graph held in HBM only, loaded from a FORMAT v2 file (vdx_gpu_hnsw_load): the file is
+    mapped and each section streamed to the device through pinned chunks, every chunk checked
+    before its copy; no host copy of the graph is made. Serves the same searches as HnswIndex
+    (search, search_batch, search_arrays, SearchBatch); adding needs the host graph
+    (HnswIndex.load)."""
 
-    def search(self, query: Sequence[float], k: int, ef: int) -> list:
-        """Up to k approximate nearest neighbors, closest first (HnswIndex::search)."""
-        q = np.asarray(query, dtype=np.float32).reshape(1, -1)
-        if k == 0:
-            raise ValueError("HnswIndex::search: k must be positive")
-        out, count = self.search_arrays(q, k, ef, where="HnswIndex::search")
-        return [Neighbor(float(d), int(i)) for d, i in out[0][: count[0]]]
+    def __init__(self, handle, device: int):
+        self._gpu = handle
+        self._device = device
 
-    def search_batch(self, queries, k: int, ef: int, num_threads: int = 0) -> list:
-        """search() for every row (HnswIndex::search_batch); num_threads is accepted and has no
-        effect on results."""
-        q = _as_matrix(queries)
-        if k == 0 and q.shape[0]:
-            raise ValueError("HnswIndex::search_batch: k must be positive")
-        out, count = self.search_arrays(q, k, ef, where="HnswIndex::search_batch")
-        return [[Neighbor(float(d), int(i)) for d, i in out[r][: count[r]]]
-                for r in range(q.shape[0])]
+    @classmethod
+    def load(cls, path, device: int = 0) -> "HnswDeviceIndex":
+        ctx = nat.Context.get(device)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib.vdx_gpu_hnsw_load(ctx.handle, os.fsencode(os.fspath(path)), ctypes.byref(h)))
+        return cls(h, device)
+
+    def gpu_handle(self):
+        return self._gpu
+
+    def _info(self):
+        metric, dim, n, M = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        nat.check(nat.lib.vdx_gpu_hnsw_info(self._gpu, ctypes.byref(metric), ctypes.byref(dim), ctypes.byref(n),
+                                            ctypes.byref(M)))
+        return metric.value, dim.value, n.value, M.value
+
+    def dim(self) -> int:
+        return self._info()[1]
+
+    def size(self) -> int:
+        return self._info()[2]
+
+    def metric(self) -> int:
+        return self._info()[0]
+
+    def __len__(self) -> int:
+        return self.size()
+
+    def __del__(self):
+        if getattr(self, "_gpu", None):
+            nat.lib.vdx_gpu_hnsw_free(self._gpu)
+            self._gpu = None
 
 
 class SearchBatch:
@@ -351,7 +404,7 @@ class SearchBatch:
     prepared (unit length under cosine). Each query's result equals HnswIndex.search at its
     (ef, k). Keep the index alive while the batch is in use."""
 
-    def __init__(self, index: HnswIndex, max_nq: int, max_k: int, max_ef: int,
+    def __init__(self, index: "HnswIndex | HnswDeviceIndex", max_nq: int, max_k: int, max_ef: int,
                  opts: nat.SearchOpts | None = None):
         self.index = index
         self.max_k = int(max_k)

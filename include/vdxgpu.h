@@ -77,9 +77,14 @@ VDX_API int vdx_host_graph_create(uint64_t dim, uint32_t metric, uint64_t M, uin
 /* HnswIndex::add(MatrixView, num_threads) (include/vecdex/hnsw.hpp:87; src/hnsw.cpp:259-291). */
 VDX_API int vdx_host_graph_add(vdx_host_graph* g, const float* data, uint64_t rows, uint64_t given_dim,
                        uint64_t num_threads);
-/* HnswIndex::save / load (include/vecdex/hnsw.hpp:115,123; src/hnsw_io.cpp:341-467). */
+/* HnswIndex::save / load (include/vecdex/hnsw.hpp:115,123; src/hnsw_io.cpp:341-467). load reads
+ * FORMAT v1 and this library's FORMAT v2 (dispatching on the version field). */
 VDX_API int vdx_host_graph_save(const vdx_host_graph* g, const char* path);
 VDX_API int vdx_host_graph_load(const char* path, vdx_host_graph** out);
+/* FORMAT v2 (no counterpart in vecdex, whose v1 is not mappable, README.md:329): every section
+ * 4 KiB-aligned and laid out as the device copy holds it (vectors padded to rows of dim rounded
+ * up to 4), per-section XXH64, written atomically and durably as save does. */
+VDX_API int vdx_host_graph_save_v2(const vdx_host_graph* g, const char* path);
 /* Zero-copy view of the graph's sections, valid until the next add or free. */
 VDX_API int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out);
 VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
@@ -215,6 +220,13 @@ VDX_API int vdx_gpu_hnsw_update(vdx_gpu_hnsw* index, const vdx_graph_view* graph
 VDX_API int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const float* data, uint64_t rows,
                                    uint64_t dim, uint64_t num_threads, const vdx_build_plan* plan,
                                    uint64_t synced_version, vdx_build_stats* stats);
+/* A FORMAT v2 file mapped and streamed to HBM with no host graph: each section goes over in
+ * chunks through pinned buffers, every chunk checked (v1's load checks, v2's layout, checksums
+ * and padding) before its copy is queued; on any failure nothing is kept. */
+VDX_API int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out);
+/* The device copy's shape. */
+VDX_API int vdx_gpu_hnsw_info(const vdx_gpu_hnsw* index, uint32_t* metric, uint64_t* dim, uint64_t* n,
+                              uint64_t* M);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 
 /* Knobs of one search launch. Zero-initialise for defaults (autotuned). */
