@@ -42,7 +42,7 @@ class GraphView(ctypes.Structure):
 
 
 SYNC_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(GraphView), ctypes.POINTER(c_u32), c_u64)
-SEARCH_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(c_u32), c_u64, c_u32, c_ptr,
+SEARCH_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), c_u64, c_u32, c_ptr,
                              ctypes.POINTER(c_u32))
 
 

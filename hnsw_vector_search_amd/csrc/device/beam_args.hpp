@@ -38,6 +38,9 @@ struct BeamArgs {
     std::uint32_t k_cap;  // per-query k is clamped to this (<= out_stride, <= ef_cap)
     // launch state
     std::uint32_t* queue;  // next query to take; zero before the launch
+    // optional per-query layer-0 entry: entries[q] < n starts the layer-0 search there with no
+    // descent (an insertion search, which starts from its layer-1 result); other values descend
+    const std::uint32_t* entries;
     // optional work list: the launch serves queries order[range[0] .. range[1]) (device
     // values, written by an earlier launch on the stream); null: queries 0 .. nq
     const std::uint32_t* order;

@@ -83,7 +83,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t max_efp, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
-                  const vdx::dev::BeamArgs* trace = nullptr) {
+                  const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr) {
     const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
     const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
     vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
@@ -96,6 +96,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
         a.trace_final = trace->trace_final;
         a.trace_cap = trace->trace_cap;
     }
+    a.entries = d_entries;
     check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
     vdx::gpu::launch_beam(h, plan, mode, a, stream);
 }
@@ -293,7 +294,7 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
 // batch's insertion searches one beam launch over the nodes' own vectors gathered from HBM.
 struct BuildSearch {
     vdx_gpu_hnsw* h = nullptr;
-    DeviceBuffer ids, queries, out, count, queue;
+    DeviceBuffer ids, queries, out, count, queue;  // ids: the nodes, then their layer-0 entries
     vdx::gpu::PinnedBuffer h_ids, h_out, h_count;
 };
 
@@ -301,8 +302,8 @@ int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* chang
     return guarded([&] { update_index(static_cast<BuildSearch*>(ctx)->h, view, changed, count, true); });
 }
 
-int build_search(void* ctx, const std::uint32_t* nodes, std::uint64_t count, std::uint32_t ef, vdx_neighbor* out,
-                 std::uint32_t* counts) {
+int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* entries, std::uint64_t count,
+                 std::uint32_t ef, vdx_neighbor* out, std::uint32_t* counts) {
     return guarded([&] {
         BuildSearch& b = *static_cast<BuildSearch*>(ctx);
         vdx_gpu_hnsw& h = *b.h;
@@ -313,29 +314,36 @@ int build_search(void* ctx, const std::uint32_t* nodes, std::uint64_t count, std
         require(count < 0x80000000ull, "HnswIndex::add: batch too large");
         for (std::uint64_t i = 0; i < count; ++i) {
             require(nodes[i] < h.n, "HnswIndex::add: a batch node is not in the device copy");
+            require(entries == nullptr || entries[i] < h.n || entries[i] == 0xFFFFFFFFu,
+                    "HnswIndex::add: a layer-0 entry is not in the device copy");
         }
         if (count == 0) return;
         DeviceGuard guard(h.ctx->device);
         hipStream_t s = h.ctx->stream;
         // sizes with headroom: batches grow
         const auto grow = [](std::uint64_t bytes) { return bytes + bytes / 2; };
-        b.h_ids.reserve(grow(count * 4));
+        b.h_ids.reserve(grow(count * 8));
         b.h_out.reserve(grow(count * ef * sizeof(vdx_neighbor)));
         b.h_count.reserve(grow(count * 4));
-        b.ids.reserve(grow(count * 4));
+        b.ids.reserve(grow(count * 8));
         b.queries.reserve(grow(count * h.dim * 4));
         b.out.reserve(grow(count * ef * sizeof(vdx_neighbor)));
         b.count.reserve(grow(count * 4));
         b.queue.reserve(64);
-        std::memcpy(b.h_ids.as<void>(), nodes, count * 4);
-        check(hipMemcpyAsync(b.ids.as<void>(), b.h_ids.as<void>(), count * 4, hipMemcpyHostToDevice, s),
+        std::uint32_t* staged = b.h_ids.as<std::uint32_t>();
+        std::memcpy(staged, nodes, count * 4);
+        if (entries != nullptr) {
+            std::memcpy(staged + count, entries, count * 4);
+        }
+        check(hipMemcpyAsync(b.ids.as<void>(), staged, (entries != nullptr ? 8 : 4) * count, hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(batch ids)");
         check(vdx::dev::launch_gather_rows(b.queries.as<float>(), static_cast<std::uint32_t>(h.dim), h.vectors.as<float>(),
                                            h.split.ld, b.ids.as<std::uint32_t>(), count,
                                            static_cast<std::uint32_t>(h.dim), s),
               "gather(batch vectors)");
         enqueue_beam(h, b.queries.as<float>(), count, nullptr, ef, nullptr, ef, ef, b.out.as<vdx::dev::Neighbor>(), ef,
-                     b.count.as<std::uint32_t>(), b.queue.as<std::uint32_t>(), nullptr, nullptr, s);
+                     b.count.as<std::uint32_t>(), b.queue.as<std::uint32_t>(), nullptr, nullptr, s, nullptr,
+                     entries != nullptr ? b.ids.as<std::uint32_t>() + count : nullptr);
         check(hipMemcpyAsync(b.h_out.as<void>(), b.out.as<void>(), count * ef * sizeof(vdx_neighbor),
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(batch results)");

@@ -176,8 +176,17 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             qt[e] = (e < TAIL && (FIXED || e < tail)) ? qp[first_tail + e] : 0.f;
         }
 
-        // Entry point and greedy descent (src/hnsw.cpp:513-516, 416-431).
+        // Entry point and greedy descent (src/hnsw.cpp:513-516, 416-431), or a given layer-0
+        // entry (src/hnsw.cpp:324-335).
         std::uint32_t cur = a.entry_point;
+        std::uint32_t top = a.max_level;
+        if (a.entries != nullptr) {
+            const std::uint32_t e = uniform(a.entries[q]);
+            if (e < a.n) {
+                cur = e;
+                top = 0;
+            }
+        }
         if (lane == 0) {
             fid[0] = cur;
         }
@@ -186,7 +195,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         wave_lds_fence();
         float cur_d = fd[0];
         ++n_dist;
-        for (std::uint32_t layer = a.max_level; layer > 0; --layer) {
+        for (std::uint32_t layer = top; layer > 0; --layer) {
             for (;;) {
                 const std::uint32_t* blk =
                     a.upper + (static_cast<std::size_t>(a.upper_offset[cur]) + (layer - 1)) * a.bu;

@@ -53,12 +53,12 @@ public:
         fill_view(g, &view);
         check(hooks_.sync(hooks_.ctx, &view, changed.data(), changed.size()), "sync");
     }
-    void search(const std::vector<Id>& nodes, std::size_t ef, std::vector<Neighbor>& out,
-                std::vector<std::uint32_t>& count) override {
+    void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef,
+                std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) override {
         out.resize(nodes.size() * ef);
         count.assign(nodes.size(), 0u);
         static_assert(sizeof(Neighbor) == sizeof(vdx_neighbor));
-        check(hooks_.search(hooks_.ctx, nodes.data(), nodes.size(), static_cast<std::uint32_t>(ef),
+        check(hooks_.search(hooks_.ctx, nodes.data(), entries.data(), nodes.size(), static_cast<std::uint32_t>(ef),
                             reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
               "search");
         for (const std::uint32_t c : count) {

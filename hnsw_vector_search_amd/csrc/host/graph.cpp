@@ -276,34 +276,51 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
         st.sync_s += since();
     };
     const std::size_t ef = std::max(params_.ef_construction, params_.M);
-    std::vector<Id> high, low;
+    std::vector<Id> nodes, entries, high;
+    std::vector<std::size_t> high_at;
+    std::vector<UpperPlan> plans;
     std::vector<Neighbor> found;
     std::vector<std::uint32_t> count;
     while (next < total) {
         const auto scaled = static_cast<std::size_t>(plan.batch_fraction * static_cast<double>(next));
         const std::size_t batch = std::clamp(scaled, plan.batch_min, plan.batch_max);
         const std::size_t end = std::min(total, next + batch);
+        nodes.clear();
+        entries.clear();
         high.clear();
-        low.clear();
+        high_at.clear();
         for (std::size_t id = next; id < end; ++id) {
-            (levels_[id] > 0 ? high : low).push_back(static_cast<Id>(id));
+            nodes.push_back(static_cast<Id>(id));
+            entries.push_back(BatchSearcher::kDescend);
+            if (levels_[id] > 0) {
+                high.push_back(static_cast<Id>(id));
+                high_at.push_back(id - next);
+            }
         }
+        // The upper-layer part of the inserts: nothing is written, so no locks.
+        plans.resize(high.size());
         if (!high.empty()) {
             parallel_for(high.size(), std::min(threads, high.size()),
-                         [this, &high](std::size_t i) { insert<Locking::PerNode>(high[i]); });
+                         [this, &high, &plans](std::size_t i) { plan_upper(high[i], plans[i]); });
+            for (std::size_t i = 0; i < high.size(); ++i) {
+                entries[high_at[i]] = plans[i].entry0;
+            }
             st.host_inserted += high.size();
         }
         st.host_insert_s += since();
         sync();
-        if (!low.empty()) {
-            searcher.search(low, ef, found, count);
-            st.search_s += since();
-            st.searched += low.size();
-            parallel_for(low.size(), std::min(threads, low.size()), [this, &low, &found, &count, ef](std::size_t i) {
-                link_searched(low[i], found.data() + i * ef, count[i]);
-            });
-            st.link_s += since();
+        searcher.search(nodes, entries, ef, found, count);
+        st.search_s += since();
+        st.searched += nodes.size();
+        std::vector<UpperPlan*> plan_of(nodes.size(), nullptr);
+        for (std::size_t i = 0; i < high.size(); ++i) {
+            plan_of[high_at[i]] = &plans[i];
         }
+        parallel_for(nodes.size(), std::min(threads, nodes.size()),
+                     [this, &nodes, &found, &count, &plan_of, ef](std::size_t i) {
+                         link_searched(nodes[i], found.data() + i * ef, count[i], plan_of[i]);
+                     });
+        st.link_s += since();
         ++st.batches;
         next = end;
     }
@@ -311,19 +328,58 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
     st.total_s = std::chrono::duration<double>(clock::now() - start).count();
 }
 
-// The linking half of insert for a level-0 node whose layer-0 search results are `found`.
-void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count) {
-    std::vector<Neighbor> chosen = select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
+// The first half of insert (src/hnsw.cpp:315-335) for a node with upper layers, against the
+// graph as it stands: the descent to its level, then for each of its layers down to 1 the layer
+// search and the neighbour choice; the layer-0 search is left to the batch's searcher, from the
+// layer-1 result.
+void HnswGraph::plan_upper(Id node, UpperPlan& plan) {
+    const std::size_t level = levels_[node];
+    Lease lease(*scratch_);
+    Scratch& scratch = *lease;
+    const float* query = vector_of(node);
+    Neighbor entry{distance(query, vector_of(entry_point_)), entry_point_};
+    for (std::size_t layer = max_level_; layer > level; --layer) {
+        entry = greedy_closest<Locking::None>(query, entry, layer, scratch);
+    }
+    const std::size_t ef = std::max(params_.ef_construction, params_.M);
+    const std::size_t top = std::min(level, max_level_);
+    plan.chosen.assign(top + 1, {});
+    for (std::size_t layer = top; layer >= 1; --layer) {
+        search_layer<Locking::None>(query, entry, ef, layer, scratch);
+        entry = *std::min_element(scratch.results.begin(), scratch.results.end(), closer);
+        plan.chosen[layer] = select_neighbors(scratch.results, params_.M);
+    }
+    plan.entry0 = entry.id;
+}
+
+// The linking half of insert for a node whose layer-0 search results are `found` (and, for a
+// node with upper layers, whose upper choices are in `plan`).
+void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count, UpperPlan* plan) {
+    std::vector<Neighbor> chosen0 = select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
+    const std::size_t top = plan != nullptr ? plan->chosen.size() - 1 : 0;
     {
         const std::lock_guard<std::mutex> lock(build_locks_->links[node]);
         touched_[node] = static_cast<std::uint32_t>(version_);
-        Id* b = block(node, 0);
-        b[0] = static_cast<Id>(chosen.size());
-        for (std::size_t i = 0; i < chosen.size(); ++i) {
-            b[1 + i] = chosen[i].id;
+        for (std::size_t layer = 0; layer <= top; ++layer) {
+            const std::vector<Neighbor>& chosen = layer == 0 ? chosen0 : plan->chosen[layer];
+            Id* b = block(node, layer);
+            b[0] = static_cast<Id>(chosen.size());
+            for (std::size_t i = 0; i < chosen.size(); ++i) {
+                b[1 + i] = chosen[i].id;
+            }
         }
     }
-    link_back<Locking::PerNode>(node, 0, chosen);
+    for (std::size_t layer = 0; layer <= top; ++layer) {
+        link_back<Locking::PerNode>(node, layer, layer == 0 ? chosen0 : plan->chosen[layer]);
+    }
+    const std::size_t level = levels_[node];
+    if (level > 0) {
+        const std::lock_guard<std::mutex> lock(build_locks_->entry);
+        if (level > max_level_) {
+            max_level_ = level;
+            entry_point_ = node;
+        }
+    }
 }
 
 template <HnswGraph::Locking L>

@@ -53,9 +53,12 @@ public:
     // Makes the copy current: `changed` lists, ascending, the nodes stored or whose link
     // blocks were written since the previous sync (all of them for the first).
     virtual void sync(const HnswGraph& g, const std::vector<Id>& changed) = 0;
-    // out[i * ef ..] receives the count[i] results of nodes[i], in any order.
-    virtual void search(const std::vector<Id>& nodes, std::size_t ef, std::vector<Neighbor>& out,
-                        std::vector<std::uint32_t>& count) = 0;
+    // out[i * ef ..] receives the count[i] results of nodes[i], in any order. entries[i] is a
+    // layer-0 entry that replaces the descent (insert's layer-0 search starts from its layer-1
+    // result, src/hnsw.cpp:324-335), or kDescend.
+    static constexpr Id kDescend = 0xFFFFFFFFu;
+    virtual void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef,
+                        std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) = 0;
 };
 
 // How add_batched splits the rows: the first `host_prefix` nodes of the graph are inserted on
@@ -108,13 +111,15 @@ public:
     void save_v2(const std::string& path) const;
     static std::unique_ptr<HnswGraph> load_v2(const std::string& path);
 
-    // add() with the layer-0 searches of level-0 nodes done in batches by `searcher`: per batch,
-    // the batch's nodes with upper layers are inserted on the host (all layers), the searcher's
-    // copy is synced, the level-0 nodes' candidates come from one batched search over it, and
-    // each is linked as insert links it (neighbour selection, link-back with pruning). Nodes of
-    // one batch do not see each other as candidates. `synced_version` is the version the
-    // searcher's copy was last synced at; the copy is current when this returns. A batch of one
-    // node with one thread builds exactly what add() with one thread builds.
+    // add() with the layer-0 insertion searches done in batches by `searcher`: per batch, the
+    // upper-layer part of each insert runs on the host (descent, and for nodes with upper
+    // layers their layer searches and neighbour choices), the searcher's copy is synced, one
+    // batched search returns every node's layer-0 candidates (from its descent, or from its
+    // layer-1 result), and each node is linked as insert links it (neighbour selection,
+    // link-back with pruning, entry point). Nodes of one batch do not see each other as
+    // candidates. `synced_version` is the version the searcher's copy was last synced at; the
+    // copy is current when this returns. A batch of one node with one thread builds exactly what
+    // add() with one thread builds.
     void add_batched(const float* data, std::size_t rows, std::size_t num_threads, BatchSearcher& searcher,
                      const BatchPlan& plan, std::uint64_t synced_version, BuildStats* stats = nullptr);
 
@@ -145,7 +150,12 @@ private:
     void insert(Id node);
     template <Locking L>
     void link_back(Id node, std::size_t layer, const std::vector<Neighbor>& chosen);
-    void link_searched(Id node, const Neighbor* found, std::size_t count);
+    struct UpperPlan {
+        std::vector<std::vector<Neighbor>> chosen;  // per layer 1..top (index = layer)
+        Id entry0 = 0;                              // where the layer-0 search starts
+    };
+    void plan_upper(Id node, UpperPlan& plan);
+    void link_searched(Id node, const Neighbor* found, std::size_t count, UpperPlan* plan);
     std::vector<Neighbor> select_neighbors(std::vector<Neighbor> candidates, std::size_t m) const;
     template <Locking L>
     const Id* read_links(Id node, std::size_t layer, std::vector<Id>& copy,

@@ -90,19 +90,21 @@ VDX_API int vdx_host_graph_view(const vdx_host_graph* g, vdx_graph_view* out);
 VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construction, uint64_t* seed,
                           uint64_t* version);
 VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
-/* HnswIndex::add (src/hnsw.cpp:247-291) with the layer-0 insertion searches of level-0 nodes
- * run in batches by a searcher over its own copy of the graph (no counterpart in vecdex, which
- * searches node by node). Per batch: the batch's nodes with upper layers are inserted on the
- * host, sync() brings the searcher's copy up to the graph (`changed`: ascending ids of the nodes
- * stored or rewritten since the last sync), search() returns each level-0 node's search_layer
- * results at breadth ef = max(ef_construction, M) after the greedy descent (src/hnsw.cpp:315-335),
- * any order, and each node is linked as insert links it. Nodes of one batch are not candidates
- * of each other. Callbacks run on the calling thread and return a vdx status. */
+/* HnswIndex::add (src/hnsw.cpp:247-291) with the layer-0 insertion searches run in batches by a
+ * searcher over its own copy of the graph (no counterpart in vecdex, which searches node by
+ * node). Per batch: the upper-layer part of each insert runs on the host (descent; layer
+ * searches and choices for nodes with upper layers), sync() brings the searcher's copy up to the
+ * graph (`changed`: ascending ids of the nodes stored or rewritten since the last sync), search()
+ * returns each node's layer-0 search_layer results at breadth ef = max(ef_construction, M), any
+ * order, starting from entries[i] (its layer-1 result, src/hnsw.cpp:324-335) or, where
+ * entries[i] is UINT32_MAX, from the greedy descent (src/hnsw.cpp:315-323), and each node is
+ * linked as insert links it. Nodes of one batch are not candidates of each other. Callbacks run
+ * on the calling thread and return a vdx status. */
 typedef struct vdx_batch_searcher {
     void* ctx;
     int (*sync)(void* ctx, const vdx_graph_view* graph, const uint32_t* changed, uint64_t count);
-    int (*search)(void* ctx, const uint32_t* nodes, uint64_t count, uint32_t ef, vdx_neighbor* out,
-                  uint32_t* counts);
+    int (*search)(void* ctx, const uint32_t* nodes, const uint32_t* entries, uint64_t count, uint32_t ef,
+                  vdx_neighbor* out, uint32_t* counts);
 } vdx_batch_searcher;
 /* Batch sizes: the first host_prefix nodes of the graph are inserted on the host; then batches
  * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 1024,

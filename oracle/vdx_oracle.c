@@ -421,6 +421,7 @@ typedef struct {
     const vdxo_graph* g;
     uint64_t* upper_offset;
     const float* queries;
+    const uint32_t* entries; /* per query: a layer-0 entry (no descent), or UINT32_MAX; may be NULL */
     uint64_t k, ef;
     vdxo_neighbor* out;
     uint64_t out_stride;
@@ -456,8 +457,16 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
     vdxo_neighbor entry = {metric_dist(g->metric, c->l2, c->ip, query,
                                        g->vectors + g->entry_point * dim, dim),
                            (uint32_t)g->entry_point};
+    /* A given layer-0 entry replaces the descent: insert's layer-0 search starts from the best
+     * result of its layer-1 search (src/hnsw.cpp:324-335). */
+    uint64_t top = g->max_level;
+    if (c->entries != NULL && c->entries[q] != UINT32_MAX) {
+        entry.id = c->entries[q];
+        entry.distance = metric_dist(g->metric, c->l2, c->ip, query, g->vectors + (uint64_t)entry.id * dim, dim);
+        top = 0;
+    }
     /* greedy descent: move to any strictly closer neighbor, in link order (src/hnsw.cpp:416-431) */
-    for (uint64_t layer = g->max_level; layer > 0; --layer) {
+    for (uint64_t layer = top; layer > 0; --layer) {
         for (int moved = 1; moved;) {
             moved = 0;
             const uint32_t* b = links_of(c, entry.id, layer);
@@ -533,12 +542,13 @@ static void hnsw_cleanup(void* local) {
     free(s->fresh);
 }
 
-int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
-                           uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
-                           uint32_t* out_count, int threads, int use_fast_kernel) {
+int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const uint32_t* entries,
+                                uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
+                                uint32_t* out_count, int threads, int use_fast_kernel) {
     hnsw_ctx c;
     c.g = g;
     c.queries = queries;
+    c.entries = entries;
     c.k = k;
     c.ef = ef;
     c.out = out;
@@ -557,6 +567,13 @@ int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t n
     run_parallel(nq, threads, hnsw_worker_entry, &c, sizeof(hnsw_local), hnsw_cleanup);
     free(c.upper_offset);
     return 0;
+}
+
+int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
+                           uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
+                           uint32_t* out_count, int threads, int use_fast_kernel) {
+    return vdxo_hnsw_search_batch_from(g, queries, NULL, nq, k, ef, out, out_stride, out_count, threads,
+                                       use_fast_kernel);
 }
 
 /* ---------------------------------------------------------------- XXH64 */

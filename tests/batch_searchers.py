@@ -50,6 +50,7 @@ class OracleSearcher:
         self.graph = graph
         self.syncs = 0
         self.batches = []
+        self.entries = []
         self.changed = []
 
         def sync(ctx, view, changed, count):
@@ -59,13 +60,15 @@ class OracleSearcher:
             self.syncs += 1
             return 0
 
-        def search(ctx, nodes, count, ef, out, counts):
+        def search(ctx, nodes, entries, count, ef, out, counts):
             if fail_search:
                 return fail_search
             ids = np.ctypeslib.as_array(nodes, (count,)).copy()
+            ent = np.ctypeslib.as_array(entries, (count,)).copy()
             self.batches.append(ids)
+            self.entries.append(ent)
             g = self.graph
-            found, cnt = o.hnsw_search(g, g["vectors"][ids], ef, ef)
+            found, cnt = o.hnsw_search(g, g["vectors"][ids], ef, ef, entries=ent)
             ctypes.memmove(out, found.ctypes.data, found.nbytes)
             ctypes.memmove(counts, cnt.ctypes.data, cnt.nbytes)
             return 0

@@ -36,9 +36,12 @@ def test_batch_of_one_builds_the_serial_graph():
         assert np.array_equal(a[key], b[key]), key
     assert fingerprint(b, queries, 1, 1) == GREEDY_FP
     assert fingerprint(b, queries, 10, 10) == TOP10_FP
-    # every level-0 node after the first was searched once, in id order
+    # every node after the first was searched once, in id order; nodes with upper layers from
+    # the layer-1 result their host-side part found, the others after the descent
     searched = np.concatenate(s.batches)
-    assert np.array_equal(searched, np.nonzero(a["levels"] == 0)[0][np.nonzero(a["levels"] == 0)[0] > 0])
+    assert np.array_equal(searched, np.arange(1, 1000))
+    entries = np.concatenate(s.entries)
+    assert np.array_equal(entries != 0xFFFFFFFF, a["levels"][1:] > 0)
     # the first sync lists every stored node; later ones what the previous batch wrote
     assert np.array_equal(s.changed[0], np.arange(1000, dtype=np.uint32))
 
