@@ -116,7 +116,9 @@ typedef struct vdx_build_plan {
 } vdx_build_plan;
 /* Where a batched add spent its time (seconds, wall clock) and how it was split. */
 typedef struct vdx_build_stats {
-    uint64_t batches, searched, host_inserted;  /* batches; nodes searched by the searcher; on the host */
+    uint64_t batches, searched, host_inserted;  /* batches; nodes whose layer-0 search the searcher ran;
+                                                   nodes inserted (the prefix) or with upper layers
+                                                   searched on the host */
     double host_insert_s, sync_s, search_s, link_s, total_s;
 } vdx_build_stats;
 /* `synced_version`: the graph version the searcher's copy reflects (vdx_host_graph_params);
