@@ -25,7 +25,7 @@ evaluation recall is >= 0.95, else the fixed-ef QPS.
 roofline: algorithmic bytes of the headline beam launch (query, link blocks read, vectors of
 evaluated neighbours, results, from the kernel's counters in a counting pass with an 8K-slot
 visited cache) over its measured launch time, against 8 TB/s. cpu_baseline: the oracle's C
-restatement of vecdex's search_batch (AVX2 order) on this box's cores, fixed-ef operating
+restatement of vecdex's search_batch (AVX2 order) on this machine's cores, fixed-ef operating
 point, vecdex's protocol (one warmup, median of 5).
 """
 
@@ -202,11 +202,11 @@ def main(argv=None) -> int:
 
     threads = host_threads()
     t = time.perf_counter()
-    mix = datasets.c1_mixture()
+    mix = datasets.gmm1024()
     base = mix.sample(args.n, 1)
     queries = mix.sample(args.nq, 2)
     log(f"generated {args.n} x 128 base and {args.nq} queries in {time.perf_counter() - t:.1f} s")
-    index, build_s = build_or_load_graph(vdx, "c1", base, vdx.Metric.L2, threads)
+    index, build_s = build_or_load_graph(vdx, "gmm1m", base, vdx.Metric.L2, threads)
 
     # Ground truth (and the brute-force leg): FP32 vs BF16 MFMA shortlist, identical ids.
     t = time.perf_counter()
@@ -381,7 +381,7 @@ def cosine_setup(args, vdx, datasets, threads):
     if not _COS:
         gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
         base = gen.sample(args.n_cos, 1)
-        index, _ = build_or_load_graph(vdx, "c2b", base, vdx.Metric.Cosine, threads)
+        index, _ = build_or_load_graph(vdx, "aniso1200k_cos", base, vdx.Metric.Cosine, threads)
         _COS.update(gen=gen, base=base, index=index)
     return _COS["gen"], _COS["base"], _COS["index"]
 
@@ -461,7 +461,7 @@ def run_10m(args, vdx, nat, datasets, threads):
     queries = mix.sample(10_000, 2)
     log(f"10M: generated {len(base):,} x 128 in {time.perf_counter() - t:.1f} s")
     CACHE.mkdir(parents=True, exist_ok=True)
-    path = CACHE / f"c3_n{len(base)}_M32_efc200_seed0.vdx"
+    path = CACHE / f"gmm4096_n{len(base)}_M32_efc200_seed0.vdx"
     build_s = None
     if path.exists():
         index = vdx.HnswIndex.load(path)

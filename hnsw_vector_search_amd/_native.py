@@ -17,8 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libvdxgpu.so"
 
 if not LIB_PATH.exists():
     raise ImportError(
-        f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
-        "or `python -m hnsw_vector_search_amd.build` first")
+        f"{LIB_PATH} is not built; run `python -m hnsw_vector_search_amd.build` first")
 
 lib = ctypes.CDLL(str(LIB_PATH))
 

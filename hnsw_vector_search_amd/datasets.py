@@ -53,12 +53,12 @@ class GaussianMixture:
         return out
 
 
-def c0_mixture() -> GaussianMixture:
+def gmm64() -> GaussianMixture:
     """10,000 x 128 L2: 64 components, centers U[0,100], sigma 10."""
     return GaussianMixture(128, 64, 0.0, 100.0, 10.0, 10.0, mixture_seed=0)
 
 
-def c1_mixture(components: int = 1024) -> GaussianMixture:
+def gmm1024(components: int = 1024) -> GaussianMixture:
     """1,000,000 x 128 L2: 1,024 components, centers U[0,128], per-dim sigma log-uniform [4,32]."""
     return GaussianMixture(128, components, 0.0, 128.0, 4.0, 32.0, mixture_seed=0)
 

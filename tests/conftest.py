@@ -1,7 +1,7 @@
 """Test setup: the `gpu` marker, the repository on sys.path, and built native libraries.
 
-Tests load what is built and build only when nothing is (the GPU box receives prebuilt
-libraries with the tree)."""
+Tests load what is built and build only when nothing is (a machine that runs the GPU tests
+may receive the libraries prebuilt, with no compiler)."""
 
 import sys
 from pathlib import Path

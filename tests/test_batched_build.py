@@ -48,7 +48,7 @@ def test_batch_of_one_builds_the_serial_graph():
 
 @pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
 def test_batches_build_a_valid_graph_with_the_host_build_recall(metric):
-    mix = datasets.c0_mixture()
+    mix = datasets.gmm64()
     base, queries = mix.sample(12_000, 1) + 1, mix.sample(200, 2) + 1
     host = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
     host.add(base, num_threads=4)

@@ -60,7 +60,7 @@ def test_exact_mode_is_bit_equal_to_cpu(metric, dim, k):
 
 
 def test_exact_mode_matches_float64_ids_on_integer_data():
-    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 10_000, 100)
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 10_000, 100)
     index = vdx.BruteForceIndex(128)
     index.add(base)
     got, _ = index.search_arrays(queries, 10, "exact")
@@ -109,7 +109,7 @@ def test_mfma_modes_equal_exact_with_sampled_threshold(precision, metric):
     """n = 60,000: the threshold pass, the filter and the certification all run."""
     print(f"case: bf {precision} metric {metric} n 60000", flush=True)
     if metric == 0:
-        base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 60_000, 300)
+        base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 60_000, 300)
         dim = 128
     else:
         gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
@@ -129,7 +129,7 @@ def test_mfma_modes_equal_exact_with_sampled_threshold(precision, metric):
 
 def test_bf16_is_exact_on_small_integers():
     """Integer data in [0, 255] is exact in BF16, so the GEMM distances are exact."""
-    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 20_000, 100)
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 20_000, 100)
     index = vdx.BruteForceIndex(128)
     index.add(base)
     got, _ = index.search_arrays(queries, 10, "bf16")

@@ -47,7 +47,7 @@ def test_batch_of_one_on_the_gpu_builds_the_serial_graph():
 @pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
 def test_gpu_batches_equal_oracle_batches(metric):
     print(f"case: GPU build vs oracle-driven batched build, metric {metric}, 12000 x 128", flush=True)
-    mix = datasets.c0_mixture()
+    mix = datasets.gmm64()
     base = mix.sample(12_000, 1) + 1
     plan = nat.BuildPlan(1000, 128, 2048, 0.05)
     ref = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
@@ -59,7 +59,7 @@ def test_gpu_batches_equal_oracle_batches(metric):
 
 def test_threaded_gpu_build_recall_and_device_copy():
     print("case: threaded GPU build, 60000 x 128, M=16, efC=200", flush=True)
-    mix = datasets.c1_mixture()
+    mix = datasets.gmm1024()
     base, queries = mix.sample(60_000, 1), mix.sample(300, 2)
     host = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=200, seed=0)
     host.add(base, num_threads=8)

@@ -42,7 +42,7 @@ def test_device_load_serves_the_saved_graph(tmp_path, dim, metric):
 
 def test_multi_chunk_load_and_batches(tmp_path):
     print("case: v2 device load across pinned chunks, 150000 x 128", flush=True)
-    mix = datasets.c1_mixture()
+    mix = datasets.gmm1024()
     index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=100, seed=0)
     index.add(mix.sample(150_000, 1), num_threads=16)
     path = tmp_path / "big.vdx2"

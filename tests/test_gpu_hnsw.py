@@ -49,23 +49,23 @@ def test_gpu_reproduces_the_golden_fingerprints():
 
 
 @pytest.fixture(scope="module")
-def c0():
+def gmm10k():
     """10,000 x 128 integer GMM, M=16, efConstruction=200, seed 0 (the CPU-runnable config)."""
-    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 10_000, 100)
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 10_000, 100)
     index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=200, seed=0)
     index.add(base, num_threads=8)
     return index, base, queries
 
 
 @pytest.mark.parametrize("ef", [10, 16, 64, 128, 512])
-def test_c0_identical_to_cpu_search(c0, ef):
-    print(f"case: c0 10k x 128, ef {ef}", flush=True)
-    index, _, queries = c0
+def test_gmm10k_identical_to_cpu_search(gmm10k, ef):
+    print(f"case: gmm 10k x 128, ef {ef}", flush=True)
+    index, _, queries = gmm10k
     assert_identical(index, queries, 10, ef, 0)
 
 
-def test_c0_recall_against_float64_brute_force(c0):
-    index, base, queries = c0
+def test_gmm10k_recall_against_float64_brute_force(gmm10k):
+    index, base, queries = gmm10k
     truth, _ = o.bruteforce_f64(base, queries, 10)
     got, _ = gpu_search(index, queries, 10, 64)
     assert o.recall_at_k(got["id"], truth, 10) >= 0.95
@@ -105,9 +105,9 @@ def test_exact_when_ef_covers_every_node():
     np.testing.assert_array_equal(got[:, :10], exact)
 
 
-def test_mixed_k_and_ef_in_one_launch(c0):
+def test_mixed_k_and_ef_in_one_launch(gmm10k):
     print("case: mixed k/ef", flush=True)
-    index, _, queries = c0
+    index, _, queries = gmm10k
     rng = np.random.default_rng(7)
     ks = rng.choice([1, 10, 100], len(queries)).astype(np.uint32)
     efs = rng.choice([16, 32, 64, 128, 256, 512], len(queries)).astype(np.uint32)
@@ -119,16 +119,16 @@ def test_mixed_k_and_ef_in_one_launch(c0):
         assert got[q, :cnt[q]].tobytes() == ref[0, :ref_cnt[0]].tobytes()
 
 
-def test_results_are_deterministic(c0):
-    index, _, queries = c0
+def test_results_are_deterministic(gmm10k):
+    index, _, queries = gmm10k
     a, _ = gpu_search(index, queries, 10, 64)
     b, _ = gpu_search(index, queries, 10, 64)
     assert a.tobytes() == b.tobytes()
 
 
-def test_device_entry_point_matches_host_entry_point(c0):
+def test_device_entry_point_matches_host_entry_point(gmm10k):
     import ctypes
-    index, _, queries = c0
+    index, _, queries = gmm10k
     host, _ = gpu_search(index, queries, 10, 64)
     handle = index.gpu_handle()
     ctx = nat.Context.get(0)

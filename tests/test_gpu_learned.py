@@ -41,8 +41,8 @@ def numpy_logits(params, features):
 
 
 @pytest.fixture(scope="module")
-def c0():
-    base, queries = datasets.gmm_base_and_queries(datasets.c0_mixture(), 10_000, 2000)
+def gmm10k():
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 10_000, 2000)
     index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=200, seed=0)
     index.add(base, num_threads=8)
     bf = vdx.BruteForceIndex(128)
@@ -62,8 +62,8 @@ def test_inference_matches_numpy_bf16_forward():
     np.testing.assert_allclose(got, ref, rtol=2e-3, atol=2e-3)
 
 
-def test_trace_agrees_with_plain_search(c0):
-    index, queries, truth = c0
+def test_trace_agrees_with_plain_search(gmm10k):
+    index, queries, truth = gmm10k
     q = queries[:500]
     f, labels, steps, final = learned.collect_traces(index, q, truth[:500], k=10, ef=64, max_steps=256)
     # plain search: same expansions, same result
@@ -84,8 +84,8 @@ def test_trace_agrees_with_plain_search(c0):
     assert 0.05 < labels.mean() < 0.95
 
 
-def test_degenerate_models_bracket_fixed_ef(c0):
-    index, queries, truth = c0
+def test_degenerate_models_bracket_fixed_ef(gmm10k):
+    index, queries, truth = gmm10k
     nq = 500
     ctx = nat.Context.get(0)
     dq = nat.DeviceArray.from_numpy(queries[:nq], ctx)
@@ -106,8 +106,8 @@ def test_degenerate_models_bracket_fixed_ef(c0):
     assert stats[4] == nq and stats[0] == 12 * nq
 
 
-def test_training_and_calibrated_learned_search(c0):
-    index, queries, truth = c0
+def test_training_and_calibrated_learned_search(gmm10k):
+    index, queries, truth = gmm10k
     train_q, eval_q = queries[:1500], queries[1500:]
     f, labels, steps, _ = learned.collect_traces(index, train_q, truth[:1500], k=10, ef=128)
     model, losses = learned.train_stop_model(f, labels, epochs=6, batch=4096, lr=1e-3)
