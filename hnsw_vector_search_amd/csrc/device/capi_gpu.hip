@@ -23,6 +23,7 @@
 #include "fill_copy.hpp"
 #include "gpu_index.hpp"
 #include "gpu_state.hpp"
+#include "select_links.hpp"
 
 using vdx::guarded;
 using vdx::require;
@@ -295,6 +296,7 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
 struct BuildSearch {
     vdx_gpu_hnsw* h = nullptr;
     DeviceBuffer ids, queries, out, count, queue;  // ids: the nodes, then their layer-0 entries
+    DeviceBuffer chosen, chosen_count;             // the neighbour choice (m per node)
     vdx::gpu::PinnedBuffer h_ids, h_out, h_count;
 };
 
@@ -303,7 +305,7 @@ int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* chang
 }
 
 int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* entries, std::uint64_t count,
-                 std::uint32_t ef, vdx_neighbor* out, std::uint32_t* counts) {
+                 std::uint32_t ef, std::uint32_t m, vdx_neighbor* out, std::uint32_t* counts) {
     return guarded([&] {
         BuildSearch& b = *static_cast<BuildSearch*>(ctx);
         vdx_gpu_hnsw& h = *b.h;
@@ -323,7 +325,9 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
         // sizes with headroom: batches grow
         const auto grow = [](std::uint64_t bytes) { return bytes + bytes / 2; };
         b.h_ids.reserve(grow(count * 8));
-        b.h_out.reserve(grow(count * ef * sizeof(vdx_neighbor)));
+        require(m <= kMaxGpuM, "HnswIndex::add: M above the GPU path's limit");
+        const std::uint64_t stride = m > 0 ? m : ef;
+        b.h_out.reserve(grow(count * stride * sizeof(vdx_neighbor)));
         b.h_count.reserve(grow(count * 4));
         b.ids.reserve(grow(count * 8));
         b.queries.reserve(grow(count * h.dim * 4));
@@ -344,13 +348,36 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
         enqueue_beam(h, b.queries.as<float>(), count, nullptr, ef, nullptr, ef, ef, b.out.as<vdx::dev::Neighbor>(), ef,
                      b.count.as<std::uint32_t>(), b.queue.as<std::uint32_t>(), nullptr, nullptr, s, nullptr,
                      entries != nullptr ? b.ids.as<std::uint32_t>() + count : nullptr);
-        check(hipMemcpyAsync(b.h_out.as<void>(), b.out.as<void>(), count * ef * sizeof(vdx_neighbor),
-                             hipMemcpyDeviceToHost, s),
+        const void* d_res = b.out.as<void>();
+        const void* d_cnt = b.count.as<void>();
+        if (m > 0) {
+            b.chosen.reserve(grow(count * m * sizeof(vdx_neighbor)));
+            b.chosen_count.reserve(grow(count * 4));
+            vdx::dev::SelectArgs sa{};
+            sa.vectors = h.vectors.as<float>();
+            sa.ld = h.split.ld;
+            sa.l2 = h.metric == 0 ? 1u : 0u;
+            sa.nb = h.split.nb;
+            sa.n8 = h.split.n8;
+            sa.tail = h.split.tail;
+            sa.first_tail = h.split.first_tail;
+            sa.found = b.out.as<vdx::dev::Neighbor>();
+            sa.found_stride = ef;
+            sa.found_count = b.count.as<std::uint32_t>();
+            sa.m = m;
+            sa.count = count;
+            sa.chosen = b.chosen.as<vdx::dev::Neighbor>();
+            sa.chosen_count = b.chosen_count.as<std::uint32_t>();
+            check(vdx::dev::launch_select_links(sa, vdx::gpu::beam_shape(h.dim), s), "select_links launch");
+            d_res = b.chosen.as<void>();
+            d_cnt = b.chosen_count.as<void>();
+        }
+        check(hipMemcpyAsync(b.h_out.as<void>(), d_res, count * stride * sizeof(vdx_neighbor), hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(batch results)");
-        check(hipMemcpyAsync(b.h_count.as<void>(), b.count.as<void>(), count * 4, hipMemcpyDeviceToHost, s),
+        check(hipMemcpyAsync(b.h_count.as<void>(), d_cnt, count * 4, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(batch counts)");
         check(hipStreamSynchronize(s), "batch search");
-        std::memcpy(out, b.h_out.as<void>(), count * ef * sizeof(vdx_neighbor));
+        std::memcpy(out, b.h_out.as<void>(), count * stride * sizeof(vdx_neighbor));
         std::memcpy(counts, b.h_count.as<void>(), count * 4);
     });
 }
@@ -569,7 +596,7 @@ int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const flo
         require(index != nullptr, "vdx_gpu_hnsw_build_add: index is null");
         BuildSearch b;
         b.h = index;
-        const vdx_batch_searcher searcher{&b, &build_sync, &build_search};
+        const vdx_batch_searcher searcher{&b, &build_sync, &build_search, 1u};
         vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version, stats);
     });
 }

@@ -53,20 +53,22 @@ public:
         fill_view(g, &view);
         check(hooks_.sync(hooks_.ctx, &view, changed.data(), changed.size()), "sync");
     }
-    void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef,
+    void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef, std::size_t m,
                 std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) override {
-        out.resize(nodes.size() * ef);
+        const std::size_t stride = m > 0 ? m : ef;
+        out.resize(nodes.size() * stride);
         count.assign(nodes.size(), 0u);
         static_assert(sizeof(Neighbor) == sizeof(vdx_neighbor));
         check(hooks_.search(hooks_.ctx, nodes.data(), entries.data(), nodes.size(), static_cast<std::uint32_t>(ef),
-                            reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
+                            static_cast<std::uint32_t>(m), reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
               "search");
         for (const std::uint32_t c : count) {
-            if (c > ef) {
-                throw std::runtime_error("HnswIndex::add: the batch searcher returned more than ef results");
+            if (c > stride) {
+                throw std::runtime_error("HnswIndex::add: the batch searcher returned more results than asked for");
             }
         }
     }
+    bool selects() const override { return hooks_.selects != 0; }
 
 private:
     void check(int rc, const char* which) const {

@@ -276,6 +276,8 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
         st.sync_s += since();
     };
     const std::size_t ef = std::max(params_.ef_construction, params_.M);
+    const std::size_t m = searcher.selects() ? params_.M : 0;  // the searcher makes the choice
+    const std::size_t stride = m > 0 ? m : ef;
     std::vector<Id> nodes, entries, high;
     std::vector<std::size_t> high_at;
     std::vector<UpperPlan> plans;
@@ -309,7 +311,7 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
         }
         st.host_insert_s += since();
         sync();
-        searcher.search(nodes, entries, ef, found, count);
+        searcher.search(nodes, entries, ef, m, found, count);
         st.search_s += since();
         st.searched += nodes.size();
         std::vector<UpperPlan*> plan_of(nodes.size(), nullptr);
@@ -317,8 +319,8 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
             plan_of[high_at[i]] = &plans[i];
         }
         parallel_for(nodes.size(), std::min(threads, nodes.size()),
-                     [this, &nodes, &found, &count, &plan_of, ef](std::size_t i) {
-                         link_searched(nodes[i], found.data() + i * ef, count[i], plan_of[i]);
+                     [this, &nodes, &found, &count, &plan_of, stride, m](std::size_t i) {
+                         link_searched(nodes[i], found.data() + i * stride, count[i], m > 0, plan_of[i]);
                      });
         st.link_s += since();
         ++st.batches;
@@ -352,10 +354,12 @@ void HnswGraph::plan_upper(Id node, UpperPlan& plan) {
     plan.entry0 = entry.id;
 }
 
-// The linking half of insert for a node whose layer-0 search results are `found` (and, for a
-// node with upper layers, whose upper choices are in `plan`).
-void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count, UpperPlan* plan) {
-    std::vector<Neighbor> chosen0 = select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
+// The linking half of insert for a node whose layer-0 search results are `found` (or, with
+// `chosen`, already its layer-0 choice) and, for a node with upper layers, whose upper choices
+// are in `plan`.
+void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count, bool chosen, UpperPlan* plan) {
+    std::vector<Neighbor> chosen0 = chosen ? std::vector<Neighbor>(found, found + count)
+                                           : select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
     const std::size_t top = plan != nullptr ? plan->chosen.size() - 1 : 0;
     {
         const std::lock_guard<std::mutex> lock(build_locks_->links[node]);

@@ -56,9 +56,12 @@ public:
     // out[i * ef ..] receives the count[i] results of nodes[i], in any order. entries[i] is a
     // layer-0 entry that replaces the descent (insert's layer-0 search starts from its layer-1
     // result, src/hnsw.cpp:324-335), or kDescend.
+    // With m > 0 (only when selects()), out[i * m ..] receives select_neighbors(results, m)
+    // instead: the neighbour choice made where the results are.
     static constexpr Id kDescend = 0xFFFFFFFFu;
     virtual void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef,
-                        std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) = 0;
+                        std::size_t m, std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) = 0;
+    virtual bool selects() const { return false; }
 };
 
 // How add_batched splits the rows: the first `host_prefix` nodes of the graph are inserted on
@@ -155,7 +158,7 @@ private:
         Id entry0 = 0;                              // where the layer-0 search starts
     };
     void plan_upper(Id node, UpperPlan& plan);
-    void link_searched(Id node, const Neighbor* found, std::size_t count, UpperPlan* plan);
+    void link_searched(Id node, const Neighbor* found, std::size_t count, bool chosen, UpperPlan* plan);
     std::vector<Neighbor> select_neighbors(std::vector<Neighbor> candidates, std::size_t m) const;
     template <Locking L>
     const Id* read_links(Id node, std::size_t layer, std::vector<Id>& copy,

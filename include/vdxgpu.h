@@ -103,8 +103,12 @@ VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
 typedef struct vdx_batch_searcher {
     void* ctx;
     int (*sync)(void* ctx, const vdx_graph_view* graph, const uint32_t* changed, uint64_t count);
+    /* m == 0: out[i * ef ..] receives node i's results. m > 0 (only if `selects`): out[i * m ..]
+     * receives insert's neighbour choice among them (select_neighbors(results, m),
+     * src/hnsw.cpp:350-409 heuristic, closest first). */
     int (*search)(void* ctx, const uint32_t* nodes, const uint32_t* entries, uint64_t count, uint32_t ef,
-                  vdx_neighbor* out, uint32_t* counts);
+                  uint32_t m, vdx_neighbor* out, uint32_t* counts);
+    uint32_t selects; /* nonzero: search() also makes the neighbour choice */
 } vdx_batch_searcher;
 /* Batch sizes: the first host_prefix nodes of the graph are inserted on the host; then batches
  * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 1024,

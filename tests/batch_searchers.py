@@ -60,7 +60,9 @@ class OracleSearcher:
             self.syncs += 1
             return 0
 
-        def search(ctx, nodes, entries, count, ef, out, counts):
+        def search(ctx, nodes, entries, count, ef, m, out, counts):
+            if m != 0:
+                return 2  # this searcher returns results; the graph makes the choice
             if fail_search:
                 return fail_search
             ids = np.ctypeslib.as_array(nodes, (count,)).copy()
@@ -74,7 +76,7 @@ class OracleSearcher:
             return 0
 
         self._keep = (nat.SYNC_FN(sync), nat.SEARCH_FN(search))
-        self.c = nat.BatchSearcher(None, *self._keep)
+        self.c = nat.BatchSearcher(None, *self._keep, 0)
 
 
 def add_batched(index, data, threads, plan, searcher):
