@@ -44,15 +44,20 @@ def test_batch_of_one_on_the_gpu_builds_the_serial_graph():
     assert fingerprint(*gpu.search_arrays(queries, 10, 10)) == TOP10_FP
 
 
-@pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
-def test_gpu_batches_equal_oracle_batches(metric):
-    print(f"case: GPU build vs oracle-driven batched build, metric {metric}, 12000 x 128", flush=True)
-    mix = datasets.gmm64()
-    base = mix.sample(12_000, 1) + 1
+@pytest.mark.parametrize("dim,metric,M", [(128, vdx.Metric.L2, 12), (128, vdx.Metric.Cosine, 12),
+                                          (100, vdx.Metric.Cosine, 32), (37, vdx.Metric.InnerProduct, 8)])
+def test_gpu_batches_equal_oracle_batches(dim, metric, M):
+    """The GPU's searches and neighbour choices against the oracle's searches and the host's
+    choices: on one thread the two batched builds must agree node for node."""
+    print(f"case: GPU build vs oracle-driven batched build, {dim}-d metric {metric} M={M}, 12000 rows", flush=True)
+    if dim == 128:
+        base = datasets.gmm64().sample(12_000, 1) + 1
+    else:
+        base = datasets.AnisotropicGaussian(dim, 0.8, seed=4).sample(12_000, 1)
     plan = nat.BuildPlan(1000, 128, 2048, 0.05)
-    ref = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
+    ref = vdx.HnswIndex(dim, metric, M=M, ef_construction=100, seed=3)
     assert add_batched(ref, base, 1, plan, OracleSearcher()) == 0
-    gpu = vdx.HnswIndex(128, metric, M=12, ef_construction=100, seed=3)
+    gpu = vdx.HnswIndex(dim, metric, M=M, ef_construction=100, seed=3)
     gpu.add(base, 1, gpu_assisted=True, plan=plan)
     assert_same_graph(ref, gpu)
 
