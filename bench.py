@@ -208,38 +208,12 @@ def main(argv=None) -> int:
     log(f"generated {args.n} x 128 base and {args.nq} queries in {time.perf_counter() - t:.1f} s")
     index, build_s = build_or_load_graph(vdx, "gmm1m", base, vdx.Metric.L2, threads)
 
-    # Ground truth (and the brute-force leg): FP32 vs BF16 MFMA shortlist, identical ids.
+    # Ground truth (and the brute-force leg): exact scan, FP32 vs BF16 MFMA shortlist.
     t = time.perf_counter()
     truth, bf = ground_truth(vdx, base, vdx.Metric.L2, queries)
     gt_s = time.perf_counter() - t
-    bf_leg = {}
-    ctx = nat.Context.get(0)
-    dq = nat.DeviceArray.from_numpy(queries, ctx)
-    bout = nat.DeviceArray((args.nq, K), nat.NEIGHBOR_DTYPE, ctx)
-    bcnt = nat.DeviceArray(args.nq, np.uint32, ctx)
-    for prec in ("fp32", "bf16"):
-        st = nat.BfStats()
-        times = []
-        for _ in range(3):
-            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
-            t0 = time.perf_counter()
-            nat.check(nat.lib.vdx_gpu_bf_search_device(bf.gpu_handle(), dq.ptr, args.nq, K,
-                                                       nat.PRECISIONS[prec], bout.ptr, K, bcnt.ptr,
-                                                       ctypes.byref(st)))
-            times.append((time.perf_counter() - t0) * 1000)
-        res = bout.numpy()
-        ms = min(times)
-        flops = 2.0 * args.nq * args.n * 128
-        peak = FP32_MFMA_TFLOPS if prec == "fp32" else BF16_MFMA_TFLOPS
-        bf_leg[prec] = {"ms": round(ms, 2), "qps": round(args.nq / ms * 1000, 1),
-                        "identical_to_exact": bool(res.tobytes() == truth.tobytes()),
-                        "exact_fallbacks": int(st.exact_fallback), "shortlist": int(st.shortlist),
-                        "end_to_end_tflops": round(flops / ms / 1e9, 1),
-                        "frac_of_mfma_peak": round(flops / ms / 1e9 / peak, 4)}
-        log(f"brute force {prec}: {ms:.1f} ms per 10k queries, identical={bf_leg[prec]['identical_to_exact']}, "
-            f"fallbacks {st.exact_fallback}")
-    bf_leg["bf16_speedup_vs_fp32"] = round(bf_leg["fp32"]["ms"] / bf_leg["bf16"]["ms"], 2)
-    del bf, dq, bout, bcnt
+    bf_leg = bruteforce_leg(args, nat, bf, queries, vdx.Metric.L2, threads)
+    del bf
 
     # Fixed-ef sweep.
     run = Runner(nat, index, queries)
@@ -391,7 +365,11 @@ def run_serving(args, vdx, nat, datasets, threads):
     and k ~ U{1, 10, 100}. One launch sized for the largest (ef, k) against the class-batched
     path (eager, then tuned and replayed as a HIP graph), results compared byte for byte; then
     batch sizes 1 .. 65,536 through the tuned graph."""
-    gen, _, index = cosine_setup(args, vdx, datasets, threads)
+    gen, base, index = cosine_setup(args, vdx, datasets, threads)
+    bf = vdx.BruteForceIndex(base.shape[1], vdx.Metric.Cosine)
+    bf.add(base)
+    bf_cos = bruteforce_leg(args, nat, bf, prepared(nat, gen.sample(10_000, 9), True), vdx.Metric.Cosine, threads)
+    del bf
     nq = args.serving_nq
     rng = np.random.default_rng(5)
     efs = np.array([16, 32, 64, 128, 256, 512], dtype=np.uint32)[rng.integers(0, 6, nq)]
@@ -436,7 +414,7 @@ def run_serving(args, vdx, nat, datasets, threads):
         log(f"  class cap {c['ef_cap']}: {c['queries']} queries, visited 2^{c['visited_log2']}, "
             f"{c['waves_per_group']} waves/group, blocks/CU {c['blocks_per_cu'] or 'max'}, {c['ms']:.2f} ms")
     sizes = []
-    for b in sorted({*(4 ** i for i in range(9) if 4 ** i < nq), nq}):
+    for b in sorted({*(2 ** i for i in range(17) if 2 ** i < nq), nq}):
         r = 20 if b <= 1024 else 5
         ms = timed(lambda: batch.run(dq, b, out, cnt, ks=dks, efs=defs, graph=True), r)
         sizes.append({"batch": b, "ms": round(ms, 4), "qps": round(b / ms * 1000, 1)})
@@ -448,7 +426,7 @@ def run_serving(args, vdx, nat, datasets, threads):
             "tuned_graph_qps": round(nq / ms_graph * 1000, 1),
             "speedup_vs_one_launch": round(ms_single / ms_graph, 3),
             "identical_to_one_launch": identical, "tune_s": round(tune_s, 2),
-            "classes": classes, "batch_sizes": sizes}
+            "classes": classes, "batch_sizes": sizes, "bruteforce_cosine_1200k_x_100": bf_cos}
 
 
 def run_10m(args, vdx, nat, datasets, threads):
@@ -529,6 +507,59 @@ def run_10m(args, vdx, nat, datasets, threads):
             "qps_at_recall_0.95": at[0] if at else None,
             "serving": {"queries": nq, "one_launch_ms": round(ms_one, 2), "tuned_graph_ms": round(ms_graph, 2),
                         "tuned_graph_qps": round(nq / ms_graph * 1000, 1), "identical_to_one_launch": identical}}
+
+
+def bruteforce_leg(args, nat, bf, queries, metric, threads):
+    """BruteForceIndex::search on the GPU over all (prepared) queries at k=10: the exact scan (the
+    reference's result), then the FP32 and BF16 MFMA shortlists with exact re-rank, each checked
+    byte for byte against the exact scan; plus the host's exact scan (the oracle's restatement of
+    the reference, on all host threads) over a bounded sample of the queries."""
+    base = bf.vectors()  # as stored: normalized under cosine
+    nq, (n, d) = len(queries), base.shape
+    ctx = nat.Context.get(0)
+    dq = nat.DeviceArray.from_numpy(np.ascontiguousarray(queries, dtype=np.float32), ctx)
+    bout = nat.DeviceArray((nq, K), nat.NEIGHBOR_DTYPE, ctx)
+    bcnt = nat.DeviceArray(nq, np.uint32, ctx)
+    leg, exact = {}, None
+    flops = 2.0 * nq * n * d
+    for prec in ("exact", "fp32", "bf16"):
+        st = nat.BfStats()
+        times = []
+        for _ in range(1 if prec == "exact" else 3):
+            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+            t0 = time.perf_counter()
+            nat.check(nat.lib.vdx_gpu_bf_search_device(bf.gpu_handle(), dq.ptr, nq, K, nat.PRECISIONS[prec],
+                                                       bout.ptr, K, bcnt.ptr, ctypes.byref(st)))
+            times.append((time.perf_counter() - t0) * 1000)
+        res = bout.numpy()
+        ms = min(times)
+        if prec == "exact":
+            exact = res
+            leg["exact"] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1)}
+            log(f"brute force exact scan ({n:,} x {d}): {ms:.1f} ms per {nq:,} queries")
+            continue
+        peak = FP32_MFMA_TFLOPS if prec == "fp32" else BF16_MFMA_TFLOPS
+        leg[prec] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1),
+                     "identical_to_exact": bool(res.tobytes() == exact.tobytes()),
+                     "exact_fallbacks": int(st.exact_fallback), "shortlist": int(st.shortlist),
+                     "end_to_end_tflops": round(flops / ms / 1e9, 1),
+                     "frac_of_mfma_peak": round(flops / ms / 1e9 / peak, 4)}
+        log(f"brute force {prec}: {ms:.1f} ms per {nq:,} queries, identical to the exact scan "
+            f"{leg[prec]['identical_to_exact']}, fallbacks {st.exact_fallback}")
+    leg["bf16_speedup_vs_fp32"] = round(leg["fp32"]["ms"] / leg["bf16"]["ms"], 2)
+    if not args.no_cpu:
+        from oracle import oracle as o
+        sample = 200
+        t0 = time.perf_counter()
+        host = o.bruteforce_f32(base, np.ascontiguousarray(queries[:sample]), K, metric=metric, threads=threads)
+        secs = time.perf_counter() - t0
+        leg["cpu_exact"] = {"qps": round(sample / secs, 1), "cores": threads, "kind": "port",
+                            "sample": f"{sample} of the {nq:,} queries, oracle exact scan (FP32, AVX2 order)",
+                            "identical_ids": bool(np.array_equal(host["id"], exact["id"][:sample]))}
+        log(f"host exact scan: {sample / secs:,.0f} QPS on {threads} threads (sample of {sample}); GPU/host "
+            f"speed-up of the BF16 path {leg['bf16']['qps'] / (sample / secs):,.0f}x")
+    del dq, bout, bcnt
+    return leg
 
 
 def run_load(vdx, index, queries):
