@@ -450,7 +450,12 @@ def run_10m(args, vdx, nat, datasets, threads):
         how = "host" if args.host_build_10m else "GPU-assisted"
         for r0 in range(0, len(base), step):  # in slices, reporting progress
             index.add(base[r0:r0 + step], num_threads=threads, gpu_assisted=not args.host_build_10m)
-            log(f"10M: {min(r0 + step, len(base)):,} rows linked ({how}), {time.perf_counter() - t:.0f} s")
+            split = ""
+            if not args.host_build_10m:
+                st = index.last_build_stats
+                split = (f"; this slice: host {st['host_insert_s']:.1f} s, sync {st['sync_s']:.1f} s, "
+                         f"search {st['search_s']:.1f} s, link {st['link_s']:.1f} s")
+            log(f"10M: {min(r0 + step, len(base)):,} rows linked ({how}), {time.perf_counter() - t:.0f} s{split}")
         build_s = time.perf_counter() - t
         log(f"10M: built graph (M=32, efC=200, {how}) on {threads} threads in {build_s:.0f} s")
         if "VDX_BENCH_CACHE" in os.environ:  # an 8 GB file: kept only where asked for
