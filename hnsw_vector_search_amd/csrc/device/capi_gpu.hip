@@ -15,6 +15,7 @@
 #include "../host/batched_build.hpp"
 #include "../host/capi_common.hpp"
 #include "../host/format_v2.hpp"
+#include "../host/parallel_for.hpp"
 #include "../host/validate.hpp"
 #include "beam_args.hpp"
 #include "beam_host.hpp"
@@ -239,31 +240,29 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
                   "hipMemcpyAsync(upper)");
         }
     }
-    // When a quarter or more of the old nodes changed, their layer-0 section goes over as one
-    // contiguous copy (cheaper than gathering scattered blocks); upper blocks are always scattered.
-    const bool dense = !changed.empty() && changed.size() * 4 >= old_n;
-    if (dense) {
-        check(hipMemcpyAsync(h->layer0.as<void>(), g->layer0, old_n * b0 * 4, hipMemcpyHostToDevice, s),
-              "hipMemcpyAsync(layer0)");
-    }
     std::vector<std::uint32_t> ublocks;
     for (const std::uint32_t i : changed) {
         for (std::uint32_t l = 0; l < g->levels[i]; ++l) ublocks.push_back(h->offsets[i] + l);
     }
-    if (dense) {
-        changed.clear();
-    }
     const std::uint64_t words = changed.size() * (1 + b0) + ublocks.size() * (1 + bu);
     if (words > 0) {
         // The changed blocks with their ids, gathered on the host into pinned staging (headroom:
-        // the updates of a batched build grow), one copy over, scattered on the device.
+        // the updates of a batched build grow; the gather runs on several threads, since a
+        // batch of a large build rewrites millions of blocks), one copy over, scattered on the
+        // device. Copying only the changed blocks from pinned memory beats a whole-section copy
+        // from pageable memory even when most blocks changed.
         h->staging.reserve(words * 6);
         std::uint32_t* st = h->staging.as<std::uint32_t>();
         std::memcpy(st, changed.data(), changed.size() * 4);
         std::uint32_t* l0 = st + changed.size();
-        for (std::size_t r = 0; r < changed.size(); ++r) {
-            std::memcpy(l0 + r * b0, g->layer0 + static_cast<std::uint64_t>(changed[r]) * b0, b0 * 4);
-        }
+        constexpr std::size_t kRowsPerTask = 4096;
+        const std::size_t tasks = (changed.size() + kRowsPerTask - 1) / kRowsPerTask;
+        vdx::parallel_for(tasks, std::min<std::size_t>(16, vdx::resolve_threads(0)), [&](std::size_t t) {
+            const std::size_t end = std::min(changed.size(), (t + 1) * kRowsPerTask);
+            for (std::size_t r = t * kRowsPerTask; r < end; ++r) {
+                std::memcpy(l0 + r * b0, g->layer0 + static_cast<std::uint64_t>(changed[r]) * b0, b0 * 4);
+            }
+        });
         std::uint32_t* uids = l0 + changed.size() * b0;
         std::memcpy(uids, ublocks.data(), ublocks.size() * 4);
         std::uint32_t* ub = uids + ublocks.size();

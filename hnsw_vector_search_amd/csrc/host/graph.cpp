@@ -6,6 +6,7 @@
 #include "graph.hpp"
 
 #include <algorithm>
+#include <bit>
 #include <chrono>
 #include <cassert>
 #include <cmath>
@@ -70,9 +71,16 @@ private:
     std::vector<std::unique_ptr<Scratch>> free_;
 };
 
+// Locks of the threaded build: a node's link blocks are guarded by one of a fixed set of
+// striped mutexes (no code path holds two of them at once, so sharing a stripe only
+// serializes), and the entry point by its own.
 struct HnswGraph::BuildLocks {
-    explicit BuildLocks(std::size_t nodes) : links(new std::mutex[nodes]) {}
-    std::unique_ptr<std::mutex[]> links;
+    static constexpr std::size_t kStripes = 1u << 16;
+    explicit BuildLocks(std::size_t nodes) : stripes(std::min(kStripes, std::bit_ceil(std::max<std::size_t>(nodes, 1)))),
+                                             locks(new std::mutex[stripes]) {}
+    std::mutex& link(Id node) { return locks[node & (stripes - 1)]; }
+    std::size_t stripes;
+    std::unique_ptr<std::mutex[]> locks;
     std::mutex entry;
 };
 
@@ -362,7 +370,7 @@ void HnswGraph::link_searched(Id node, const Neighbor* found, std::size_t count,
                                            : select_neighbors(std::vector<Neighbor>(found, found + count), params_.M);
     const std::size_t top = plan != nullptr ? plan->chosen.size() - 1 : 0;
     {
-        const std::lock_guard<std::mutex> lock(build_locks_->links[node]);
+        const std::lock_guard<std::mutex> lock(build_locks_->link(node));
         touched_[node] = static_cast<std::uint32_t>(version_);
         for (std::size_t layer = 0; layer <= top; ++layer) {
             const std::vector<Neighbor>& chosen = layer == 0 ? chosen0 : plan->chosen[layer];
@@ -390,7 +398,7 @@ template <HnswGraph::Locking L>
 const Id* HnswGraph::read_links(Id node, std::size_t layer, std::vector<Id>& copy,
                                 std::size_t& count) const {
     if constexpr (L == Locking::PerNode) {
-        const std::lock_guard<std::mutex> lock(build_locks_->links[node]);
+        const std::lock_guard<std::mutex> lock(build_locks_->link(node));
         const Id* b = block(node, layer);
         copy.assign(b + 1, b + 1 + b[0]);
         count = copy.size();
@@ -501,7 +509,7 @@ void HnswGraph::insert(Id node) {
     {
         std::unique_lock<std::mutex> lock;
         if constexpr (L == Locking::PerNode) {
-            lock = std::unique_lock<std::mutex>(build_locks_->links[node]);
+            lock = std::unique_lock<std::mutex>(build_locks_->link(node));
         }
         touched_[node] = static_cast<std::uint32_t>(version_);
         for (std::size_t layer = 0; layer <= top; ++layer) {
@@ -526,7 +534,7 @@ void HnswGraph::link_back(Id node, std::size_t layer, const std::vector<Neighbor
     for (const Neighbor& neighbor : chosen) {
         std::unique_lock<std::mutex> lock;
         if constexpr (L == Locking::PerNode) {
-            lock = std::unique_lock<std::mutex>(build_locks_->links[neighbor.id]);
+            lock = std::unique_lock<std::mutex>(build_locks_->link(neighbor.id));
         }
         Id* b = block(neighbor.id, layer);
         touched_[neighbor.id] = static_cast<std::uint32_t>(version_);
