@@ -297,6 +297,16 @@ struct BuildSearch {
     DeviceBuffer ids, queries, out, count, queue;  // ids: the nodes, then their layer-0 entries
     DeviceBuffer chosen, chosen_count;             // the neighbour choice (m per node)
     vdx::gpu::PinnedBuffer h_ids, h_out, h_count;
+    ~BuildSearch() {
+        // a call that failed part-way may have left copies from or to these buffers queued
+        if (h != nullptr) {
+            int previous = 0;
+            const bool got = hipGetDevice(&previous) == hipSuccess;
+            (void)hipSetDevice(h->ctx->device);
+            (void)hipStreamSynchronize(h->ctx->stream);
+            if (got) (void)hipSetDevice(previous);
+        }
+    }
 };
 
 int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* changed, std::uint64_t count) {
