@@ -42,11 +42,12 @@ class GraphView(ctypes.Structure):
 
 SYNC_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(GraphView), ctypes.POINTER(c_u32), c_u64)
 SEARCH_FN = ctypes.CFUNCTYPE(c_int, c_ptr, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), c_u64, c_u32, c_u32,
-                             c_ptr, ctypes.POINTER(c_u32))
+                             c_u32, c_ptr, ctypes.POINTER(c_u32))
+SEARCHER_SELECTS, SEARCHER_UPPER_LAYERS = 1, 2
 
 
 class BatchSearcher(ctypes.Structure):
-    _fields_ = [("ctx", c_ptr), ("sync", SYNC_FN), ("search", SEARCH_FN), ("selects", c_u32)]
+    _fields_ = [("ctx", c_ptr), ("sync", SYNC_FN), ("search", SEARCH_FN), ("caps", c_u32)]
 
 
 class BuildPlan(ctypes.Structure):

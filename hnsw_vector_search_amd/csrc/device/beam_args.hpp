@@ -38,9 +38,12 @@ struct BeamArgs {
     std::uint32_t k_cap;  // per-query k is clamped to this (<= out_stride, <= ef_cap)
     // launch state
     std::uint32_t* queue;  // next query to take; zero before the launch
-    // optional per-query layer-0 entry: entries[q] < n starts the layer-0 search there with no
-    // descent (an insertion search, which starts from its layer-1 result); other values descend
+    // optional per-query entry: entries[q] < n starts the beam there with no descent (an
+    // insertion search, which starts from the layer above's result); other values descend
     const std::uint32_t* entries;
+    // the layer the beam searches (0 for queries; an insertion searches each of its layers):
+    // the descent stops above it and link blocks are the layer's (upper blocks for layer > 0)
+    std::uint32_t layer;
     // optional work list: the launch serves queries order[range[0] .. range[1]) (device
     // values, written by an earlier launch on the stream); null: queries 0 .. nq
     const std::uint32_t* order;

@@ -85,7 +85,8 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t max_efp, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
-                  const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr) {
+                  const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr,
+                  std::uint32_t layer = 0) {
     const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
     const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
     vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
@@ -99,6 +100,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
         a.trace_cap = trace->trace_cap;
     }
     a.entries = d_entries;
+    a.layer = layer;
     check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
     vdx::gpu::launch_beam(h, plan, mode, a, stream);
 }
@@ -314,7 +316,7 @@ int build_sync(void* ctx, const vdx_graph_view* view, const std::uint32_t* chang
 }
 
 int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* entries, std::uint64_t count,
-                 std::uint32_t ef, std::uint32_t m, vdx_neighbor* out, std::uint32_t* counts) {
+                 std::uint32_t layer, std::uint32_t ef, std::uint32_t m, vdx_neighbor* out, std::uint32_t* counts) {
     return guarded([&] {
         BuildSearch& b = *static_cast<BuildSearch*>(ctx);
         vdx_gpu_hnsw& h = *b.h;
@@ -325,9 +327,12 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
         require(count < 0x80000000ull, "HnswIndex::add: batch too large");
         for (std::uint64_t i = 0; i < count; ++i) {
             require(nodes[i] < h.n, "HnswIndex::add: a batch node is not in the device copy");
-            require(entries == nullptr || entries[i] < h.n || entries[i] == 0xFFFFFFFFu,
-                    "HnswIndex::add: a layer-0 entry is not in the device copy");
+            require(entries == nullptr || entries[i] == 0xFFFFFFFFu ||
+                        (entries[i] < h.n && h.levels[entries[i]] >= layer),
+                    "HnswIndex::add: an entry is not a node of the searched layer");
+            require(layer == 0 || h.levels[nodes[i]] >= layer, "HnswIndex::add: a node does not reach the searched layer");
         }
+        require(layer <= h.max_level, "HnswIndex::add: the searched layer is above the graph's top");
         if (count == 0) return;
         DeviceGuard guard(h.ctx->device);
         hipStream_t s = h.ctx->stream;
@@ -356,7 +361,7 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
               "gather(batch vectors)");
         enqueue_beam(h, b.queries.as<float>(), count, nullptr, ef, nullptr, ef, ef, b.out.as<vdx::dev::Neighbor>(), ef,
                      b.count.as<std::uint32_t>(), b.queue.as<std::uint32_t>(), nullptr, nullptr, s, nullptr,
-                     entries != nullptr ? b.ids.as<std::uint32_t>() + count : nullptr);
+                     entries != nullptr ? b.ids.as<std::uint32_t>() + count : nullptr, layer);
         const void* d_res = b.out.as<void>();
         const void* d_cnt = b.count.as<void>();
         if (m > 0) {
@@ -605,7 +610,8 @@ int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const flo
         require(index != nullptr, "vdx_gpu_hnsw_build_add: index is null");
         BuildSearch b;
         b.h = index;
-        const vdx_batch_searcher searcher{&b, &build_sync, &build_search, 1u};
+        const vdx_batch_searcher searcher{&b, &build_sync, &build_search,
+                                          VDX_SEARCHER_SELECTS | VDX_SEARCHER_UPPER_LAYERS};
         vdx::host_graph_add_batched(graph, data, rows, dim, num_threads, searcher, true, plan, synced_version, stats);
     });
 }

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         wave_lds_fence();
         float cur_d = fd[0];
         ++n_dist;
-        for (std::uint32_t layer = top; layer > 0; --layer) {
+        for (std::uint32_t layer = top; layer > a.layer; --layer) {
             for (;;) {
                 const std::uint32_t* blk =
                     a.upper + (static_cast<std::size_t>(a.upper_offset[cur]) + (layer - 1)) * a.bu;
@@ -334,7 +334,10 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
             hint = static_cast<std::uint32_t>(p) + 1;
             ++n_exp;
 
-            const std::uint32_t* blk = a.layer0 + static_cast<std::size_t>(node) * a.b0;
+            // the searched layer's block (layer 0 for queries; an insertion may search an upper one)
+            const std::uint32_t* blk =
+                a.layer == 0 ? a.layer0 + static_cast<std::size_t>(node) * a.b0
+                             : a.upper + (static_cast<std::size_t>(a.upper_offset[node]) + (a.layer - 1)) * a.bu;
             const std::uint32_t cnt = uniform(blk[0]);
             const bool valid = static_cast<std::uint32_t>(lane) < cnt;
             const std::uint32_t id = valid ? blk[1 + lane] : 0;

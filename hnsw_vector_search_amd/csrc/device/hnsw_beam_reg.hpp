@@ -195,7 +195,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         wave_lds_fence();
         float cur_d = fd[0];
         ++n_dist;
-        for (std::uint32_t layer = top; layer > 0; --layer) {
+        for (std::uint32_t layer = top; layer > a.layer; --layer) {
             for (;;) {
                 const std::uint32_t* blk =
                     a.upper + (static_cast<std::size_t>(a.upper_offset[cur]) + (layer - 1)) * a.bu;
@@ -226,6 +226,16 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 cur_d = best;
             }
         }
+
+        // The searched layer's link block of a node (layer 0 for queries); an insertion may
+        // search an upper layer, whose blocks hold M links.
+        const std::uint32_t entry_node = cur;
+        const std::uint32_t lcap = a.layer == 0 ? a.M2 : (a.M2 >> 1);
+        const auto block_of = [&](std::uint32_t nd) -> const std::uint32_t* {
+            return a.layer == 0
+                       ? a.layer0 + static_cast<std::size_t>(nd) * b0
+                       : a.upper + (static_cast<std::size_t>(a.upper_offset[nd]) + (a.layer - 1)) * a.bu;
+        };
 
         // Layer 0 (src/hnsw.cpp:438-482) with ef' = max(ef, k) (src/hnsw.cpp:517).
         for (std::uint32_t i = static_cast<std::uint32_t>(lane) * 4; i < vslots; i += kWave * 4) {
@@ -286,7 +296,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 id = pref_id;
                 ++n_pref;
             } else {
-                const std::uint32_t* blk = a.layer0 + static_cast<std::size_t>(node) * b0;
+                const std::uint32_t* blk = block_of(node);
                 cnt = uniform(blk[0]);
                 id = static_cast<std::uint32_t>(lane) < cnt ? blk[1 + lane] : 0u;
             }
@@ -325,9 +335,9 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 }
                 spec_key = sc >= 0 ? (ks & kKeyMask) : kSentinel;
                 pref_node = sc >= 0 ? key_id(ks) : kNone;
-                const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(key_id(ks)) * b0;
+                const std::uint32_t* nb2 = block_of(sc >= 0 ? key_id(ks) : entry_node);
                 pref_cnt = nb2[0];
-                pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
+                pref_id = static_cast<std::uint32_t>(lane) < lcap ? nb2[1 + lane] : 0u;
             }
 
             n_dist += nf;
@@ -392,9 +402,9 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 // block now, while the merge completes.
                 if (fmin_pos < efp && fmin < spec_key) {
                     pref_node = key_id(fmin);
-                    const std::uint32_t* nb2 = a.layer0 + static_cast<std::size_t>(pref_node) * b0;
+                    const std::uint32_t* nb2 = block_of(pref_node);
                     pref_cnt = nb2[0];
-                    pref_id = static_cast<std::uint32_t>(lane) < a.M2 ? nb2[1 + lane] : 0u;
+                    pref_id = static_cast<std::uint32_t>(lane) < lcap ? nb2[1 + lane] : 0u;
                 }
                 inserted = static_cast<std::uint32_t>(__popcll(ballot(ins)));
 #pragma unroll

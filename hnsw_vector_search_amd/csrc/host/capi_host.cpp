@@ -53,14 +53,15 @@ public:
         fill_view(g, &view);
         check(hooks_.sync(hooks_.ctx, &view, changed.data(), changed.size()), "sync");
     }
-    void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef, std::size_t m,
-                std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) override {
+    void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t layer, std::size_t ef,
+                std::size_t m, std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) override {
         const std::size_t stride = m > 0 ? m : ef;
         out.resize(nodes.size() * stride);
         count.assign(nodes.size(), 0u);
         static_assert(sizeof(Neighbor) == sizeof(vdx_neighbor));
-        check(hooks_.search(hooks_.ctx, nodes.data(), entries.data(), nodes.size(), static_cast<std::uint32_t>(ef),
-                            static_cast<std::uint32_t>(m), reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
+        check(hooks_.search(hooks_.ctx, nodes.data(), entries.data(), nodes.size(), static_cast<std::uint32_t>(layer),
+                            static_cast<std::uint32_t>(ef), static_cast<std::uint32_t>(m),
+                            reinterpret_cast<vdx_neighbor*>(out.data()), count.data()),
               "search");
         for (const std::uint32_t c : count) {
             if (c > stride) {
@@ -68,7 +69,8 @@ public:
             }
         }
     }
-    bool selects() const override { return hooks_.selects != 0; }
+    bool selects() const override { return (hooks_.caps & VDX_SEARCHER_SELECTS) != 0; }
+    bool upper_layers() const override { return (hooks_.caps & VDX_SEARCHER_UPPER_LAYERS) != 0; }
 
 private:
     void check(int rc, const char* which) const {

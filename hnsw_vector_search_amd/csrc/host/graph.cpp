@@ -307,19 +307,24 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
                 high_at.push_back(id - next);
             }
         }
-        // The upper-layer part of the inserts: nothing is written, so no locks.
-        plans.resize(high.size());
-        if (!high.empty()) {
-            parallel_for(high.size(), std::min(threads, high.size()),
-                         [this, &high, &plans](std::size_t i) { plan_upper(high[i], plans[i]); });
-            for (std::size_t i = 0; i < high.size(); ++i) {
-                entries[high_at[i]] = plans[i].entry0;
-            }
-            st.host_inserted += high.size();
-        }
         st.host_insert_s += since();
         sync();
-        searcher.search(nodes, entries, ef, m, found, count);
+        // The upper-layer part of the inserts (nothing is written, so no locks): by the searcher
+        // layer by layer, top down, or on the host threads.
+        plans.resize(high.size());
+        if (!high.empty() && searcher.upper_layers()) {
+            upper_by_searcher(searcher, high, plans, ef, m, found, count);
+            st.search_s += since();
+        } else if (!high.empty()) {
+            parallel_for(high.size(), std::min(threads, high.size()),
+                         [this, &high, &plans](std::size_t i) { plan_upper(high[i], plans[i]); });
+            st.host_inserted += high.size();
+            st.host_insert_s += since();
+        }
+        for (std::size_t i = 0; i < high.size(); ++i) {
+            entries[high_at[i]] = plans[i].entry0;
+        }
+        searcher.search(nodes, entries, 0, ef, m, found, count);
         st.search_s += since();
         st.searched += nodes.size();
         std::vector<UpperPlan*> plan_of(nodes.size(), nullptr);
@@ -360,6 +365,50 @@ void HnswGraph::plan_upper(Id node, UpperPlan& plan) {
         plan.chosen[layer] = select_neighbors(scratch.results, params_.M);
     }
     plan.entry0 = entry.id;
+}
+
+// plan_upper for a batch's nodes with upper layers, the layer searches done by the searcher: for
+// each layer from the batch's top down, one search call over the nodes that reach it, each from
+// the best result of the layer above (or, on its own top layer, from the descent), and the
+// neighbour choice among the results (by the searcher when it selects).
+void HnswGraph::upper_by_searcher(BatchSearcher& searcher, const std::vector<Id>& high, std::vector<UpperPlan>& plans,
+                                  std::size_t ef, std::size_t m, std::vector<Neighbor>& found,
+                                  std::vector<std::uint32_t>& count) {
+    std::size_t top_all = 0;
+    std::vector<std::size_t> top(high.size());
+    for (std::size_t i = 0; i < high.size(); ++i) {
+        top[i] = std::min<std::size_t>(levels_[high[i]], max_level_);
+        top_all = std::max(top_all, top[i]);
+        plans[i].chosen.assign(top[i] + 1, {});
+        plans[i].entry0 = BatchSearcher::kDescend;  // the layer-0 search descends if no upper layer was searched
+    }
+    const std::size_t stride = m > 0 ? m : ef;
+    std::vector<Id> nodes, entries;
+    std::vector<std::size_t> which;
+    for (std::size_t layer = top_all; layer >= 1; --layer) {
+        nodes.clear();
+        entries.clear();
+        which.clear();
+        for (std::size_t i = 0; i < high.size(); ++i) {
+            if (top[i] >= layer) {
+                nodes.push_back(high[i]);
+                entries.push_back(plans[i].entry0);
+                which.push_back(i);
+            }
+        }
+        searcher.search(nodes, entries, layer, ef, m, found, count);
+        for (std::size_t j = 0; j < nodes.size(); ++j) {
+            const Neighbor* row = found.data() + j * stride;
+            std::vector<Neighbor> chosen(row, row + count[j]);
+            if (m == 0) {
+                chosen = select_neighbors(std::move(chosen), params_.M);
+            }
+            UpperPlan& p = plans[which[j]];
+            // the closest result: select_neighbors keeps it first
+            p.entry0 = chosen.empty() ? p.entry0 : chosen.front().id;
+            p.chosen[layer] = std::move(chosen);
+        }
+    }
 }
 
 // The linking half of insert for a node whose layer-0 search results are `found` (or, with

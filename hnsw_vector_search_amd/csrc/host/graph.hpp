@@ -57,11 +57,15 @@ public:
     // layer-0 entry that replaces the descent (insert's layer-0 search starts from its layer-1
     // result, src/hnsw.cpp:324-335), or kDescend.
     // With m > 0 (only when selects()), out[i * m ..] receives select_neighbors(results, m)
-    // instead: the neighbour choice made where the results are.
+    // instead: the neighbour choice made where the results are. `layer` > 0 (only when
+    // upper_layers()) searches that layer, entries being nodes on it; kDescend then descends from
+    // the top to layer + 1.
     static constexpr Id kDescend = 0xFFFFFFFFu;
-    virtual void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t ef,
-                        std::size_t m, std::vector<Neighbor>& out, std::vector<std::uint32_t>& count) = 0;
+    virtual void search(const std::vector<Id>& nodes, const std::vector<Id>& entries, std::size_t layer,
+                        std::size_t ef, std::size_t m, std::vector<Neighbor>& out,
+                        std::vector<std::uint32_t>& count) = 0;
     virtual bool selects() const { return false; }
+    virtual bool upper_layers() const { return false; }
 };
 
 // How add_batched splits the rows: the first `host_prefix` nodes of the graph are inserted on
@@ -158,6 +162,9 @@ private:
         Id entry0 = 0;                              // where the layer-0 search starts
     };
     void plan_upper(Id node, UpperPlan& plan);
+    void upper_by_searcher(BatchSearcher& searcher, const std::vector<Id>& high, std::vector<UpperPlan>& plans,
+                           std::size_t ef, std::size_t m, std::vector<Neighbor>& found,
+                           std::vector<std::uint32_t>& count);
     void link_searched(Id node, const Neighbor* found, std::size_t count, bool chosen, UpperPlan* plan);
     std::vector<Neighbor> select_neighbors(std::vector<Neighbor> candidates, std::size_t m) const;
     template <Locking L>

@@ -92,24 +92,31 @@ VDX_API int vdx_host_graph_params(const vdx_host_graph* g, uint64_t* ef_construc
 VDX_API uint64_t vdx_host_graph_memory_bytes(const vdx_host_graph* g);
 /* HnswIndex::add (src/hnsw.cpp:247-291) with the layer-0 insertion searches run in batches by a
  * searcher over its own copy of the graph (no counterpart in vecdex, which searches node by
- * node). Per batch: the upper-layer part of each insert runs on the host (descent; layer
- * searches and choices for nodes with upper layers), sync() brings the searcher's copy up to the
- * graph (`changed`: ascending ids of the nodes stored or rewritten since the last sync), search()
- * returns each node's layer-0 search_layer results at breadth ef = max(ef_construction, M), any
- * order, starting from entries[i] (its layer-1 result, src/hnsw.cpp:324-335) or, where
- * entries[i] is UINT32_MAX, from the greedy descent (src/hnsw.cpp:315-323), and each node is
- * linked as insert links it. Nodes of one batch are not candidates of each other. Callbacks run
- * on the calling thread and return a vdx status. */
+ * node). Per batch: sync() brings the searcher's copy up to the graph (`changed`: ascending ids
+ * of the nodes stored or rewritten since the last sync); the layers above 0 of the batch's nodes
+ * that have them are searched top down (by the searcher with VDX_SEARCHER_UPPER_LAYERS, else on
+ * the host), each from the layer above's best result; search() then returns each node's layer-0
+ * search_layer results at breadth ef = max(ef_construction, M), any order, starting from
+ * entries[i] (its layer-1 result, src/hnsw.cpp:324-335) or, where entries[i] is UINT32_MAX,
+ * from the greedy descent (src/hnsw.cpp:315-323); each node is linked as insert links it. Nodes
+ * of one batch are not candidates of each other. Callbacks run on the calling thread and return
+ * a vdx status. */
 typedef struct vdx_batch_searcher {
     void* ctx;
     int (*sync)(void* ctx, const vdx_graph_view* graph, const uint32_t* changed, uint64_t count);
-    /* m == 0: out[i * ef ..] receives node i's results. m > 0 (only if `selects`): out[i * m ..]
-     * receives insert's neighbour choice among them (select_neighbors(results, m),
+    /* The beam on `layer` (0 unless caps has VDX_SEARCHER_UPPER_LAYERS): from entries[i], a node
+     * on that layer, or where it is UINT32_MAX from the greedy descent down to layer + 1.
+     * m == 0: out[i * ef ..] receives node i's results. m > 0 (only with VDX_SEARCHER_SELECTS):
+     * out[i * m ..] receives insert's neighbour choice among them (select_neighbors(results, m),
      * src/hnsw.cpp:350-409 heuristic, closest first). */
-    int (*search)(void* ctx, const uint32_t* nodes, const uint32_t* entries, uint64_t count, uint32_t ef,
-                  uint32_t m, vdx_neighbor* out, uint32_t* counts);
-    uint32_t selects; /* nonzero: search() also makes the neighbour choice */
+    int (*search)(void* ctx, const uint32_t* nodes, const uint32_t* entries, uint64_t count, uint32_t layer,
+                  uint32_t ef, uint32_t m, vdx_neighbor* out, uint32_t* counts);
+    uint32_t caps; /* VDX_SEARCHER_* bits */
 } vdx_batch_searcher;
+enum {
+    VDX_SEARCHER_SELECTS = 1,      /* search() also makes the neighbour choice */
+    VDX_SEARCHER_UPPER_LAYERS = 2  /* search() also searches layers above 0 (else the host does) */
+};
 /* Batch sizes: the first host_prefix nodes of the graph are inserted on the host; then batches
  * of batch_fraction x (nodes so far), clamped to [batch_min, batch_max]. NULL: 16384, 1024,
  * 131072, 0.05. batch_min = batch_max = 1 with host_prefix 0 and one thread builds exactly

@@ -55,8 +55,8 @@ _lib.vdxo_hnsw_search_batch.restype = _int
 _lib.vdxo_hnsw_search_batch.argtypes = [ctypes.POINTER(_Graph), _p, _u64, _u64, _u64, _p, _u64,
                                         _p, _int, _int]
 _lib.vdxo_hnsw_search_batch_from.restype = _int
-_lib.vdxo_hnsw_search_batch_from.argtypes = [ctypes.POINTER(_Graph), _p, _p, _u64, _u64, _u64, _p, _u64,
-                                             _p, _int, _int]
+_lib.vdxo_hnsw_search_batch_from.argtypes = [ctypes.POINTER(_Graph), _p, _p, _u64, _u64, _u64, _u64, _p,
+                                             _u64, _p, _int, _int]
 
 NEIGHBOR = np.dtype([("distance", np.float32), ("id", np.uint32)])
 
@@ -109,11 +109,12 @@ def bruteforce_f32(base, queries, k: int, metric: int = 0, threads: int = 0) -> 
 
 
 def hnsw_search(graph: dict, queries, k: int, ef: int, threads: int = 0, fast: bool = True,
-                entries=None):
+                entries=None, layer: int = 0):
     """HnswIndex::search for every row of prepared queries (normalized under cosine) over a
     graph given as FORMAT v1 sections (dict of numpy arrays and header fields).
-    entries: per query, a layer-0 entry that replaces the greedy descent (insert's layer-0
-    search, src/hnsw.cpp:324-335), or 0xFFFFFFFF to descend.
+    layer: the layer the beam searches (0 for queries; insert searches each of its layers).
+    entries: per query, an entry on that layer that replaces the greedy descent (insert's
+    searches start from the layer above's result, src/hnsw.cpp:324-335), or 0xFFFFFFFF.
     Returns (structured [nq, k] array, counts[nq])."""
     queries = _c(queries, np.float32)
     keep = [graph[name] for name in ("levels", "layer0", "upper", "vectors")]
@@ -124,7 +125,7 @@ def hnsw_search(graph: dict, queries, k: int, ef: int, threads: int = 0, fast: b
     count = np.zeros(nq, dtype=np.uint32)
     ent = None if entries is None else _c(entries, np.uint32)
     _lib.vdxo_hnsw_search_batch_from(ctypes.byref(g), queries.ctypes.data,
-                                     None if ent is None else ent.ctypes.data, nq, k, ef, out.ctypes.data,
+                                     None if ent is None else ent.ctypes.data, layer, nq, k, ef, out.ctypes.data,
                                      k, count.ctypes.data, threads, 1 if fast else 0)
     del keep
     return out, count

@@ -421,7 +421,8 @@ typedef struct {
     const vdxo_graph* g;
     uint64_t* upper_offset;
     const float* queries;
-    const uint32_t* entries; /* per query: a layer-0 entry (no descent), or UINT32_MAX; may be NULL */
+    const uint32_t* entries; /* per query: an entry on `layer` (no descent), or UINT32_MAX; may be NULL */
+    uint64_t layer;          /* the layer the beam searches: 0 for queries, any for an insertion */
     uint64_t k, ef;
     vdxo_neighbor* out;
     uint64_t out_stride;
@@ -457,8 +458,8 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
     vdxo_neighbor entry = {metric_dist(g->metric, c->l2, c->ip, query,
                                        g->vectors + g->entry_point * dim, dim),
                            (uint32_t)g->entry_point};
-    /* A given layer-0 entry replaces the descent: insert's layer-0 search starts from the best
-     * result of its layer-1 search (src/hnsw.cpp:324-335). */
+    /* A given entry replaces the descent: insert's search of each layer starts from the best
+     * result of the layer above (src/hnsw.cpp:324-335). */
     uint64_t top = g->max_level;
     if (c->entries != NULL && c->entries[q] != UINT32_MAX) {
         entry.id = c->entries[q];
@@ -466,7 +467,7 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
         top = 0;
     }
     /* greedy descent: move to any strictly closer neighbor, in link order (src/hnsw.cpp:416-431) */
-    for (uint64_t layer = top; layer > 0; --layer) {
+    for (uint64_t layer = top; layer > c->layer; --layer) {
         for (int moved = 1; moved;) {
             moved = 0;
             const uint32_t* b = links_of(c, entry.id, layer);
@@ -481,7 +482,7 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
             }
         }
     }
-    /* layer-0 beam (src/hnsw.cpp:438-482) */
+    /* the beam on the searched layer (src/hnsw.cpp:438-482) */
     if (++s->epoch == 0) {
         memset(s->marks, 0, g->n * sizeof(uint32_t));
         s->epoch = 1;
@@ -493,7 +494,7 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
     while (ncand > 0) {
         vdxo_neighbor cur = heap_pop(s->cand, &ncand, closer);
         if (nres >= ef && closer(s->res[0], cur)) break;
-        const uint32_t* b = links_of(c, cur.id, 0);
+        const uint32_t* b = links_of(c, cur.id, c->layer);
         uint32_t nf = 0;
         for (uint32_t i = 0; i < b[0]; ++i) {
             uint32_t id = b[1 + i];
@@ -543,12 +544,13 @@ static void hnsw_cleanup(void* local) {
 }
 
 int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const uint32_t* entries,
-                                uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
-                                uint32_t* out_count, int threads, int use_fast_kernel) {
+                                uint64_t layer, uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out,
+                                uint64_t out_stride, uint32_t* out_count, int threads, int use_fast_kernel) {
     hnsw_ctx c;
     c.g = g;
     c.queries = queries;
     c.entries = entries;
+    c.layer = layer;
     c.k = k;
     c.ef = ef;
     c.out = out;
@@ -572,7 +574,7 @@ int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const
 int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
                            uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
                            uint32_t* out_count, int threads, int use_fast_kernel) {
-    return vdxo_hnsw_search_batch_from(g, queries, NULL, nq, k, ef, out, out_stride, out_count, threads,
+    return vdxo_hnsw_search_batch_from(g, queries, NULL, 0, nq, k, ef, out, out_stride, out_count, threads,
                                        use_fast_kernel);
 }
 

@@ -69,11 +69,12 @@ uint64_t vdxo_bruteforce_f32(const float* base, uint64_t n, uint64_t dim, uint32
 int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
                            uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
                            uint32_t* out_count, int threads, int use_fast_kernel);
-/* The same with per-query layer-0 entries (entries[q] = UINT32_MAX: descend as above): the
- * layer-0 search of HnswIndex::insert, which starts from its layer-1 result (src/hnsw.cpp:324-335). */
+/* The same on any layer, with optional per-query entries on it (entries[q] = UINT32_MAX: descend
+ * from the top to the layer above as above): the layer searches of HnswIndex::insert, each of
+ * which starts from the result of the layer above (src/hnsw.cpp:315-335). */
 int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const uint32_t* entries,
-                                uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
-                                uint32_t* out_count, int threads, int use_fast_kernel);
+                                uint64_t layer, uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out,
+                                uint64_t out_stride, uint32_t* out_count, int threads, int use_fast_kernel);
 
 /* XXH64 from the xxHash specification (doc/xxhash_spec.md), one shot. */
 uint64_t vdxo_xxh64(const void* data, size_t size, uint64_t seed);

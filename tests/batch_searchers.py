@@ -46,11 +46,12 @@ def apply_changes(copy: dict | None, live: dict, changed: np.ndarray) -> dict:
 class OracleSearcher:
     """A vdx_batch_searcher whose copy is the synced view and whose searches are the oracle's."""
 
-    def __init__(self, fail_search: int = 0, graph: dict | None = None):
+    def __init__(self, fail_search: int = 0, graph: dict | None = None, upper_layers: bool = False):
         self.graph = graph
         self.syncs = 0
         self.batches = []
         self.entries = []
+        self.layers = []
         self.changed = []
 
         def sync(ctx, view, changed, count):
@@ -60,8 +61,8 @@ class OracleSearcher:
             self.syncs += 1
             return 0
 
-        def search(ctx, nodes, entries, count, ef, m, out, counts):
-            if m != 0:
+        def search(ctx, nodes, entries, count, layer, ef, m, out, counts):
+            if m != 0 or (layer != 0 and not upper_layers):
                 return 2  # this searcher returns results; the graph makes the choice
             if fail_search:
                 return fail_search
@@ -70,13 +71,14 @@ class OracleSearcher:
             self.batches.append(ids)
             self.entries.append(ent)
             g = self.graph
-            found, cnt = o.hnsw_search(g, g["vectors"][ids], ef, ef, entries=ent)
+            self.layers.append(layer)
+            found, cnt = o.hnsw_search(g, g["vectors"][ids], ef, ef, entries=ent, layer=layer)
             ctypes.memmove(out, found.ctypes.data, found.nbytes)
             ctypes.memmove(counts, cnt.ctypes.data, cnt.nbytes)
             return 0
 
         self._keep = (nat.SYNC_FN(sync), nat.SEARCH_FN(search))
-        self.c = nat.BatchSearcher(None, *self._keep, 0)
+        self.c = nat.BatchSearcher(None, *self._keep, nat.SEARCHER_UPPER_LAYERS if upper_layers else 0)
 
 
 def add_batched(index, data, threads, plan, searcher):

@@ -46,6 +46,33 @@ def test_batch_of_one_builds_the_serial_graph():
     assert np.array_equal(s.changed[0], np.arange(1000, dtype=np.uint32))
 
 
+def test_upper_layers_searched_by_the_searcher_build_the_same_graph():
+    """The layers above 0 searched by the searcher (top down, one call per layer, each from the
+    layer above's best result) instead of on the host: a batch of one still rebuilds the serial
+    graph, and larger batches the same graph as host-side planning."""
+    base = datasets.integer_matrix(1000, 16, 1)
+    queries = datasets.integer_matrix(200, 16, 2)
+    serial = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=64, seed=42)
+    serial.add(base)
+    one = vdx.HnswIndex(16, vdx.Metric.L2, M=8, ef_construction=64, seed=42)
+    s = OracleSearcher(upper_layers=True)
+    assert add_batched(one, base, 1, nat.BuildPlan(0, 1, 1, 0.0), s) == 0
+    a, b = sections(serial), sections(one)
+    for key in ("levels", "layer0", "upper", "max_level", "entry_point"):
+        assert np.array_equal(a[key], b[key]), key
+    assert fingerprint(b, queries, 10, 10) == TOP10_FP
+    assert max(s.layers) >= 2  # upper layers were searched by the searcher
+    mix = datasets.gmm64()
+    data = mix.sample(6000, 1) + 1
+    graphs = []
+    for upper in (False, True):
+        index = vdx.HnswIndex(128, vdx.Metric.Cosine, M=6, ef_construction=60, seed=5)
+        assert add_batched(index, data, 1, nat.BuildPlan(500, 64, 1024, 0.05), OracleSearcher(upper_layers=upper)) == 0
+        graphs.append(sections(index))
+    for key in ("levels", "layer0", "upper", "max_level", "entry_point"):
+        assert np.array_equal(graphs[0][key], graphs[1][key]), key
+
+
 @pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
 def test_batches_build_a_valid_graph_with_the_host_build_recall(metric):
     mix = datasets.gmm64()
