@@ -176,6 +176,21 @@ void HnswGraph::store(const float* vector) {
     upper_.resize(upper_.size() + level * (1 + params_.M), Id{0});
 }
 
+// Room for `total` nodes, growing by at least half of what is there: a graph built by many
+// adds then copies each section a bounded number of times, not once per add.
+void HnswGraph::reserve_sections(std::size_t total) {
+    const auto grow = [](auto& v, std::size_t need) {
+        if (need > v.capacity()) {
+            v.reserve(std::max(need, v.capacity() + v.capacity() / 2));
+        }
+    };
+    grow(vectors_, total * dim_);
+    grow(levels_, total);
+    grow(upper_offset_, total);
+    grow(touched_, total);
+    grow(layer0_, total * (1 + 2 * params_.M));
+}
+
 std::vector<Id> HnswGraph::changed_since(std::uint64_t version) const {
     std::vector<Id> out;
     for (std::size_t i = 0; i < touched_.size(); ++i) {
@@ -191,11 +206,7 @@ void HnswGraph::add(const float* data, std::size_t rows, std::size_t num_threads
     ++version_;
     const std::size_t first = size();
     const std::size_t total = first + rows;
-    vectors_.reserve(total * dim_);
-    levels_.reserve(total);
-    upper_offset_.reserve(total);
-    touched_.reserve(total);
-    layer0_.reserve(total * (1 + 2 * params_.M));
+    reserve_sections(total);
     const std::size_t threads = std::min(resolve_threads(num_threads), rows);
     if (threads <= 1) {
         for (std::size_t r = 0; r < rows; ++r) {
@@ -243,11 +254,7 @@ void HnswGraph::add_batched(const float* data, std::size_t rows, std::size_t num
     ++version_;
     const std::size_t first = size();
     const std::size_t total = first + rows;
-    vectors_.reserve(total * dim_);
-    levels_.reserve(total);
-    upper_offset_.reserve(total);
-    touched_.reserve(total);
-    layer0_.reserve(total * (1 + 2 * params_.M));
+    reserve_sections(total);
     // As in add(): every row is stored (levels drawn in serial order) before any is linked.
     for (std::size_t r = 0; r < rows; ++r) {
         store(data + r * dim_);

@@ -151,6 +151,7 @@ private:
     const float* vector_of(Id node) const noexcept { return vectors_.data() + node * dim_; }
     float distance(const float* a, const float* b) const noexcept;
 
+    void reserve_sections(std::size_t total);
     std::size_t random_level();
     void store(const float* vector);
     template <Locking L>
