@@ -152,6 +152,7 @@ _SIGS = {
     "vdx_gpu_hnsw_batch_classes": (c_int, [c_ptr, ctypes.POINTER(BatchClass), c_u32,
                                            ctypes.POINTER(c_u32)]),
     "vdx_gpu_bf_upload": (c_int, [c_ptr, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
+    "vdx_gpu_bf_append": (c_int, [c_ptr, c_ptr, c_u64, c_u64]),
     "vdx_gpu_bf_free": (None, [c_ptr]),
     "vdx_gpu_bf_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u32, c_u32, c_ptr, c_u64,
                                   c_ptr, c_char_p, ctypes.POINTER(BfStats)]),

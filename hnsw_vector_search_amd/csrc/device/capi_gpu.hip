@@ -925,6 +925,39 @@ int vdx_gpu_bf_upload(vdx_gpu_ctx* ctx, uint32_t metric, uint64_t dim, uint64_t 
     });
 }
 
+int vdx_gpu_bf_append(vdx_gpu_bf* b, const float* vectors, uint64_t count, uint64_t dim) {
+    return guarded([&] {
+        require(b != nullptr, "vdx_gpu_bf_append: index is null");
+        require(dim == b->dim, "vdx_gpu_bf_append: rows of another dimension");
+        require(count == 0 || vectors != nullptr, "vdx_gpu_bf_append: vectors is null");
+        require(b->n + count < 0x80000000ull, "vdx_gpu_bf_append: more than 2^31 - 1 vectors");
+        if (count == 0) return;
+        DeviceGuard guard(b->ctx->device);
+        hipStream_t s = b->ctx->stream;
+        const std::uint64_t ld = b->split.ld;
+        const std::uint64_t need = (b->n + count) * ld * 4;
+        if (need > b->vectors.bytes()) {  // grow with a quarter of headroom, keeping the rows
+            DeviceBuffer grown(need + need / 4);
+            if (b->n > 0) {
+                check(hipMemcpyAsync(grown.as<void>(), b->vectors.as<void>(), b->n * ld * 4, hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync(grow)");
+            }
+            check(hipStreamSynchronize(s), "hipStreamSynchronize(grow)");
+            b->vectors.swap(grown);
+        }
+        copy_rows(b->vectors.as<float>() + b->n * ld, vectors, count, dim, ld, s);
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(append)");
+        {
+            // the MFMA operands were made for the old rows: made again on next use
+            const std::lock_guard<std::mutex> lock(b->prep_mutex);
+            for (auto& p : b->prep) {
+                p.ready = false;
+            }
+        }
+        b->n += count;
+    });
+}
+
 void vdx_gpu_bf_free(vdx_gpu_bf* index) {
     if (index != nullptr) {
         DeviceGuard guard(index->ctx->device);

@@ -497,16 +497,19 @@ class BruteForceIndex:
         return self._vectors
 
     def gpu_handle(self):
-        if self._gpu is None or self._gpu_rows != self.size():
+        """The HBM copy of the stored rows: uploaded on first use; rows added since are appended
+        (vdx_gpu_bf_append)."""
+        if self._gpu is None:
             ctx = nat.Context.get(self._device)
-            if self._gpu is not None:
-                nat.lib.vdx_gpu_bf_free(self._gpu)
-                self._gpu = None
             h = ctypes.c_void_p()
             v = np.ascontiguousarray(self._vectors)
             nat.check(nat.lib.vdx_gpu_bf_upload(ctx.handle, self._metric, self._dim, v.shape[0],
                                                 nat.ptr(v), ctypes.byref(h)))
             self._gpu, self._gpu_rows = h, self.size()
+        elif self._gpu_rows != self.size():
+            new = np.ascontiguousarray(self._vectors[self._gpu_rows:])
+            nat.check(nat.lib.vdx_gpu_bf_append(self._gpu, nat.ptr(new), new.shape[0], self._dim))
+            self._gpu_rows = self.size()
         return self._gpu
 
     def __del__(self):

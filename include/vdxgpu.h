@@ -371,6 +371,11 @@ enum vdx_precision {
 /* Vectors as BruteForceIndex stores them (unit length under cosine). */
 VDX_API int vdx_gpu_bf_upload(vdx_gpu_ctx* ctx, uint32_t metric, uint64_t dim, uint64_t n,
                       const float* vectors, vdx_gpu_bf** out);
+/* BruteForceIndex::add after upload (no counterpart in vecdex, whose index has no device copy):
+ * appends `count` stored rows (normalized under cosine, as add stores them) to the device copy,
+ * growing it with headroom; the MFMA operands are made again on next use. Not concurrent with
+ * searches on the same handle. */
+VDX_API int vdx_gpu_bf_append(vdx_gpu_bf* index, const float* vectors, uint64_t count, uint64_t dim);
 VDX_API void vdx_gpu_bf_free(vdx_gpu_bf* index);
 
 typedef struct vdx_bf_stats {

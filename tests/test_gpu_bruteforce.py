@@ -135,3 +135,26 @@ def test_bf16_is_exact_on_small_integers():
     got, _ = index.search_arrays(queries, 10, "bf16")
     assert index.last_stats.max_abs_error == 0.0
     assert index.last_stats.exact_fallback == 0
+
+
+@pytest.mark.parametrize("metric,dim", [(0, 128), (2, 100)])
+def test_rows_added_after_the_upload_are_appended(metric, dim):
+    """BruteForceIndex::add after a search appends to the device copy (vdx_gpu_bf_append): every
+    mode then searches all rows, with results equal to the host's exact scan over them."""
+    print(f"case: bf append, metric {metric}, dim {dim}", flush=True)
+    gen = datasets.AnisotropicGaussian(dim, 0.8, seed=6)
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(gen.sample(20_000, 1))
+    queries = gen.sample(64, 2)
+    prepared = o.normalize(queries) if metric == 2 else queries
+    index.search_arrays(queries, 10, "exact")
+    index.search_arrays(queries, 10, "bf16")  # builds the BF16 operands of the first rows
+    handle = index.gpu_handle().value
+    for step, rows in enumerate((3_000, 40_000)):  # the second outgrows the headroom
+        index.add(gen.sample(rows, 10 + step))
+        ref = o.bruteforce_f32(index.vectors(), prepared, 10, metric=metric)
+        for precision in ("exact", "fp32", "bf16"):
+            got, cnt = index.search_arrays(queries, 10, precision)
+            assert (cnt == 10).all()
+            assert got.tobytes() == ref.tobytes(), (step, precision)
+        assert index.gpu_handle().value == handle
