@@ -35,6 +35,9 @@ namespace {
 
 constexpr std::uint32_t kStageBytes = 64 * 8 + 64 * 4;  // per wave: 64 survivor keys and queries
 
+// K steps of the query operand held in registers (F32: float2 pairs of K=2 steps; else K=16).
+constexpr std::uint32_t bf_gemm_steps(int prec) { return prec == kPrecF32 ? 32 : 8; }
+
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -118,26 +121,92 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
     }
 }
 
-// The GEMM. A workgroup of 4 waves takes 128 queries (32 per wave, one MFMA column tile held
-// in registers for the whole launch) and one slab of base rows. Base tiles of NT rows (64; 32
-// for FP32) are the MFMA row operand and go through LDS, double buffered: the 256 threads issue
-// the 16-byte global loads of tile t+1 before the MFMAs of tile t and store them to the other
-// buffer afterwards, so one barrier per tile suffices and load latency hides behind compute.
-// LDS rows are padded by 16 bytes so the 32 lanes of a fragment read hit distinct banks; the
-// tile's |x|^2 and scales sit next to it in LDS.
+// The epilogue of the GEMM kernels, over the accumulators acc[NH] of base tile tile_t, whose
+// |x|^2 (and FP8 scales) sit in LDS at vals + tile_cb * NT (vals + (2 + tile_cb) * NT): rows
+// 8 * g + 4 * half + (0..3) of each 32-row MFMA tile, four at a time. MODE 0 stores the dense
+// distances; MODE 1 (always over consecutive rows, row_stride 1: a survivor's id is its row)
+// stages the survivors (distance <= tau) for the wave's next flush. Padding
+// queries never pass (tau = -inf); the row bound is tested only in a tile that crosses the end
+// of the rows.
+#define VDX_EPILOGUE(CHECK_ROWS)                                                                  \
+        _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
+            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
+                const int r0 = h * 32 + 8 * g + 4 * half;                                         \
+                const float4 xn4 = *reinterpret_cast<const float4*>(vals + tile_cb * NT + r0);         \
+                float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);                                     \
+                if constexpr (PREC == kPrecFP8) {                                                 \
+                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + tile_cb) * NT + r0);            \
+                }                                                                                 \
+                float d4[4];                                                                      \
+                std::uint64_t m4[4];                                                              \
+                _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                   \
+                    const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w)); \
+                    const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w)); \
+                    float dot = acc[h][4 * g + e];                                                \
+                    if constexpr (PREC == kPrecFP8) {                                             \
+                        dot = dot * qs * xsv;                                                     \
+                    }                                                                             \
+                    const std::uint32_t j = tile_t * NT + r0 + e;                                      \
+                    d4[e] = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;               \
+                    if constexpr (MODE == 0) {                                                    \
+                        if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
+                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d4[e];        \
+                        }                                                                         \
+                    } else {                                                                      \
+                        /* the lanes whose distance passes, straight from the compare */          \
+                        m4[e] = __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);           \
+                        if (CHECK_ROWS) {                                                         \
+                            m4[e] &= ballot(j < a.rows);                                          \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+                if constexpr (MODE != 0) {                                                        \
+                    if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
+                        _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
+                            const std::uint64_t m = m4[e];                                        \
+                            if (m != 0) {                                                         \
+                                const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m)); \
+                                if (staged + np > 64) {                                           \
+                                    flush();                                                      \
+                                }                                                                 \
+                                if ((m >> lane) & 1u) {                                           \
+                                    const std::uint32_t at = staged + lanes_below(m);             \
+                                    skey[at] = make_key(d4[e], tile_t * NT + r0 + e);              \
+                                    squery[at] = q;                                               \
+                                }                                                                 \
+                                staged += np;                                                     \
+                            }                                                                     \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+            }                                                                                     \
+        }
+
+// The GEMM. A workgroup of WAVES waves (8 at launch) takes 32 * WAVES queries (32 per wave, one
+// MFMA column tile held in registers for the whole launch) and one slab of base rows. Base tiles
+// of NT rows (64; 32 for FP32) are the MFMA row operand and go through LDS, double buffered: the
+// workgroup issues the 16-byte global loads of tile t+2 before the MFMAs of tile t and stores
+// tile t+1 to the other buffer afterwards, so one barrier per tile suffices and load latency
+// hides behind compute. LDS rows are padded by 16 bytes so the 32 lanes of a fragment read hit
+// distinct banks; the tile's |x|^2 and scales sit next to it in LDS.
+//
+// BF16 and FP8 are held to 128 VGPRs (4 waves per SIMD), so two 8-wave workgroups share a CU
+// and one's epilogue and barrier overlap the other's MFMAs (1.3x over 3 waves per SIMD, which
+// fits only one workgroup); FP32's larger fragments would spill at that bound.
 //
 // MFMA 32x32 output layout: lane l holds column n = l % 32 (its query) and rows
 // m = 8 * (i / 4) + 4 * (l / 32) + (i % 4) (base rows) of its 16 accumulators i. Operand
 // fragments: lane l holds row/column l % 32 and the K slice
 // [kstep * K + (l / 32) * K / 2, ... + K / 2).
 // MODE 0: dense distances to a sample of rows; MODE 1: the threshold filter. L2: squared L2,
-// else 1 - <q, x>.
-template <int PREC, int WAVES, int MODE, bool L2>
-__global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
+// else 1 - <q, x>. FULL: the operand rows fill every K step (a.ks == KS), so the step loop has
+// no run-time guard; a guarded step makes the compiler copy all accumulators after each MFMA.
+template <int PREC, int WAVES, int MODE, bool L2, bool FULL>
+__global__ __launch_bounds__(WAVES * 64, PREC == kPrecF32 ? 2 : 4) void bf_gemm_kernel(GemmArgs a) {
     constexpr int EB = PREC == kPrecF32 ? 4 : (PREC == kPrecBF16 ? 2 : 1);  // bytes per element
     constexpr int NT = PREC == kPrecF32 ? 32 : 64;                           // rows per tile
     constexpr int NH = NT / 32;                                              // MFMA row tiles
-    constexpr int KS = PREC == kPrecF32 ? 32 : 8;  // K steps (F32: float2 pairs of K=2 steps)
+    constexpr int KS = static_cast<int>(bf_gemm_steps(PREC));
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
@@ -279,62 +348,6 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
     // the compiler then waits only for the stage a step is about to store, never for vmcnt(0).
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) only
     __syncthreads();
-        // Epilogue: rows 8 * g + 4 * half + (0..3) of each row tile, four at a time. Padding
-        // queries never pass (tau = -inf); the row bound is tested only in a tile that crosses
-        // the end of the rows.
-#define VDX_EPILOGUE(CHECK_ROWS)                                                                  \
-        _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
-            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
-                const int r0 = h * 32 + 8 * g + 4 * half;                                         \
-                const float4 xn4 = *reinterpret_cast<const float4*>(vals + tile_cb * NT + r0);         \
-                float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);                                     \
-                if constexpr (PREC == kPrecFP8) {                                                 \
-                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + tile_cb) * NT + r0);            \
-                }                                                                                 \
-                float d4[4];                                                                      \
-                std::uint64_t m4[4];                                                              \
-                _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                   \
-                    const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w)); \
-                    const float xsv = e == 0 ? xs4.x : (e == 1 ? xs4.y : (e == 2 ? xs4.z : xs4.w)); \
-                    float dot = acc[h][4 * g + e];                                                \
-                    if constexpr (PREC == kPrecFP8) {                                             \
-                        dot = dot * qs * xsv;                                                     \
-                    }                                                                             \
-                    const std::uint32_t j = tile_t * NT + r0 + e;                                      \
-                    d4[e] = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;               \
-                    if constexpr (MODE == 0) {                                                    \
-                        if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
-                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d4[e];        \
-                        }                                                                         \
-                    } else {                                                                      \
-                        /* the lanes whose distance passes, straight from the compare */          \
-                        m4[e] = __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);           \
-                        if (CHECK_ROWS) {                                                         \
-                            m4[e] &= ballot(j < a.rows);                                          \
-                        }                                                                         \
-                    }                                                                             \
-                }                                                                                 \
-                if constexpr (MODE != 0) {                                                        \
-                    if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
-                        _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
-                            const std::uint64_t m = m4[e];                                        \
-                            if (m != 0) {                                                         \
-                                const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m)); \
-                                if (staged + np > 64) {                                           \
-                                    flush();                                                      \
-                                }                                                                 \
-                                if ((m >> lane) & 1u) {                                           \
-                                    const std::uint32_t at = staged + lanes_below(m);             \
-                                    skey[at] = make_key(d4[e], (tile_t * NT + r0 + e) * a.row_stride); \
-                                    squery[at] = q;                                               \
-                                }                                                                 \
-                                staged += np;                                                     \
-                            }                                                                     \
-                        }                                                                         \
-                    }                                                                             \
-                }                                                                                 \
-            }                                                                                     \
-        }
     for (std::uint32_t t = t0; t < t1; t += 2) {
         {
             const std::uint32_t tile_t = t;
@@ -347,7 +360,7 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
             }
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                if (static_cast<std::uint32_t>(s) < ks) {
+                if (FULL || static_cast<std::uint32_t>(s) < ks) {
 #pragma unroll
                     for (int h = 0; h < NH; ++h) {
                         const unsigned char* rowp = lds + tile_cb * NT * rs + (h * 32 + col) * rs;
@@ -392,7 +405,7 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
             }
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                if (static_cast<std::uint32_t>(s) < ks) {
+                if (FULL || static_cast<std::uint32_t>(s) < ks) {
 #pragma unroll
                     for (int h = 0; h < NH; ++h) {
                         const unsigned char* rowp = lds + tile_cb * NT * rs + (h * 32 + col) * rs;
@@ -424,7 +437,6 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
             __syncthreads();
         }
     }
-#undef VDX_EPILOGUE
     if (MODE != 0 && staged > 0) {
         flush();
     }
@@ -434,6 +446,173 @@ __global__ __launch_bounds__(WAVES * 64) void bf_gemm_kernel(GemmArgs a) {
 #undef VDX_STORE_PIECE
 #undef VDX_FETCH_STAGE
 #undef VDX_STORE_STAGE
+
+// BF16 with every K step filled (operand rows of 128 elements, dimensions 113 to 128). Base
+// tiles go from global memory straight into LDS (global_load_lds_dwordx4: no staging registers,
+// no LDS stores), triple buffered: step t issues the loads of tile t + 2 into the buffer that
+// tile t - 1 used, computes tile t, waits until tile t + 1 has landed and meets the workgroup at
+// one barrier. The registers this frees let four waves per SIMD keep MFMA operands in flight.
+//
+// A wave's load instruction fills 1 KB of LDS, 16 bytes per lane in lane order, so rows are not
+// padded: row r keeps its 16-byte chunk c at slot c ^ (r % 16), which spreads the 32 rows of a
+// fragment read over all banks. Lane (col, half) reads chunk 2s + half of row h * 32 + col at
+// row * 256 + 16 * ((2s + half) ^ (col % 16)) = A0 ^ (32 s) + 8192 h, one XOR per step.
+//
+// The loads are issued from inline asm, which the compiler does not see writing LDS: its own
+// bookkeeping for these loads would wait for every one in flight before any LDS read (the
+// epilogue's |x|^2 reads included), which serializes the prefetch. The kernel orders them
+// itself: a buffer is read only after wait_older() and the barrier that follow its loads, and
+// is loaded again only after the barrier that ends the last step reading it.
+__device__ __forceinline__ std::uint32_t lds_offset(const void* p) {
+    return static_cast<std::uint32_t>(reinterpret_cast<std::uintptr_t>(p));  // LDS aperture offset
+}
+
+// 16 bytes per lane from `src` to LDS at m0 + 16 * lane (m0 wave-uniform).
+__device__ __forceinline__ void lds_load16(const void* src, std::uint32_t m0) {
+    asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(src), "{m0}"(m0));
+}
+
+// 4 bytes per lane from `src` to LDS at m0 + 4 * lane (m0 wave-uniform).
+__device__ __forceinline__ void lds_load4(const void* src, std::uint32_t m0) {
+    asm volatile("global_load_lds_dword %0, off" ::"v"(src), "{m0}"(m0));
+}
+
+template <int MODE, bool L2>
+__global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
+    constexpr int PREC = kPrecBF16;
+    constexpr int NT = 64;
+    constexpr int NH = 2;
+    constexpr int KS = 8;
+    constexpr std::uint32_t kRowBytes = 256;
+    constexpr std::uint32_t kTileBytes = NT * kRowBytes;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 31;
+    const int half = lane >> 5;
+    const std::uint32_t q = blockIdx.x * 256 + wave * 32 + col;  // this lane's query
+    // |x|^2 of the three buffers, then the survivor staging
+    float* const vals = reinterpret_cast<float*>(lds + 3 * kTileBytes);
+
+    v8bf bq[KS];
+    const unsigned char* qbase = static_cast<const unsigned char*>(a.qq) + static_cast<std::size_t>(q) * kRowBytes;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bq[s] = *reinterpret_cast<const v8bf*>(qbase + (16 * s + 8 * half) * 2);
+    }
+    const bool qok = q < a.nq;
+    const float qn = qok ? a.qn[q] : 0.f;
+    constexpr float qs = 1.f;
+    const float tau = !qok ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[q] : __int_as_float(0x7f800000));
+    (void)qs;
+
+    const std::uint32_t tiles = (a.rows + NT - 1) / NT;
+    const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
+    const std::uint32_t t0 = blockIdx.y * per_slab;
+    const std::uint32_t t1 = min(tiles, t0 + per_slab);
+    if (t0 >= t1) {
+        return;
+    }
+    const std::uint32_t last = t1 - 1;
+    // Wave w's two load instructions fill LDS bytes [2w KB, 2w KB + 2 KB) of a buffer: rows
+    // 8w + lane / 16 (+4 for the second), slot lane % 16, holding chunk slot ^ (row % 16).
+    const std::uint32_t drow = 8 * wave + (lane >> 4);
+    const std::uint32_t dcol0 = ((lane & 15) ^ (drow & 15)) * 16;
+    const std::uint32_t dcol1 = ((lane & 15) ^ ((drow + 4) & 15)) * 16;
+    const unsigned char* const xq_bytes = static_cast<const unsigned char*>(a.xq);
+    // A tile past the end of the rows repeats the last row (its distances are never used).
+    const auto issue = [&](std::uint32_t tile, std::uint32_t b) {
+        unsigned char* dst = lds + b * kTileBytes + wave * 2048;
+        const std::uint32_t j0 = tile * NT + drow;
+        const std::uint32_t j1 = j0 + 4;
+        const std::uint32_t x0 = (j0 < a.rows ? j0 : a.rows - 1) * a.row_stride;
+        const std::uint32_t x1 = (j1 < a.rows ? j1 : a.rows - 1) * a.row_stride;
+        lds_load16(xq_bytes + static_cast<std::size_t>(x0) * kRowBytes + dcol0, lds_offset(dst));
+        lds_load16(xq_bytes + static_cast<std::size_t>(x1) * kRowBytes + dcol1, lds_offset(dst + 1024));
+        if (wave == 0) {
+            const std::uint32_t j = tile * NT + lane;
+            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+            lds_load4(a.xn + xr, lds_offset(vals + b * NT));
+        }
+    };
+    // Loads complete in issue order: leaving the newest tile's loads (3 on wave 0, 2 elsewhere)
+    // in flight means every older one has landed.
+    const auto wait_older = [&]() {
+        if (wave == 0) {
+            __builtin_amdgcn_s_waitcnt(0x0F73);  // vmcnt(3)
+        } else {
+            __builtin_amdgcn_s_waitcnt(0x0F72);  // vmcnt(2)
+        }
+    };
+
+    std::uint64_t* skey = reinterpret_cast<std::uint64_t*>(lds + 3 * kTileBytes + 3 * NT * 4 + wave * kStageBytes);
+    std::uint32_t* squery = reinterpret_cast<std::uint32_t*>(skey + 64);
+    std::uint32_t staged = 0;  // wave-uniform
+    const auto flush = [&]() {
+        wave_lds_fence();
+        if (static_cast<std::uint32_t>(lane) < staged) {
+            const std::uint32_t sq = squery[lane];
+            const std::uint32_t slot = atomicAdd(a.cnt + sq, 1u);
+            if (slot < a.cap) {
+                a.buf[static_cast<std::size_t>(sq) * a.cap + slot] = skey[lane];
+            }
+        }
+        wave_lds_fence();
+        staged = 0;
+    };
+
+    // The query fragments, |q|^2 and tau land before the first tile load is issued, so the
+    // compiler's own waits for them (it does not see the tile loads) never fall inside the loop.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    issue(t0, 0);
+    issue(min(t0 + 1, last), 1);
+    wait_older();
+    __syncthreads();
+    const std::uint32_t a0 = static_cast<std::uint32_t>(col) * kRowBytes +
+                             16u * (static_cast<std::uint32_t>(half) ^ (static_cast<std::uint32_t>(col) & 15u));
+    std::uint32_t tile_cb = 0;  // buffer of tile t
+    for (std::uint32_t t = t0; t < t1; ++t) {
+        const std::uint32_t tile_t = t;
+        issue(min(t + 2, last), tile_cb == 0 ? 2 : tile_cb - 1);
+        const std::uint32_t ab = a0 + tile_cb * kTileBytes;
+        v16f acc[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            acc[h] = v16f{};
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const unsigned char* p = lds + (ab ^ (32u * s));
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const v8bf x = *reinterpret_cast<const v8bf*>(p + h * 32 * kRowBytes);
+                acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
+            }
+        }
+        // Fragment reads one K step ahead of the MFMAs: reads of steps 0 and 1, then per step
+        // its two MFMAs and the next step's reads.
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+        for (int s = 0; s < KS - 2; ++s) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        if ((tile_t + 1) * NT <= a.rows) {
+            VDX_EPILOGUE(false)
+        } else {
+            VDX_EPILOGUE(true)
+        }
+        wait_older();
+        __syncthreads();
+        tile_cb = tile_cb == 2 ? 0 : tile_cb + 1;
+    }
+    if (MODE != 0 && staged > 0) {
+        flush();
+    }
+}
+
+#undef VDX_EPILOGUE
 
 // Per query (one workgroup): the `rank`-th smallest (1-based) of `count` u32 sortable
 // distances (dense sample row), by an MSB-first radix select, 8 bits per pass.
@@ -734,6 +913,8 @@ hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
     return hipGetLastError();
 }
 
+std::size_t bf_gemm_dma_lds_bytes() { return 3 * 64 * 256 + 3 * 64 * sizeof(float) + 8 * kStageBytes; }
+
 std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
     const std::size_t eb = prec == kPrecF32 ? 4 : (prec == kPrecBF16 ? 2 : 1);
     const std::size_t nt = prec == kPrecF32 ? 32 : 64;
@@ -743,30 +924,51 @@ std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
                           hipStream_t stream) {
     if (waves != 8) return hipErrorInvalidValue;
+    if (args.mode != 0 && args.row_stride != 1) return hipErrorInvalidValue;  // the filter's ids are rows
     const dim3 grid(qtiles, slabs);
     const std::size_t lds = bf_gemm_lds_bytes(prec, args.kp, waves);
     const dim3 block(64 * waves);
-#define VDX_GEMM_LAUNCH(P)                                                                        \
+#define VDX_GEMM_MODES(P, F)                                                                      \
     if (args.mode == 0) {                                                                         \
         if (args.l2) {                                                                            \
-            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, true>), grid, block, lds, stream, args);  \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, true, F>), grid, block, lds, stream, args);  \
         } else {                                                                                  \
-            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, false>), grid, block, lds, stream, args); \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 0, false, F>), grid, block, lds, stream, args); \
         }                                                                                         \
     } else {                                                                                      \
         if (args.l2) {                                                                            \
-            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, true>), grid, block, lds, stream, args);  \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, true, F>), grid, block, lds, stream, args);  \
         } else {                                                                                  \
-            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, false>), grid, block, lds, stream, args); \
+            hipLaunchKernelGGL((bf_gemm_kernel<P, 8, 1, false, F>), grid, block, lds, stream, args); \
         }                                                                                         \
     }
-    if (prec == kPrecF32) {
+#define VDX_GEMM_LAUNCH(P)                                                                        \
+    if (args.ks == bf_gemm_steps(P)) {                                                            \
+        VDX_GEMM_MODES(P, true)                                                                   \
+    } else {                                                                                      \
+        VDX_GEMM_MODES(P, false)                                                                  \
+    }
+    if (prec == kPrecBF16 && args.ks == bf_gemm_steps(kPrecBF16) && args.kp == 128) {
+        const std::size_t dma_lds = bf_gemm_dma_lds_bytes();
+        if (args.mode == 0) {
+            if (args.l2) {
+                hipLaunchKernelGGL((bf_gemm_bf16_dma_kernel<0, true>), grid, block, dma_lds, stream, args);
+            } else {
+                hipLaunchKernelGGL((bf_gemm_bf16_dma_kernel<0, false>), grid, block, dma_lds, stream, args);
+            }
+        } else if (args.l2) {
+            hipLaunchKernelGGL((bf_gemm_bf16_dma_kernel<1, true>), grid, block, dma_lds, stream, args);
+        } else {
+            hipLaunchKernelGGL((bf_gemm_bf16_dma_kernel<1, false>), grid, block, dma_lds, stream, args);
+        }
+    } else if (prec == kPrecF32) {
         VDX_GEMM_LAUNCH(kPrecF32)
     } else if (prec == kPrecBF16) {
         VDX_GEMM_LAUNCH(kPrecBF16)
     } else {
         VDX_GEMM_LAUNCH(kPrecFP8)
     }
+#undef VDX_GEMM_MODES
 #undef VDX_GEMM_LAUNCH
     return hipGetLastError();
 }

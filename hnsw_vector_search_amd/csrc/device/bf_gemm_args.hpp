@@ -35,7 +35,7 @@ struct GemmArgs {
     const float* tau;   // per-query threshold (filter mode), or null
     std::uint32_t nq, kp, ks, l2;
     std::uint32_t rows;        // rows covered: row j is base row j * row_stride
-    std::uint32_t row_stride;
+    std::uint32_t row_stride;  // 1 in the filter (mode 1)
     std::uint32_t mode;        // 0: dense store, 1: threshold filter
     float* dense;              // [nq][dense_ld]
     std::uint64_t dense_ld;

@@ -104,17 +104,16 @@ def test_mfma_modes_equal_exact_small_base(precision, metric, dim):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
-@pytest.mark.parametrize("metric", [0, 2])
-def test_mfma_modes_equal_exact_with_sampled_threshold(precision, metric):
-    """n = 60,000: the threshold pass, the filter and the certification all run."""
-    print(f"case: bf {precision} metric {metric} n 60000", flush=True)
+@pytest.mark.parametrize("metric,dim", [(0, 128), (2, 100), (1, 128), (2, 128)])
+def test_mfma_modes_equal_exact_with_sampled_threshold(precision, metric, dim):
+    """n = 60,000: the threshold pass, the filter and the certification all run (at dim 128
+    BF16 takes the kernel whose base tiles are loaded straight into LDS)."""
+    print(f"case: bf {precision} metric {metric} dim {dim} n 60000", flush=True)
     if metric == 0:
         base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 60_000, 300)
-        dim = 128
     else:
-        gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
+        gen = datasets.AnisotropicGaussian(dim, 0.8, seed=dim)
         base, queries = gen.sample(60_000, 1), gen.sample(300, 2)
-        dim = 100
     index = vdx.BruteForceIndex(dim, metric)
     index.add(base)
     for k in (1, 10, 100):
