@@ -121,21 +121,36 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
     }
 }
 
+// The filter's pre-test (L2, FP32 and BF16). A row passes the filter iff its GEMM distance
+// d = fl(fl(qn + xn) - 2 dot) (one fma) is <= tau. The pre-test computes v = fl(xn - 2 dot),
+// one fma cheaper per distance, and compares it with tq = fl(tau - qn) + 2^-20 (|tau| + qn + X),
+// X >= every xn: d <= tau gives xn - 2 dot <= tau - qn + u (|tau| + qn + xn + |xn - 2 dot|)
+// + O(u^2) (u = 2^-24), with |xn - 2 dot| <= 2 (qn + xn) up to the operands' quantization, so
+// every row that passes the filter passes the pre-test. The exact test then runs only on the
+// groups of four rows where some lane's pre-test passed, and the survivors are exactly those of
+// the plain test. Padding queries get tq = -inf.
+__device__ __forceinline__ float filter_pretest_bound(float tau, float qn, float xn_max, bool qok) {
+    if (!qok) return -__int_as_float(0x7f800000);
+    const float margin = 0x1p-20f * (fabsf(tau) + qn + xn_max);
+    return (tau - qn) + margin;
+}
+
 // The epilogue of the GEMM kernels, over the accumulators acc[NH] of base tile tile_t, whose
 // |x|^2 (and FP8 scales) sit in LDS at vals + tile_cb * NT (vals + (2 + tile_cb) * NT): rows
 // 8 * g + 4 * half + (0..3) of each 32-row MFMA tile, four at a time. MODE 0 stores the dense
 // distances; MODE 1 (always over consecutive rows, row_stride 1: a survivor's id is its row)
-// stages the survivors (distance <= tau) for the wave's next flush. Padding
-// queries never pass (tau = -inf); the row bound is tested only in a tile that crosses the end
-// of the rows.
+// stages the survivors (distance <= tau) for the wave's next flush, through the pre-test above
+// where it applies (PRETEST). Padding queries never pass (tau = tq = -inf); the row bound is
+// tested only in a tile that crosses the end of the rows.
 #define VDX_EPILOGUE(CHECK_ROWS)                                                                  \
         _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
             _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
+                constexpr bool PRETEST = MODE != 0 && L2 && PREC != kPrecFP8;                     \
                 const int r0 = h * 32 + 8 * g + 4 * half;                                         \
-                const float4 xn4 = *reinterpret_cast<const float4*>(vals + tile_cb * NT + r0);         \
+                const float4 xn4 = *reinterpret_cast<const float4*>(vals + tile_cb * NT + r0);    \
                 float4 xs4 = make_float4(1.f, 1.f, 1.f, 1.f);                                     \
                 if constexpr (PREC == kPrecFP8) {                                                 \
-                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + tile_cb) * NT + r0);            \
+                    xs4 = *reinterpret_cast<const float4*>(vals + (2 + tile_cb) * NT + r0);       \
                 }                                                                                 \
                 float d4[4];                                                                      \
                 std::uint64_t m4[4];                                                              \
@@ -146,22 +161,36 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
                     if constexpr (PREC == kPrecFP8) {                                             \
                         dot = dot * qs * xsv;                                                     \
                     }                                                                             \
-                    const std::uint32_t j = tile_t * NT + r0 + e;                                      \
-                    d4[e] = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;               \
-                    if constexpr (MODE == 0) {                                                    \
-                        if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
-                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d4[e];        \
-                        }                                                                         \
+                    const std::uint32_t j = tile_t * NT + r0 + e;                                 \
+                    if constexpr (PRETEST) {                                                      \
+                        m4[e] = __builtin_amdgcn_fcmpf(__builtin_fmaf(-2.0f, dot, xnv), tq, 5);   \
                     } else {                                                                      \
-                        /* the lanes whose distance passes, straight from the compare */          \
-                        m4[e] = __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);           \
-                        if (CHECK_ROWS) {                                                         \
-                            m4[e] &= ballot(j < a.rows);                                          \
+                        d4[e] = L2 ? __builtin_fmaf(-2.0f, dot, qn + xnv) : 1.0f - dot;           \
+                        if constexpr (MODE == 0) {                                                \
+                            if (qok && (!(CHECK_ROWS) || j < a.rows)) {                           \
+                                a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d4[e];    \
+                            }                                                                     \
+                        } else {                                                                  \
+                            /* the lanes whose distance passes, straight from the compare */      \
+                            m4[e] = __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);       \
+                            if (CHECK_ROWS) {                                                     \
+                                m4[e] &= ballot(j < a.rows);                                      \
+                            }                                                                     \
                         }                                                                         \
                     }                                                                             \
                 }                                                                                 \
                 if constexpr (MODE != 0) {                                                        \
                     if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
+                        if constexpr (PRETEST) {                                                  \
+                            _Pragma("unroll") for (int e = 0; e < 4; ++e) {                       \
+                                const float xnv = e == 0 ? xn4.x : (e == 1 ? xn4.y : (e == 2 ? xn4.z : xn4.w)); \
+                                d4[e] = __builtin_fmaf(-2.0f, acc[h][4 * g + e], qn + xnv);       \
+                                m4[e] &= __builtin_amdgcn_fcmpf(d4[e], tau, 5 /* ordered <= */);  \
+                                if (CHECK_ROWS) {                                                 \
+                                    m4[e] &= ballot(tile_t * NT + r0 + e < a.rows);               \
+                                }                                                                 \
+                            }                                                                     \
+                        }                                                                         \
                         _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
                             const std::uint64_t m = m4[e];                                        \
                             if (m != 0) {                                                         \
@@ -171,7 +200,7 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
                                 }                                                                 \
                                 if ((m >> lane) & 1u) {                                           \
                                     const std::uint32_t at = staged + lanes_below(m);             \
-                                    skey[at] = make_key(d4[e], tile_t * NT + r0 + e);              \
+                                    skey[at] = make_key(d4[e], tile_t * NT + r0 + e);             \
                                     squery[at] = q;                                               \
                                 }                                                                 \
                                 staged += np;                                                     \
@@ -202,7 +231,7 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
 // else 1 - <q, x>. FULL: the operand rows fill every K step (a.ks == KS), so the step loop has
 // no run-time guard; a guarded step makes the compiler copy all accumulators after each MFMA.
 template <int PREC, int WAVES, int MODE, bool L2, bool FULL>
-__global__ __launch_bounds__(WAVES * 64, PREC == kPrecF32 ? 2 : 4) void bf_gemm_kernel(GemmArgs a) {
+__global__ __launch_bounds__(WAVES * 64, PREC == kPrecF32 || !FULL ? 2 : 4) void bf_gemm_kernel(GemmArgs a) {
     constexpr int EB = PREC == kPrecF32 ? 4 : (PREC == kPrecBF16 ? 2 : 1);  // bytes per element
     constexpr int NT = PREC == kPrecF32 ? 32 : 64;                           // rows per tile
     constexpr int NH = NT / 32;                                              // MFMA row tiles
@@ -243,6 +272,7 @@ __global__ __launch_bounds__(WAVES * 64, PREC == kPrecF32 ? 2 : 4) void bf_gemm_
     // Padding lanes (q >= nq) get -inf, so no distance passes their filter: their candidate
     // rows would lie past the end of the buffer.
     const float tau = !qok ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[q] : __int_as_float(0x7f800000));
+    const float tq = filter_pretest_bound(tau, qn, a.xn_max, qok);
 
     const std::uint32_t tiles = (a.rows + NT - 1) / NT;
     const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
@@ -504,6 +534,7 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     const float qn = qok ? a.qn[q] : 0.f;
     constexpr float qs = 1.f;
     const float tau = !qok ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[q] : __int_as_float(0x7f800000));
+    const float tq = filter_pretest_bound(tau, qn, a.xn_max, qok);
     (void)qs;
 
     const std::uint32_t tiles = (a.rows + NT - 1) / NT;
@@ -515,30 +546,45 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     }
     const std::uint32_t last = t1 - 1;
     // Wave w's two load instructions fill LDS bytes [2w KB, 2w KB + 2 KB) of a buffer: rows
-    // 8w + lane / 16 (+4 for the second), slot lane % 16, holding chunk slot ^ (row % 16).
+    // 8w + lane / 16 (+4 for the second), slot lane % 16, holding chunk slot ^ (row % 16). The
+    // per-lane source addresses are fixed; a tile adds tile * NT * row_stride rows (a scalar).
+    const std::uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);  // m0 must be uniform
     const std::uint32_t drow = 8 * wave + (lane >> 4);
     const std::uint32_t dcol0 = ((lane & 15) ^ (drow & 15)) * 16;
     const std::uint32_t dcol1 = ((lane & 15) ^ ((drow + 4) & 15)) * 16;
     const unsigned char* const xq_bytes = static_cast<const unsigned char*>(a.xq);
+    const unsigned char* const src0 = xq_bytes + static_cast<std::size_t>(drow * a.row_stride) * kRowBytes + dcol0;
+    const unsigned char* const src1 = xq_bytes + static_cast<std::size_t>((drow + 4) * a.row_stride) * kRowBytes + dcol1;
+    const float* const xn_src = a.xn + static_cast<std::size_t>(lane) * a.row_stride;
+    const std::uint32_t lds_base = lds_offset(lds);
     // A tile past the end of the rows repeats the last row (its distances are never used).
     const auto issue = [&](std::uint32_t tile, std::uint32_t b) {
-        unsigned char* dst = lds + b * kTileBytes + wave * 2048;
-        const std::uint32_t j0 = tile * NT + drow;
-        const std::uint32_t j1 = j0 + 4;
-        const std::uint32_t x0 = (j0 < a.rows ? j0 : a.rows - 1) * a.row_stride;
-        const std::uint32_t x1 = (j1 < a.rows ? j1 : a.rows - 1) * a.row_stride;
-        lds_load16(xq_bytes + static_cast<std::size_t>(x0) * kRowBytes + dcol0, lds_offset(dst));
-        lds_load16(xq_bytes + static_cast<std::size_t>(x1) * kRowBytes + dcol1, lds_offset(dst + 1024));
-        if (wave == 0) {
-            const std::uint32_t j = tile * NT + lane;
-            const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
-            lds_load4(a.xn + xr, lds_offset(vals + b * NT));
+        const std::uint32_t dst = lds_base + b * kTileBytes + wave_u * 2048;
+        const std::size_t first = static_cast<std::size_t>(tile) * NT * a.row_stride;  // first row's index
+        if ((tile + 1) * NT <= a.rows) {
+            lds_load16(src0 + first * kRowBytes, dst);
+            lds_load16(src1 + first * kRowBytes, dst + 1024);
+            if (wave_u == 0) {
+                lds_load4(xn_src + first, lds_base + 3 * kTileBytes + b * NT * 4);
+            }
+        } else {
+            const std::uint32_t j0 = tile * NT + drow;
+            const std::uint32_t j1 = j0 + 4;
+            const std::uint32_t x0 = (j0 < a.rows ? j0 : a.rows - 1) * a.row_stride;
+            const std::uint32_t x1 = (j1 < a.rows ? j1 : a.rows - 1) * a.row_stride;
+            lds_load16(xq_bytes + static_cast<std::size_t>(x0) * kRowBytes + dcol0, dst);
+            lds_load16(xq_bytes + static_cast<std::size_t>(x1) * kRowBytes + dcol1, dst + 1024);
+            if (wave_u == 0) {
+                const std::uint32_t j = tile * NT + lane;
+                const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+                lds_load4(a.xn + xr, lds_base + 3 * kTileBytes + b * NT * 4);
+            }
         }
     };
     // Loads complete in issue order: leaving the newest tile's loads (3 on wave 0, 2 elsewhere)
     // in flight means every older one has landed.
     const auto wait_older = [&]() {
-        if (wave == 0) {
+        if (wave_u == 0) {
             __builtin_amdgcn_s_waitcnt(0x0F73);  // vmcnt(3)
         } else {
             __builtin_amdgcn_s_waitcnt(0x0F72);  // vmcnt(2)
@@ -614,35 +660,87 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
 
 #undef VDX_EPILOGUE
 
-// Per query (one workgroup): the `rank`-th smallest (1-based) of `count` u32 sortable
-// distances (dense sample row), by an MSB-first radix select, 8 bits per pass.
+// Radix selection helpers for one 256-thread workgroup (4 waves). Each wave counts into its own
+// 256-bin histogram (rows padded to 260 words so the waves' rows start on different banks); a
+// digit that every active lane of the wave shares is added once, by one lane (the first passes
+// over distances of one magnitude see mostly that case), others lane by lane.
+constexpr int kSelWaves = 4;
+constexpr int kSelRow = 260;
+
+__device__ __forceinline__ void sel_count(std::uint32_t* hist, std::uint32_t digit) {
+    const std::uint32_t d0 = __builtin_amdgcn_readfirstlane(digit);
+    const std::uint64_t same = ballot(digit == d0);
+    if (same == __builtin_amdgcn_read_exec()) {
+        if (lanes_below(same) == 0) atomicAdd(&hist[d0], static_cast<std::uint32_t>(__popcll(same)));
+    } else {
+        atomicAdd(&hist[digit], 1u);
+    }
+}
+
+// Wave 0 of the workgroup: the digit whose bin holds the want-th smallest counted key, and the
+// rank within that bin, to sel[0] and sel[1] (want <= the number of keys counted).
+__device__ __forceinline__ void sel_pick(const std::uint32_t* hist, std::uint32_t want, std::uint32_t* sel) {
+    const int lane = lane_id();
+    std::uint32_t c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        c[j] = 0;
+#pragma unroll
+        for (int w = 0; w < kSelWaves; ++w) c[j] += hist[w * kSelRow + 4 * lane + j];
+    }
+    const std::uint32_t t = c[0] + c[1] + c[2] + c[3];
+    std::uint32_t incl = t;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const std::uint32_t v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+    }
+    const std::uint64_t reach = ballot(incl >= want);
+    if (reach != 0 && lane == __builtin_ctzll(reach)) {
+        std::uint32_t cum = incl - t;
+        int j = 0;
+        for (; j < 3; ++j) {
+            if (cum + c[j] >= want) break;
+            cum += c[j];
+        }
+        sel[0] = 4u * static_cast<std::uint32_t>(lane) + static_cast<std::uint32_t>(j);
+        sel[1] = want - cum;
+    }
+}
+
+// Per query (one workgroup): the `rank`-th smallest (1-based) of `count` (<= 8192) u32 sortable
+// distances (dense sample row), by an MSB-first radix select, 8 bits per pass, over keys held in
+// registers (32 per thread).
 __global__ __launch_bounds__(256) void bf_threshold_kernel(ThresholdArgs a) {
-    __shared__ std::uint32_t hist[256];
-    __shared__ std::uint32_t sel_digit, sel_want;
+    constexpr int kPer = 32;
+    __shared__ std::uint32_t hist[kSelWaves * kSelRow];
+    __shared__ std::uint32_t sel[2];
     const std::uint32_t q = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
     const float* row = a.dense + static_cast<std::size_t>(q) * a.dense_ld;
+    std::uint32_t key[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const std::uint32_t idx = threadIdx.x + 256u * i;
+        key[i] = idx < a.count ? sortable(row[idx]) : 0u;
+    }
     std::uint32_t prefix = 0, mask = 0, want = a.rank;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        for (int i = threadIdx.x; i < kSelWaves * kSelRow; i += blockDim.x) hist[i] = 0;
         __syncthreads();
-        for (std::uint32_t i = threadIdx.x; i < a.count; i += blockDim.x) {
-            const std::uint32_t v = sortable(row[i]);
-            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            std::uint32_t cum = 0, b = 0;
-            for (; b < 255; ++b) {
-                if (cum + hist[b] >= want) break;
-                cum += hist[b];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const std::uint32_t idx = threadIdx.x + 256u * i;
+            if (idx < a.count && (key[i] & mask) == prefix) {
+                sel_count(hist + wave * kSelRow, (key[i] >> shift) & 255u);
             }
-            sel_digit = b;
-            sel_want = want - cum;
         }
         __syncthreads();
-        prefix |= sel_digit << shift;
+        if (wave == 0) sel_pick(hist, want, sel);
+        __syncthreads();
+        prefix |= sel[0] << shift;
         mask |= 255u << shift;
-        want = sel_want;
+        want = sel[1];
         __syncthreads();
     }
     if (threadIdx.x == 0) a.tau[q] = unsortable(prefix);
@@ -652,8 +750,8 @@ __global__ __launch_bounds__(256) void bf_threshold_kernel(ThresholdArgs a) {
 // keys are unique) in the shortlist, and record the smallest GEMM distance left out: the
 // (S+1)-th key's when the buffer held more than S, else tau (everything left out was > tau).
 __global__ __launch_bounds__(256) void bf_shortlist_kernel(ShortlistArgs a) {
-    __shared__ std::uint32_t hist[256];
-    __shared__ std::uint32_t sel_digit, sel_want, taken;
+    __shared__ std::uint32_t hist[kSelWaves * kSelRow];
+    __shared__ std::uint32_t sel[2], taken;
     const std::uint32_t q = blockIdx.x;
     const std::uint32_t total = a.cnt[q];
     const std::uint64_t* buf = a.buf + static_cast<std::size_t>(q) * a.cap;
@@ -680,27 +778,20 @@ __global__ __launch_bounds__(256) void bf_shortlist_kernel(ShortlistArgs a) {
     }
     std::uint64_t prefix = 0, mask = 0;
     std::uint32_t want = a.S;
+    const int wave = threadIdx.x >> 6;
     for (int shift = 56; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        for (int i = threadIdx.x; i < kSelWaves * kSelRow; i += blockDim.x) hist[i] = 0;
         __syncthreads();
         for (std::uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
             const std::uint64_t v = buf[i];
-            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+            if ((v & mask) == prefix) sel_count(hist + wave * kSelRow, static_cast<std::uint32_t>(v >> shift) & 255u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            std::uint32_t cum = 0, b = 0;
-            for (; b < 255; ++b) {
-                if (cum + hist[b] >= want) break;
-                cum += hist[b];
-            }
-            sel_digit = b;
-            sel_want = want - cum;
-        }
+        if (wave == 0) sel_pick(hist, want, sel);
         __syncthreads();
-        prefix |= static_cast<std::uint64_t>(sel_digit) << shift;
+        prefix |= static_cast<std::uint64_t>(sel[0]) << shift;
         mask |= 255ull << shift;
-        want = sel_want;
+        want = sel[1];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -974,6 +1065,7 @@ hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs,
 }
 
 hipError_t launch_bf_threshold(const ThresholdArgs& args, unsigned nq, hipStream_t stream) {
+    if (args.count > 8192 || args.rank == 0 || args.rank > args.count) return hipErrorInvalidValue;  // keys in registers
     hipLaunchKernelGGL(bf_threshold_kernel, dim3(nq), dim3(256), 0, stream, args);
     return hipGetLastError();
 }

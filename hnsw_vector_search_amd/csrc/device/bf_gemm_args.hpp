@@ -33,6 +33,7 @@ struct GemmArgs {
     const float* qn;    // ||q||^2
     const float* qs;    // FP8 scales, or null
     const float* tau;   // per-query threshold (filter mode), or null
+    float xn_max;       // >= every ||x||^2 (the filter's pre-test margin)
     std::uint32_t nq, kp, ks, l2;
     std::uint32_t rows;        // rows covered: row j is base row j * row_stride
     std::uint32_t row_stride;  // 1 in the filter (mode 1)

@@ -57,6 +57,7 @@ BfOperands& base_operands(vdx_gpu_bf& b, int prec, hipStream_t s) {
     if (op.ready) return op;
     op.kp = operand_kp(prec, b.dim);
     op.rows = (b.n + 63) / 64 * 64;
+    op.max_norm2 = op.max_qerr = op.max_qnorm = op.max_norm = 0.f;  // rows appended: made again
     op.xq.resize(op.rows * op.kp * operand_bytes(prec));
     op.norm2.resize(op.rows * 4);
     op.scale.resize(op.rows * 4);
@@ -135,11 +136,15 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
     if (sample) {
         m = static_cast<std::uint32_t>(std::min<std::uint64_t>(8192, std::max<std::uint64_t>(512, b.n / 8)));
         stride = static_cast<std::uint32_t>(b.n / m);
-        // rank >= 16 keeps the survivor count's spread small (Gamma(rank): cv = 1/sqrt(rank));
-        // the buffer holds 4x the expected survivors, so overflows (redone exactly) are rare.
-        rank = static_cast<std::uint32_t>(std::min<double>(m, std::max(16.0, std::ceil(2.0 * S * m / b.n))));
+        // The fraction of the base within tau is about Gamma(rank) / m. The shortlist and its
+        // certification are the same for any tau that keeps more than S rows; with fewer, tau
+        // itself bounds what was left out, which certifies unless barely k rows survive. rank
+        // >= 6 and two above the rank of 2S survivors make fewer than k survivors a 1e-6 event
+        // or rarer, while every survivor costs filter and shortlist work; the buffer holds 6x the
+        // expected survivors, so overflows (redone exactly) are rarer still.
+        rank = static_cast<std::uint32_t>(std::min<double>(m, std::max(6.0, std::ceil(2.0 * S * m / b.n) + 2.0)));
         const double expect = static_cast<double>(rank) * b.n / m;
-        cap = static_cast<std::uint32_t>(std::clamp<double>(std::ceil(4.0 * expect / 256) * 256, 1024, 16384));
+        cap = static_cast<std::uint32_t>(std::clamp<double>(std::ceil(6.0 * expect / 256) * 256, 1024, 16384));
         cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(cap, op.rows));
     }
 
@@ -180,6 +185,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         dev::GemmArgs g{};
         g.xq = op.xq.as<void>();
         g.xn = op.norm2.as<float>();
+        g.xn_max = op.max_norm2;
         g.xs = prec == dev::kPrecFP8 ? op.scale.as<float>() : nullptr;
         g.qq = qq.as<void>();
         g.qn = qn2.as<float>();
