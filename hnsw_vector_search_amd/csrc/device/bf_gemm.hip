@@ -11,6 +11,8 @@
 //     survivors over the whole base is a small multiple of the shortlist size.
 //  3. filter GEMM: Q x X^T on MFMA, dist = |q|^2 + |x|^2 - 2 q.x (L2) or 1 - q.x; each
 //     distance <= tau is appended (key = (distance, id)) to its query's candidate buffer.
+//     (The BF16 kernel for 128-element rows accumulates q.x - |x|^2 / 2 from an accumulator
+//     start of -|x|^2 / 2 and takes dist = |q|^2 - 2 acc.)
 //  4. shortlist: per query, radix select of the S smallest keys, and the smallest distance
 //     left out (L).
 //  5. re-rank: exact-order distances of the S candidates (the library's arithmetic, bit-equal
@@ -114,6 +116,7 @@ __global__ __launch_bounds__(256) void bf_prepare_rows(PrepArgs a) {
     qn2 = wave_sum(qn2);
     if (lane == 0) {
         a.norm2[row] = n2;
+        if (a.nhalf) a.nhalf[row] = -0.5f * n2;
         a.scale[row] = scale;
         a.qerr[row] = sqrtf(err2);
         a.qnorm[row] = sqrtf(qn2);
@@ -507,9 +510,79 @@ __device__ __forceinline__ void lds_load4(const void* src, std::uint32_t m0) {
     asm volatile("global_load_lds_dword %0, off" ::"v"(src), "{m0}"(m0));
 }
 
+// The filter's accept bound for the LDS-fed kernel. Its accumulator starts at -|x|^2 / 2 under
+// L2 (at 0 otherwise), so the MFMA chain leaves a = q~.x~ - |x|^2 / 2 and the GEMM distance is
+// f(a) = fl(qn - 2a), one fma (fl(1 - a) for inner product and cosine). f never increases with
+// a, so f(a) <= tau iff a >= a*, the smallest float with f(a*) <= tau: one compare per distance.
+// a* comes from a bisection over the floats in order (32 steps, once per lane); NaN, which
+// nothing passes, for tau = -inf (padding queries) and -inf for tau = +inf (no threshold).
+__device__ float filter_accept_bound(float tau, float qn, bool l2) {
+    if (!(tau > -__int_as_float(0x7f800000))) return __int_as_float(0x7fc00000);
+    if (tau == __int_as_float(0x7f800000)) return -__int_as_float(0x7f800000);
+    const auto f = [&](float x) { return l2 ? __builtin_fmaf(-2.0f, x, qn) : 1.0f - x; };
+    std::uint32_t lo = sortable(-3.40282347e38f), hi = sortable(3.40282347e38f);
+    if (f(unsortable(lo)) <= tau) return -__int_as_float(0x7f800000);
+    if (f(unsortable(hi)) > tau) return __int_as_float(0x7fc00000);
+    while (hi - lo > 1) {  // f(lo) > tau >= f(hi)
+        const std::uint32_t mid = lo + (hi - lo) / 2;
+        if (f(unsortable(mid)) <= tau) {
+            hi = mid;
+        } else {
+            lo = mid;
+        }
+    }
+    return unsortable(hi);
+}
+
+// The LDS-fed kernel's epilogue over acc[NH] of tile tile_t: MODE 0 stores f(acc); MODE 1
+// stages the rows with acc >= a* (f(acc) <= tau) with key (f(acc), row). Lanes and rows as in
+// VDX_EPILOGUE.
+#define VDX_DMA_EPILOGUE(CHECK_ROWS)                                                              \
+        _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
+            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
+                const int r0 = h * 32 + 8 * g + 4 * half;                                         \
+                if constexpr (MODE == 0) {                                                        \
+                    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                               \
+                        const float acc_e = acc[h][4 * g + e];                                    \
+                        const float d = L2 ? __builtin_fmaf(-2.0f, acc_e, qn) : 1.0f - acc_e;     \
+                        const std::uint32_t j = tile_t * NT + r0 + e;                             \
+                        if (qok && (!(CHECK_ROWS) || j < a.rows)) {                               \
+                            a.dense[static_cast<std::size_t>(q) * a.dense_ld + j] = d;            \
+                        }                                                                         \
+                    }                                                                             \
+                } else {                                                                          \
+                    std::uint64_t m4[4];                                                          \
+                    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                               \
+                        m4[e] = __builtin_amdgcn_fcmpf(acc[h][4 * g + e], astar, 3 /* ordered >= */); \
+                        if (CHECK_ROWS) {                                                         \
+                            m4[e] &= ballot(tile_t * NT + r0 + e < a.rows);                       \
+                        }                                                                         \
+                    }                                                                             \
+                    if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
+                        _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
+                            const std::uint64_t m = m4[e];                                        \
+                            if (m != 0) {                                                         \
+                                const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m)); \
+                                if (staged + np > 64) {                                           \
+                                    flush();                                                      \
+                                }                                                                 \
+                                if ((m >> lane) & 1u) {                                           \
+                                    const float acc_e = acc[h][4 * g + e];                        \
+                                    const float d = L2 ? __builtin_fmaf(-2.0f, acc_e, qn) : 1.0f - acc_e; \
+                                    const std::uint32_t at = staged + lanes_below(m);             \
+                                    skey[at] = make_key(d, tile_t * NT + r0 + e);                 \
+                                    squery[at] = q;                                               \
+                                }                                                                 \
+                                staged += np;                                                     \
+                            }                                                                     \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+            }                                                                                     \
+        }
+
 template <int MODE, bool L2>
 __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
-    constexpr int PREC = kPrecBF16;
     constexpr int NT = 64;
     constexpr int NH = 2;
     constexpr int KS = 8;
@@ -521,7 +594,7 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     const int col = lane & 31;
     const int half = lane >> 5;
     const std::uint32_t q = blockIdx.x * 256 + wave * 32 + col;  // this lane's query
-    // |x|^2 of the three buffers, then the survivor staging
+    // -|x|^2 / 2 of the three buffers (L2), then the survivor staging
     float* const vals = reinterpret_cast<float*>(lds + 3 * kTileBytes);
 
     v8bf bq[KS];
@@ -532,10 +605,8 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     }
     const bool qok = q < a.nq;
     const float qn = qok ? a.qn[q] : 0.f;
-    constexpr float qs = 1.f;
     const float tau = !qok ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[q] : __int_as_float(0x7f800000));
-    const float tq = filter_pretest_bound(tau, qn, a.xn_max, qok);
-    (void)qs;
+    const float astar = MODE == 0 ? 0.f : filter_accept_bound(tau, qn, L2);
 
     const std::uint32_t tiles = (a.rows + NT - 1) / NT;
     const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
@@ -548,6 +619,7 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     // Wave w's two load instructions fill LDS bytes [2w KB, 2w KB + 2 KB) of a buffer: rows
     // 8w + lane / 16 (+4 for the second), slot lane % 16, holding chunk slot ^ (row % 16). The
     // per-lane source addresses are fixed; a tile adds tile * NT * row_stride rows (a scalar).
+    // Under L2 wave 0 also loads the tile's -|x|^2 / 2.
     const std::uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);  // m0 must be uniform
     const std::uint32_t drow = 8 * wave + (lane >> 4);
     const std::uint32_t dcol0 = ((lane & 15) ^ (drow & 15)) * 16;
@@ -555,7 +627,7 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     const unsigned char* const xq_bytes = static_cast<const unsigned char*>(a.xq);
     const unsigned char* const src0 = xq_bytes + static_cast<std::size_t>(drow * a.row_stride) * kRowBytes + dcol0;
     const unsigned char* const src1 = xq_bytes + static_cast<std::size_t>((drow + 4) * a.row_stride) * kRowBytes + dcol1;
-    const float* const xn_src = a.xn + static_cast<std::size_t>(lane) * a.row_stride;
+    const float* const xh_src = a.xh + static_cast<std::size_t>(lane) * a.row_stride;
     const std::uint32_t lds_base = lds_offset(lds);
     // A tile past the end of the rows repeats the last row (its distances are never used).
     const auto issue = [&](std::uint32_t tile, std::uint32_t b) {
@@ -564,8 +636,8 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
         if ((tile + 1) * NT <= a.rows) {
             lds_load16(src0 + first * kRowBytes, dst);
             lds_load16(src1 + first * kRowBytes, dst + 1024);
-            if (wave_u == 0) {
-                lds_load4(xn_src + first, lds_base + 3 * kTileBytes + b * NT * 4);
+            if (L2 && wave_u == 0) {
+                lds_load4(xh_src + first, lds_base + 3 * kTileBytes + b * NT * 4);
             }
         } else {
             const std::uint32_t j0 = tile * NT + drow;
@@ -574,17 +646,17 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
             const std::uint32_t x1 = (j1 < a.rows ? j1 : a.rows - 1) * a.row_stride;
             lds_load16(xq_bytes + static_cast<std::size_t>(x0) * kRowBytes + dcol0, dst);
             lds_load16(xq_bytes + static_cast<std::size_t>(x1) * kRowBytes + dcol1, dst + 1024);
-            if (wave_u == 0) {
+            if (L2 && wave_u == 0) {
                 const std::uint32_t j = tile * NT + lane;
                 const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
-                lds_load4(a.xn + xr, lds_base + 3 * kTileBytes + b * NT * 4);
+                lds_load4(a.xh + xr, lds_base + 3 * kTileBytes + b * NT * 4);
             }
         }
     };
-    // Loads complete in issue order: leaving the newest tile's loads (3 on wave 0, 2 elsewhere)
-    // in flight means every older one has landed.
+    // Loads complete in issue order: leaving the newest tile's loads (3 on wave 0 under L2, 2
+    // otherwise) in flight means every older one has landed.
     const auto wait_older = [&]() {
-        if (wave_u == 0) {
+        if (L2 && wave_u == 0) {
             __builtin_amdgcn_s_waitcnt(0x0F73);  // vmcnt(3)
         } else {
             __builtin_amdgcn_s_waitcnt(0x0F72);  // vmcnt(2)
@@ -621,10 +693,22 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
         const std::uint32_t tile_t = t;
         issue(min(t + 2, last), tile_cb == 0 ? 2 : tile_cb - 1);
         const std::uint32_t ab = a0 + tile_cb * kTileBytes;
+        // accumulator element 4g + e of MFMA tile h is row h * 32 + 8g + 4 * half + e
         v16f acc[NH];
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
-            acc[h] = v16f{};
+            if constexpr (L2) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const float4 c = *reinterpret_cast<const float4*>(vals + tile_cb * NT + h * 32 + 8 * g + 4 * half);
+                    acc[h][4 * g + 0] = c.x;
+                    acc[h][4 * g + 1] = c.y;
+                    acc[h][4 * g + 2] = c.z;
+                    acc[h][4 * g + 3] = c.w;
+                }
+            } else {
+                acc[h] = v16f{};
+            }
         }
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
@@ -635,19 +719,10 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
                 acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[s], acc[h], 0, 0, 0);
             }
         }
-        // Fragment reads one K step ahead of the MFMAs: reads of steps 0 and 1, then per step
-        // its two MFMAs and the next step's reads.
-        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-#pragma unroll
-        for (int s = 0; s < KS - 2; ++s) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
         if ((tile_t + 1) * NT <= a.rows) {
-            VDX_EPILOGUE(false)
+            VDX_DMA_EPILOGUE(false)
         } else {
-            VDX_EPILOGUE(true)
+            VDX_DMA_EPILOGUE(true)
         }
         wait_older();
         __syncthreads();
@@ -658,6 +733,7 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     }
 }
 
+#undef VDX_DMA_EPILOGUE
 #undef VDX_EPILOGUE
 
 // Radix selection helpers for one 256-thread workgroup (4 waves). Each wave counts into its own
@@ -923,12 +999,15 @@ __global__ __launch_bounds__(256) void bf_rerank_kernel(RerankArgs a) {
         // Exact GEMM: integral operands and lossless quantization; every partial sum of
         // products is an integer of magnitude <= |q||x| < 2^24 (Cauchy-Schwarz), |q|^2 + |x|^2 <
         // 2^24, and the distance itself is then rounded once: exact when below 2^24, and >= 2^24
-        // (so beyond a k-th distance below 2^24) otherwise. The S smallest GEMM keys then
-        // contain the k smallest exact ones: certified without a margin.
+        // (so beyond a k-th distance below 2^24) otherwise. Under L2 the LDS-fed BF16 kernel
+        // accumulates from -|x|^2 / 2: its partial sums are multiples of 1/2 of magnitude <=
+        // |q||x| + |x|^2 / 2, exact below 2^23. The S smallest GEMM keys then contain the k
+        // smallest exact ones: certified without a margin.
         const float two24 = 16777216.0f;
         const bool exact_gemm = a.x_integral && a.q_integral[q] && got == want && want > 0 &&
                                 a.q_qnorm[q] * a.x_max_qnorm < two24 - 2.0f &&
-                                (!a.l2 || a.q_norm2[q] + a.x_max_norm2 < two24) &&
+                                (!a.l2 || (a.q_norm2[q] + a.x_max_norm2 < two24 &&
+                                           a.q_qnorm[q] * a.x_max_qnorm + 0.5f * a.x_max_norm2 < 0.5f * two24 - 2.0f)) &&
                                 fabsf(key_distance(keys[want - 1])) < two24;
         if (ok && !exact_gemm) {
             const float left = a.left[q];
@@ -941,7 +1020,8 @@ __global__ __launch_bounds__(256) void bf_rerank_kernel(RerankArgs a) {
                                     a.gamma_k * qnorm * a.x_max_qnorm;
                 float lower;
                 if (a.l2) {
-                    const float e = 2.0f * e_dot + (a.gamma_d + 2.4e-7f) *
+                    // gamma_k |x|^2: the accumulation from -|x|^2 / 2 of the LDS-fed kernel
+                    const float e = 2.0f * e_dot + a.gamma_k * a.x_max_norm2 + (a.gamma_d + 2.4e-7f) *
                                     (a.q_norm2[q] + a.x_max_norm2 + 2.0f * qnorm * a.x_max_qnorm);
                     lower = left * (1.0f - a.gamma_d) - e * (1.0f + a.gamma_d);
                 } else {

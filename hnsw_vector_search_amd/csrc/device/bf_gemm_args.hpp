@@ -19,6 +19,7 @@ struct PrepArgs {
     int prec;
     void* dst;      // rows x kp operands
     float* norm2;   // ||v||^2
+    float* nhalf;   // -||v||^2 / 2 (the same float halved), or null
     float* scale;   // FP8 per-vector scale (1 otherwise)
     float* qerr;    // ||v~ - v||
     float* qnorm;   // ||v~||
@@ -28,6 +29,7 @@ struct PrepArgs {
 struct GemmArgs {
     const void* xq;     // base operands, rows of kp
     const float* xn;    // ||x||^2
+    const float* xh;    // -||x||^2 / 2: the accumulator start of the BF16 LDS-fed kernel (L2)
     const float* xs;    // FP8 scales, or null
     const void* qq;     // query operands, rows of kp (a multiple of 128 rows, zero padded)
     const float* qn;    // ||q||^2

@@ -33,7 +33,7 @@ int rerank_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 
 
 void prepare(const float* src, std::uint32_t ld_src, std::uint64_t rows, std::uint64_t dim,
              std::uint32_t kp, int prec, void* dst, float* n2, float* sc, float* qe, float* qn,
-             std::uint32_t* integral, hipStream_t s) {
+             std::uint32_t* integral, hipStream_t s, float* nhalf = nullptr) {
     if (rows == 0) return;
     dev::PrepArgs p{};
     p.src = src;
@@ -44,6 +44,7 @@ void prepare(const float* src, std::uint32_t ld_src, std::uint64_t rows, std::ui
     p.prec = prec;
     p.dst = dst;
     p.norm2 = n2;
+    p.nhalf = nhalf;
     p.scale = sc;
     p.qerr = qe;
     p.qnorm = qn;
@@ -60,17 +61,18 @@ BfOperands& base_operands(vdx_gpu_bf& b, int prec, hipStream_t s) {
     op.max_norm2 = op.max_qerr = op.max_qnorm = op.max_norm = 0.f;  // rows appended: made again
     op.xq.resize(op.rows * op.kp * operand_bytes(prec));
     op.norm2.resize(op.rows * 4);
+    op.nhalf.resize(op.rows * 4);
     op.scale.resize(op.rows * 4);
     op.qerr.resize(op.rows * 4);
     op.qnorm.resize(op.rows * 4);
     op.integral.resize(op.rows * 4);
     check(hipMemsetAsync(op.xq.as<void>(), 0, op.xq.bytes(), s), "hipMemsetAsync");
-    for (DeviceBuffer* d : {&op.norm2, &op.scale, &op.qerr, &op.qnorm, &op.integral}) {
+    for (DeviceBuffer* d : {&op.norm2, &op.nhalf, &op.scale, &op.qerr, &op.qnorm, &op.integral}) {
         check(hipMemsetAsync(d->as<void>(), 0, d->bytes(), s), "hipMemsetAsync");
     }
     prepare(b.vectors.as<float>(), b.split.ld, b.n, b.dim, op.kp, prec, op.xq.as<void>(),
             op.norm2.as<float>(), op.scale.as<float>(), op.qerr.as<float>(), op.qnorm.as<float>(),
-            op.integral.as<std::uint32_t>(), s);
+            op.integral.as<std::uint32_t>(), s, op.nhalf.as<float>());
     std::vector<float> n2(b.n), qe(b.n), qn(b.n);
     std::vector<std::uint32_t> integral(b.n);
     check(hipMemcpyAsync(integral.data(), op.integral.as<void>(), b.n * 4, hipMemcpyDeviceToHost, s), "copy");
@@ -186,6 +188,7 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         g.xq = op.xq.as<void>();
         g.xn = op.norm2.as<float>();
         g.xn_max = op.max_norm2;
+        g.xh = op.nhalf.as<float>();
         g.xs = prec == dev::kPrecFP8 ? op.scale.as<float>() : nullptr;
         g.qq = qq.as<void>();
         g.qn = qn2.as<float>();

@@ -41,7 +41,7 @@ struct BfOperands {
     bool ready = false;
     std::uint32_t kp = 0;      // operand row length (elements)
     std::uint64_t rows = 0;    // padded row count (multiple of 64)
-    DeviceBuffer xq, norm2, scale, qerr, qnorm, integral;
+    DeviceBuffer xq, norm2, nhalf, scale, qerr, qnorm, integral;
     bool all_integral = false;
     float max_qerr = 0.f, max_norm = 0.f, max_norm2 = 0.f, max_qnorm = 0.f;
 };
