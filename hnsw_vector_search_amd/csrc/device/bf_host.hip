@@ -18,7 +18,10 @@ constexpr std::uint32_t kMaxShortlist = 1024;  // re-rank sorts in LDS: 8 KB per
 constexpr std::uint64_t kQueryChunk = 16384;
 constexpr int kGemmWaves = 8;  // wavefronts (32 queries each) sharing one staged base tile
 
+// Operand row length: a multiple of the MFMA K step. BF16 rows of 97 to 128 elements are padded
+// to 128 (at most a third more MFMA work), which selects the LDS-fed kernel.
 std::uint32_t operand_kp(int prec, std::uint64_t dim) {
+    if (prec == dev::kPrecBF16 && dim > 96 && dim <= 128) return 128;
     return prec == dev::kPrecF32 ? static_cast<std::uint32_t>((dim + 3) / 4 * 4)
                                  : static_cast<std::uint32_t>((dim + 15) / 16 * 16);
 }
