@@ -22,9 +22,12 @@ the threshold calibrated on 10k further queries per base for recall 0.955 at ef 
 then timed on the 10k evaluation queries. The headline `value` is the learned QPS when its
 evaluation recall is >= 0.95, else the fixed-ef QPS.
 
-roofline: algorithmic bytes of the headline beam launch (query, link blocks read, vectors of
+roofline: algorithmic bytes of the headline beam launch (query, link blocks read, rows of
 evaluated neighbours, results, from the kernel's counters in a counting pass with an 8K-slot
-visited cache) over its measured launch time, against 8 TB/s. cpu_baseline: the oracle's C
+visited cache) over its measured launch time, against 8 TB/s. This base's values are integers
+in [0, 255], so the device index also keeps its rows as bytes and the plain search reads those
+(128 bytes per row instead of 512; the same floats, so the same results);
+`headline_over_f32_rows` times the same launches over the FP32 rows. cpu_baseline: the oracle's C
 restatement of vecdex's search_batch (AVX2 order) on this machine's cores, fixed-ef operating
 point, vecdex's protocol (one warmup, median of 5).
 """
@@ -138,15 +141,16 @@ def threshold_recall(found, truth):
     return r.mean_threshold_recall_at_k(found, truth["distance"][:, K - 1], K)
 
 
-def measured_traffic(mode, ef, nq):
+def measured_traffic(mode, ef, nq, rows):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), when it covers this
-    workload; else None."""
+    workload (rows: "u8" for the 8-bit row copy, "f32"); else None."""
     path = ROOT / "profiles" / "hnsw_beam_traffic.json"
     if not path.exists():
         return None
     for row in json.loads(path.read_text()).get("workloads", []):
-        if row.get("mode") == mode and row.get("ef") == ef and row.get("nq") == nq:
+        if (row.get("mode") == mode and row.get("ef") == ef and row.get("nq") == nq
+                and row.get("rows", "f32") == rows):
             return row.get("traffic_bytes")
     return None
 
@@ -261,12 +265,34 @@ def main(argv=None) -> int:
     counts = run.counters(headline["ef"], headline["opts"], visited_log2=13)
     expansions, distances, upper_blocks, dups = counts[:4]
     d, M, nq = 128, 16, args.nq
+    # The plain search reads the 8-bit row copy when the index keeps one (every stored value an
+    # integer in [0, 255]: the same floats, so the same results): d bytes per row, else 4d.
+    rows = "u8" if index.gpu_u8_rows() and headline["mode"] == "fixed-ef" else "f32"
+    row_bytes = d if rows == "u8" else 4 * d
     alg_bytes = (nq * 4 * d + upper_blocks * 4 * (1 + M) + expansions * 4 * (1 + 2 * M)
-                 + (distances - dups) * 4 * d + nq * 8 * K)
+                 + (distances - dups) * row_bytes + nq * 8 * K)
     achieved = alg_bytes / (ms / 1000.0) / 1e9
-    log(f"headline ({headline['mode']}, ef {headline['ef']}): recall {rec:.4f}, {ms:.3f} ms, "
+    log(f"headline ({headline['mode']}, ef {headline['ef']}, {rows} rows): recall {rec:.4f}, {ms:.3f} ms, "
         f"{qps:,.0f} QPS; {expansions / nq:.1f} expansions, {(distances - dups) / nq:.1f} distances, "
         f"{alg_bytes / nq / 1e3:.1f} KB per query -> {achieved:.0f} GB/s")
+    f32_leg = None
+    if rows == "u8":
+        # the same launches over the FP32 rows: the kernel the row copy replaces
+        f32 = nat.SearchOpts()
+        f32.row_format = 1
+        ms32, _ = run.timed(headline["ef"], args.steps, args.warmup, f32)
+        res32 = run.results()
+        counts32 = run.counters(headline["ef"], f32, visited_log2=13)
+        bytes32 = (nq * 4 * d + counts32[2] * 4 * (1 + M) + counts32[0] * 4 * (1 + 2 * M)
+                   + (counts32[1] - counts32[3]) * 4 * d + nq * 8 * K)
+        f32_leg = {"ms": round(ms32, 4), "qps": round(nq / ms32 * 1000, 1),
+                   "identical_to_u8_rows": bool(res32.tobytes() == res.tobytes()),
+                   "achieved_gbs": round(bytes32 / (ms32 / 1000.0) / 1e9, 1),
+                   "frac": round(bytes32 / (ms32 / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                   "traffic": measured_traffic(headline["mode"], headline["ef"], nq, "f32"),
+                   "algorithmic_bytes_per_launch": bytes32}
+        log(f"same ef over the FP32 rows: {ms32:.3f} ms, {nq / ms32 * 1000:,.0f} QPS "
+            f"({ms32 / ms:.2f}x the 8-bit rows' time), results identical: {f32_leg['identical_to_u8_rows']}")
 
     # The host entry point at the same ef (HnswIndex::search_batch's shape: queries from and
     # results to host memory, so PCIe is inside the time): warmup + median of 5.
@@ -323,12 +349,13 @@ def main(argv=None) -> int:
                    "recall@10": round(rec, 4), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(headline["mode"], headline["ef"], nq),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "traffic": measured_traffic(headline["mode"], headline["ef"], nq, rows),
+                     "algorithmic_bytes_per_launch": alg_bytes, "rows": rows},
         "cpu_baseline": cpu,
         "fixed_ef": {"ef": fixed["ef"], "recall": fixed["recall"], "qps": fixed["qps"],
                      "best_fine_ef": fixed_best["ef"], "best_fine_qps": fixed_best["qps"],
                      "best_fine_recall": fixed_best["recall"]},
+        "headline_over_f32_rows": f32_leg,
         "learned": learned_leg,
         "sweep": sweep,
         "bruteforce": bf_leg,

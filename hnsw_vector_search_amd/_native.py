@@ -70,7 +70,8 @@ class GraphReport(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     _fields_ = [("visited_log2", c_u32), ("waves_per_group", c_u32), ("termination", c_u32),
                 ("min_steps", c_u32), ("mlp", c_ptr), ("mlp_threshold", c_float),
-                ("reserved_f", c_float), ("stats", c_ptr)]
+                ("reserved_f", c_float), ("stats", c_ptr), ("row_format", c_u32),
+                ("reserved_u", c_u32)]
 
 
 class BatchClass(ctypes.Structure):
@@ -128,6 +129,7 @@ _SIGS = {
     "vdx_gpu_hnsw_update": (c_int, [c_ptr, ctypes.POINTER(GraphView), c_ptr, c_u64]),
     "vdx_gpu_hnsw_load": (c_int, [c_ptr, c_char_p, c_ptr]),
     "vdx_gpu_hnsw_info": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "vdx_gpu_hnsw_u8_rows": (c_int, [c_ptr, c_ptr]),
     "vdx_gpu_hnsw_build_add": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u64, c_u64, ctypes.POINTER(BuildPlan),
                                        c_u64, ctypes.POINTER(BuildStats)]),
     "vdx_gpu_hnsw_free": (None, [c_ptr]),

@@ -25,6 +25,8 @@ struct BeamArgs {
     std::uint32_t n, max_level, entry_point, l2;  // l2: 1 for L2, 0 for 1 - <a, b>
     // dimension split in the distance order: nb 32-blocks, n8 8-blocks, tail elements
     std::uint32_t nb, n8, tail, first_tail;
+    // D128U8: the rows as bytes, 128 per row in the beam's lane order (pack_rows_u8)
+    const unsigned char* rows_u8;
     // the queries (prepared), per-query k and ef
     const float* queries;
     std::uint32_t qld, nq;
@@ -70,7 +72,9 @@ constexpr int kBeamStats = 6;
 
 enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
 
-enum class BeamShape { D128, D100, Generic };
+// D128U8: dimension 128 with the rows read from the 8-bit copy (BeamArgs::rows_u8); only the
+// register-resident plain search has that variant, the other kernels take D128's.
+enum class BeamShape { D128, D100, Generic, D128U8 };
 
 // Bytes of LDS one wave uses for a given R capacity and visited-cache size.
 inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {

@@ -14,9 +14,11 @@ namespace vdx::gpu {
 namespace {
 
 // 2^11 slots up to R capacity 512: a larger cache costs more in LDS occupancy (at capacity 512 a
-// 2^12 cache leaves one 4-wave workgroup per CU) than its fewer recomputed distances save.
-std::uint32_t default_visited_log2(std::uint32_t ef_cap) {
-    if (ef_cap <= 512) return 11;
+// 2^12 cache leaves one 4-wave workgroup per CU) than its fewer recomputed distances save. Over
+// the 8-bit rows, where a recomputed distance costs a quarter of the bytes, 2^10 (five
+// workgroups per CU instead of four) is faster (measured on the 1M x 128 base at ef 36 and 64).
+std::uint32_t default_visited_log2(std::uint32_t ef_cap, bool u8_rows) {
+    if (ef_cap <= 512) return u8_rows ? 10 : 11;
     return 13;
 }
 
@@ -49,6 +51,8 @@ BeamConfig config_from(const vdx_search_opts* opts) {
     if (opts != nullptr) {
         c.visited_log2 = opts->visited_log2;
         c.waves = opts->waves_per_group;
+        vdx::require(opts->row_format <= 1, "row_format must be 0 (8-bit rows where kept) or 1 (FP32 rows)");
+        c.f32_rows = opts->row_format == 1;
     }
     return c;
 }
@@ -67,7 +71,12 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
         p.chunks = 0;
         p.ef_cap = need * 64u;
     }
-    p.visited_log2 = cfg.visited_log2 ? cfg.visited_log2 : default_visited_log2(p.ef_cap);
+    p.shape = beam_shape(h.dim);
+    if (p.shape == vdx::dev::BeamShape::D128 && h.u8 && p.chunks > 0 && mode == vdx::dev::kBeamPlain && !cfg.f32_rows) {
+        p.shape = vdx::dev::BeamShape::D128U8;
+    }
+    p.visited_log2 = cfg.visited_log2 ? cfg.visited_log2
+                                      : default_visited_log2(p.ef_cap, p.shape == vdx::dev::BeamShape::D128U8);
     vdx::require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
     p.waves = cfg.waves ? cfg.waves : 4;
     vdx::require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
@@ -80,8 +89,8 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     }
     p.lds_bytes = p.lds_per_wave * p.waves + shared;
     vdx::require(p.lds_bytes <= limit, "ef too large for the GPU beam's LDS");
-    const void* kernel = p.chunks > 0 ? vdx::dev::hnsw_beam_reg_kernel_ptr(beam_shape(h.dim), mode, p.chunks)
-                                      : vdx::dev::hnsw_beam_kernel_ptr(beam_shape(h.dim), mode);
+    const void* kernel = p.chunks > 0 ? vdx::dev::hnsw_beam_reg_kernel_ptr(p.shape, mode, p.chunks)
+                                      : vdx::dev::hnsw_beam_kernel_ptr(p.shape, mode);
     int per_cu = 0;
     check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, p.waves * 64, p.lds_bytes),
           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
@@ -103,6 +112,7 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
     vdx::require(nq <= 0xFFFFFFFFull, "too many queries in one call");
     vdx::dev::BeamArgs a{};
     a.vectors = h.vectors.as<float>();
+    a.rows_u8 = plan.shape == vdx::dev::BeamShape::D128U8 ? h.rows_u8.as<unsigned char>() : nullptr;
     a.layer0 = h.layer0.as<std::uint32_t>();
     a.upper = h.upper.as<std::uint32_t>();
     a.upper_offset = h.upper_offset.as<std::uint32_t>();
@@ -149,11 +159,11 @@ void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vd
                  hipStream_t stream) {
     if (plan.chunks > 0) {
         check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                             beam_shape(h.dim), mode, plan.chunks),
+                                             plan.shape, mode, plan.chunks),
               "hnsw_beam_reg launch");
     } else {
         check(vdx::dev::launch_hnsw_beam(a, plan.grid, plan.waves, plan.lds_bytes, stream,
-                                         beam_shape(h.dim), mode),
+                                         plan.shape, mode),
               "hnsw_beam launch");
     }
 }

@@ -18,12 +18,14 @@ struct BeamConfig {
     std::uint32_t visited_log2 = 0;   // visited-cache slots per wave, log2
     std::uint32_t waves = 0;          // wavefronts per workgroup (1..4)
     std::uint32_t blocks_per_cu = 0;  // resident workgroups per CU to fill (queries in flight)
+    bool f32_rows = false;            // never the 8-bit row copy
 };
 
 struct BeamPlan {
     std::uint32_t ef_cap, visited_log2, waves, grid;
     int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
     std::size_t lds_per_wave, lds_bytes;
+    vdx::dev::BeamShape shape;  // D128U8: the plain register kernel over the 8-bit row copy
 };
 
 vdx::dev::BeamShape beam_shape(std::uint64_t dim);

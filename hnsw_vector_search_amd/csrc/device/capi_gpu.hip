@@ -80,6 +80,39 @@ void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::u
 }
 
 // Queues one beam launch over prepared device queries.
+// Keeps h's 8-bit row copy current after rows [first, h.n) changed: dimension 128 only, and only
+// while every stored value is an integer in [0, 255] (checked on the device as the rows are
+// packed); otherwise the copy is dropped and the search reads the FP32 rows. The buffer grows
+// with a quarter of headroom, keeping the rows before `first`. Synchronizes the stream.
+void refresh_rows_u8(vdx_gpu_hnsw& h, std::uint64_t first, hipStream_t s) {
+    if (h.dim != 128 || h.metric == 2 || (first > 0 && !h.u8) || h.n == 0) {
+        h.u8 = false;
+        h.rows_u8.release();
+        return;
+    }
+    if (first == h.n) return;  // no new rows
+    const std::uint64_t need = h.n * 128;
+    if (need > h.rows_u8.bytes()) {
+        DeviceBuffer grown(need + need / 4);
+        if (first > 0) {
+            check(hipMemcpyAsync(grown.as<void>(), h.rows_u8.as<void>(), first * 128, hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync(grow rows_u8)");
+        }
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(grow rows_u8)");
+        h.rows_u8.swap(grown);
+    }
+    h.u8_flag.reserve(4);
+    check(vdx::dev::launch_fill_u32(h.u8_flag.as<std::uint32_t>(), 0, 1, s), "fill(u8 flag)");
+    check(vdx::dev::launch_pack_rows_u8(h.rows_u8.as<unsigned char>(), h.vectors.as<float>(), h.split.ld, first,
+                                        h.n - first, h.u8_flag.as<std::uint32_t>(), s),
+          "pack_rows_u8 launch");
+    std::uint32_t bad = 1;
+    check(hipMemcpyAsync(&bad, h.u8_flag.as<void>(), 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(u8 flag)");
+    check(hipStreamSynchronize(s), "hipStreamSynchronize(pack rows_u8)");
+    h.u8 = bad == 0;
+    if (!h.u8) h.rows_u8.release();
+}
+
 void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, const std::uint32_t* d_ks,
                   std::uint32_t k, const std::uint32_t* d_efs, std::uint32_t ef,
                   std::uint32_t max_efp, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
@@ -290,6 +323,7 @@ void update_index(vdx_gpu_hnsw* h, const vdx_graph_view* g, const std::uint32_t*
     h->upper_blocks = g->upper_blocks;
     h->max_level = g->max_level;
     h->entry_point = g->entry_point;
+    refresh_rows_u8(*h, old_n, s);
 }
 
 // The GPU as the searcher of a batched build: the device copy kept current by update_index, each
@@ -595,6 +629,7 @@ int vdx_gpu_hnsw_upload(vdx_gpu_ctx* ctx, const vdx_graph_view* g, vdx_gpu_hnsw*
               "hipMemcpyAsync(offsets)");
         h->ring.resize(kRingSlots * 256);
         check(hipStreamSynchronize(s), "hipStreamSynchronize(upload)");
+        refresh_rows_u8(*h, 0, s);
         *out = h.release();
     });
 }
@@ -709,6 +744,7 @@ int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out) {
         check(hipStreamSynchronize(s), "hipStreamSynchronize(load)");
         h->levels = verify.levels();
         h->offsets = verify.offsets();
+        refresh_rows_u8(*h, 0, s);
         *out = h.release();
     });
 }
@@ -720,6 +756,14 @@ int vdx_gpu_hnsw_info(const vdx_gpu_hnsw* h, uint32_t* metric, uint64_t* dim, ui
         if (dim) *dim = h->dim;
         if (n) *n = h->n;
         if (M) *M = h->M;
+    });
+}
+
+int vdx_gpu_hnsw_u8_rows(const vdx_gpu_hnsw* h, uint32_t* kept) {
+    return guarded([&] {
+        require(h != nullptr, "vdx_gpu_hnsw_u8_rows: index is null");
+        require(kept != nullptr, "vdx_gpu_hnsw_u8_rows: kept is null");
+        *kept = h->u8 ? 1u : 0u;
     });
 }
 

@@ -59,6 +59,34 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(float* dst, std::uint3
     }
 }
 
+// Rows [first, first + rows) of `src` (128 floats used of each row of ld) as 128 bytes each in
+// dst, in the beam's lane order: byte 16m + 4i + j of a row holds element 32i + 4m + j. One
+// thread per 4 elements; *bad is set when an element is not an integer in [0, 255] (the copy is
+// then of no use and is dropped by the caller).
+__global__ __launch_bounds__(256) void pack_rows_u8_kernel(unsigned char* dst, const float* src, std::uint32_t ld,
+                                                           std::uint64_t first, std::uint64_t rows,
+                                                           std::uint32_t* bad) {
+    const std::uint64_t total = rows * 32;
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += stride) {
+        const std::uint64_t row = first + t / 32;
+        const std::uint32_t quad = static_cast<std::uint32_t>(t % 32);
+        const std::uint32_t m = quad / 4, i = quad % 4;
+        const float* p = src + row * ld + 32 * i + 4 * m;
+        std::uint32_t word = 0;
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = p[j];
+            ok = ok && v >= 0.0f && v <= 255.0f && v == __builtin_rintf(v);
+            word |= (ok ? static_cast<std::uint32_t>(v) : 0u) << (8 * j);
+        }
+        if (!ok) atomicOr(bad, 1u);
+        *reinterpret_cast<std::uint32_t*>(dst + row * 128 + 16 * m + 4 * i) = word;
+    }
+}
+
 namespace {
 unsigned blocks_for(std::uint64_t items) {
     const std::uint64_t b = (items + 255) / 256;
@@ -85,6 +113,15 @@ hipError_t launch_gather_rows(float* dst, std::uint32_t dst_ld, const float* src
     if (rows == 0 || width == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(rows * width)), dim3(256), 0, s, dst, dst_ld, src, src_ld,
                        ids, rows, width);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_rows_u8(unsigned char* dst, const float* src, std::uint32_t ld, std::uint64_t first,
+                               std::uint64_t rows, std::uint32_t* bad, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (ld < 128 || ld % 4 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_rows_u8_kernel, dim3(blocks_for(rows * 32)), dim3(256), 0, s, dst, src, ld, first, rows,
+                       bad);
     return hipGetLastError();
 }
 

@@ -16,6 +16,11 @@ hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, con
 // row indices of src.
 hipError_t launch_gather_rows(float* dst, std::uint32_t dst_ld, const float* src, std::uint32_t src_ld,
                               const std::uint32_t* ids, std::uint64_t rows, std::uint32_t width, hipStream_t s);
+// Rows [first, first + rows) of src (the first 128 of each row's ld floats) packed as 128 bytes
+// each into dst (rows at dst + row * 128, the beam's lane order); *bad becomes nonzero if an
+// element is not an integer in [0, 255]. ld >= 128, a multiple of 4.
+hipError_t launch_pack_rows_u8(unsigned char* dst, const float* src, std::uint32_t ld, std::uint64_t first,
+                               std::uint64_t rows, std::uint32_t* bad, hipStream_t s);
 // dst and src 16-byte aligned (device allocations are).
 hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
 

@@ -28,6 +28,10 @@ struct vdx_gpu_hnsw {
     vdx::gpu::DeviceBuffer changes;
     vdx::gpu::DimSplit split{};
     vdx::gpu::DeviceBuffer vectors, layer0, upper, upper_offset;
+    // Dimension 128 with every stored value an integer in [0, 255]: the rows again as bytes, 128
+    // per row in the beam's lane order (a quarter of the gather traffic, the same floats).
+    vdx::gpu::DeviceBuffer rows_u8, u8_flag;
+    bool u8 = false;
     // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in
     // turn, so up to 64 calls in flight on different streams never share one.
     vdx::gpu::DeviceBuffer ring;

@@ -471,6 +471,7 @@ template <int MODE>
 const void* kernel_for(BeamShape shape) {
     switch (shape) {
         case BeamShape::D128:
+        case BeamShape::D128U8:
             return reinterpret_cast<const void*>(&hnsw_beam_kernel<4, 0, 0, 4, MODE>);
         case BeamShape::D100:
             return reinterpret_cast<const void*>(&hnsw_beam_kernel<3, 0, 4, 4, MODE>);

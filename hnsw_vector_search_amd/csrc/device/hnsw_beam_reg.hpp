@@ -95,8 +95,10 @@ __device__ __forceinline__ void first_open(const std::uint64_t (&Rk)[CH], int fr
 
 }  // namespace reg
 
-template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE>
+// U8 (dimension 128 only): rows come from the 8-bit copy a.rows_u8 (distances_rows_u8).
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE, bool U8 = false>
 __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
+    static_assert(!U8 || (NB == 4 && N8 == 0 && TAIL == 0 && FIXED), "8-bit rows are dimension 128");
     using namespace reg;
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -175,6 +177,14 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         for (int e = 0; e < 8; ++e) {
             qt[e] = (e < TAIL && (FIXED || e < tail)) ? qp[first_tail + e] : 0.f;
         }
+        // Distances of cnt candidates (ids in LDS) to the query, into LDS.
+        const auto row_distances = [&](const std::uint32_t* ids, std::uint32_t cnt, float* dst) {
+            if constexpr (U8) {
+                distances_rows_u8<RR>(a.rows_u8, a.l2, ids, cnt, qb, qt, dst);
+            } else {
+                distances_rows<NB, N8, TAIL, RR, FIXED>(src, ids, cnt, qb, q8, qt, nb, n8, tail, first_tail, dst);
+            }
+        };
 
         // Entry point and greedy descent (src/hnsw.cpp:513-516, 416-431), or a given layer-0
         // entry (src/hnsw.cpp:324-335).
@@ -191,7 +201,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             fid[0] = cur;
         }
         wave_lds_fence();
-        distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid, 1, qb, q8, qt, nb, n8, tail, first_tail, fd);
+        row_distances(fid, 1, fd);
         wave_lds_fence();
         float cur_d = fd[0];
         ++n_dist;
@@ -209,8 +219,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 n_dist += cnt;
                 wave_lds_fence();
                 for (std::uint32_t c0 = 0; c0 < cnt; c0 += 8 * RR) {
-                    distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid + c0, min(8u * RR, cnt - c0), qb, q8,
-                                                            qt, nb, n8, tail, first_tail, fd + c0);
+                    row_distances(fid + c0, min(8u * RR, cnt - c0), fd + c0);
                 }
                 wave_lds_fence();
                 const float d = static_cast<std::uint32_t>(lane) < cnt ? fd[lane] : __int_as_float(0x7f800000);
@@ -342,8 +351,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
 
             n_dist += nf;
             for (std::uint32_t f0 = 0; f0 < nf; f0 += 8 * RR) {
-                distances_rows<NB, N8, TAIL, RR, FIXED>(src, fid + f0, min(8u * RR, nf - f0), qb, q8, qt,
-                                                        nb, n8, tail, first_tail, fd + f0);
+                row_distances(fid + f0, min(8u * RR, nf - f0), fd + f0);
             }
             wave_lds_fence();
             std::uint64_t f = kSentinel;
@@ -537,10 +545,10 @@ __global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
     beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, MODE>(a);
 }
 
-template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH>
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, bool U8 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void hnsw_beam_reg_plain_kernel(
     BeamArgs a) {
-    beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain>(a);
+    beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, U8>(a);
 }
 
 // Kernel pointers of one chunk count, by shape and mode.
@@ -549,6 +557,7 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
 #define VDX_REG_KERNEL(MODE)                                                                     \
     switch (shape) {                                                                             \
         case BeamShape::D128:                                                                    \
+        case BeamShape::D128U8:                                                                  \
             return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<4, 0, 0, 4, true, CH, MODE>); \
         case BeamShape::D100:                                                                    \
             return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<3, 0, 4, 4, true, CH, MODE>); \
@@ -565,6 +574,8 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
     switch (shape) {
         case BeamShape::D128:
             return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<4, 0, 0, 4, true, CH>);
+        case BeamShape::D128U8:
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<4, 0, 0, 4, true, CH, true>);
         case BeamShape::D100:
             return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<3, 0, 4, 4, true, CH>);
         default:

@@ -196,6 +196,14 @@ class _GpuSearch:
             nat.ptr(count), where.encode(), ctypes.byref(opts) if opts is not None else None))
         return out, count
 
+    def gpu_u8_rows(self) -> bool:
+        """Whether the device copy also holds its rows as bytes (dimension 128, every value an
+        integer in [0, 255]); the plain search then reads those (same results, a quarter of the
+        row traffic) unless opts.row_format = 1 (vdx_gpu_hnsw_u8_rows)."""
+        kept = ctypes.c_uint32(0)
+        nat.check(nat.lib.vdx_gpu_hnsw_u8_rows(self.gpu_handle(), ctypes.byref(kept)))
+        return bool(kept.value)
+
     def search(self, query: Sequence[float], k: int, ef: int) -> list:
         """Up to k approximate nearest neighbors, closest first (HnswIndex::search)."""
         q = np.asarray(query, dtype=np.float32).reshape(1, -1)
