@@ -72,8 +72,9 @@ constexpr int kBeamStats = 6;
 
 enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
 
-// D128U8: dimension 128 with the rows read from the 8-bit copy (BeamArgs::rows_u8); only the
-// register-resident plain search has that variant, the other kernels take D128's.
+// D128U8: dimension 128 under squared L2 with the rows read from the 8-bit copy
+// (BeamArgs::rows_u8); only the register-resident plain search has that variant, the other
+// kernels take D128's.
 enum class BeamShape { D128, D100, Generic, D128U8 };
 
 // Bytes of LDS one wave uses for a given R capacity and visited-cache size.

@@ -72,7 +72,8 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
         p.ef_cap = need * 64u;
     }
     p.shape = beam_shape(h.dim);
-    if (p.shape == vdx::dev::BeamShape::D128 && h.u8 && p.chunks > 0 && mode == vdx::dev::kBeamPlain && !cfg.f32_rows) {
+    if (p.shape == vdx::dev::BeamShape::D128 && h.u8 && h.metric == 0 && p.chunks > 0 && mode == vdx::dev::kBeamPlain &&
+        !cfg.f32_rows) {
         p.shape = vdx::dev::BeamShape::D128U8;
     }
     p.visited_log2 = cfg.visited_log2 ? cfg.visited_log2

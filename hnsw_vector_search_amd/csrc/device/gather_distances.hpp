@@ -242,34 +242,33 @@ __device__ __forceinline__ void distances_rows(const RowSource& a, const std::ui
 }
 
 
-// distances_rows for dimension 128 with rows held as bytes: lane m of a row's 8-lane group loads
-// its 16 elements (32i + 4m + j, i and j in 0..3) as one 16-byte load from byte 16m of the row
-// (pack_rows_u8's layout) and converts them to the floats the FP32 rows hold; the arithmetic is
-// then distances_rows<4, 0, 0, R, true>'s, so every distance is bit-identical.
+// distances_rows for squared L2 at dimension 128 with rows held as bytes: lane m of a row's
+// 8-lane group loads its 16 elements (32i + 4m + j, i and j in 0..3) as one 16-byte load from
+// byte 16m of the row (pack_rows_u8's layout) and converts them to the floats the FP32 rows
+// hold; the arithmetic is then distances_rows<4, 0, 0, R, true>'s for L2, so every distance is
+// bit-identical. All R rounds' loads are issued unconditionally (a slot past cnt reloads the
+// first candidate's row, a cache hit) so that no branch sits between them and the compiler's
+// waits can release round r while later rounds are still in flight.
 template <int R>
-__device__ __forceinline__ void distances_rows_u8(const unsigned char* rows, std::uint32_t l2_flag,
-                                                  const std::uint32_t* ids, std::uint32_t cnt,
-                                                  const float4 (&qb)[4], const float (&qt)[8], float* out) {
+__device__ __forceinline__ void distances_rows_u8_l2(const unsigned char* rows, const std::uint32_t* ids,
+                                                     std::uint32_t cnt, const float4 (&qb)[4], float* out) {
     const int lane = lane_id();
     const int g = lane >> 3;
     const int m = lane & 7;
     std::uint32_t rid[R];
-    bool ok[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const std::uint32_t c = static_cast<std::uint32_t>(r * 8 + g);
-        ok[r] = c < cnt;
-        rid[r] = ok[r] ? ids[c] : 0u;
+        rid[r] = ids[c < cnt ? c : 0u];
     }
     uint4 w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        w[r] = make_uint4(0, 0, 0, 0);
-        if (ok[r]) {
-            w[r] = *reinterpret_cast<const uint4*>(rows + static_cast<std::size_t>(rid[r]) * 128 + 16 * m);
-        }
+        w[r] = *reinterpret_cast<const uint4*>(rows + static_cast<std::size_t>(rid[r]) * 128 + 16 * m);
     }
-    const bool l2 = l2_flag != 0;
+    // every row load is in flight before the first conversion (the scheduler would otherwise
+    // start round 0's arithmetic, and its wait, ahead of the later rounds' loads)
+    __builtin_amdgcn_sched_barrier(0);
     float xt[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -285,16 +284,13 @@ __device__ __forceinline__ void distances_rows_u8(const unsigned char* rows, std
                 // byte j of the word converts exactly (v_cvt_f32_ubyte<j>)
                 const float4 x = make_float4(static_cast<float>(wi[i] & 0xFFu), static_cast<float>((wi[i] >> 8) & 0xFFu),
                                              static_cast<float>((wi[i] >> 16) & 0xFFu), static_cast<float>(wi[i] >> 24));
-                if (l2) {
-                    const float4 d = f4_sub(qb[i], x);
-                    acc = f4_fma(d, d, acc);
-                } else {
-                    acc = f4_fma(qb[i], x, acc);
-                }
+                const float4 d = f4_sub(qb[i], x);
+                acc = f4_fma(d, d, acc);
             }
-            const float d = finish_distance_dpp(l2, acc, qt, xt, 0);
-            if (m == 0 && ok[r]) {
-                out[r * 8 + g] = d;
+            const float dist = finish_distance_dpp(true, acc, xt, xt, 0);
+            const std::uint32_t c = static_cast<std::uint32_t>(r * 8 + g);
+            if (m == 0 && c < cnt) {
+                out[c] = dist;
             }
         }
     }

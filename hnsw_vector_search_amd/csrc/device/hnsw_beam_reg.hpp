@@ -95,7 +95,8 @@ __device__ __forceinline__ void first_open(const std::uint64_t (&Rk)[CH], int fr
 
 }  // namespace reg
 
-// U8 (dimension 128 only): rows come from the 8-bit copy a.rows_u8 (distances_rows_u8).
+// U8 (dimension 128, squared L2 only): rows come from the 8-bit copy a.rows_u8
+// (distances_rows_u8_l2).
 template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE, bool U8 = false>
 __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
     static_assert(!U8 || (NB == 4 && N8 == 0 && TAIL == 0 && FIXED), "8-bit rows are dimension 128");
@@ -180,7 +181,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         // Distances of cnt candidates (ids in LDS) to the query, into LDS.
         const auto row_distances = [&](const std::uint32_t* ids, std::uint32_t cnt, float* dst) {
             if constexpr (U8) {
-                distances_rows_u8<RR>(a.rows_u8, a.l2, ids, cnt, qb, qt, dst);
+                distances_rows_u8_l2<RR>(a.rows_u8, ids, cnt, qb, dst);
             } else {
                 distances_rows<NB, N8, TAIL, RR, FIXED>(src, ids, cnt, qb, q8, qt, nb, n8, tail, first_tail, dst);
             }
@@ -350,7 +351,13 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             }
 
             n_dist += nf;
-            for (std::uint32_t f0 = 0; f0 < nf; f0 += 8 * RR) {
+            // The first 8 RR rows outside the loop: in straight-line code the compiler's waits
+            // leave the prefetched block in flight behind the row loads (at a loop header it
+            // waits for every load already issued). More rows than that only for M > 4 RR.
+            if (nf > 0) {
+                row_distances(fid, min(8u * RR, nf), fd);
+            }
+            for (std::uint32_t f0 = 8 * RR; f0 < nf; f0 += 8 * RR) {
                 row_distances(fid + f0, min(8u * RR, nf - f0), fd + f0);
             }
             wave_lds_fence();
