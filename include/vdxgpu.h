@@ -244,9 +244,9 @@ VDX_API int vdx_gpu_hnsw_info(const vdx_gpu_hnsw* index, uint32_t* metric, uint6
                               uint64_t* M);
 /* No vecdex counterpart: *kept = 1 if the device copy also holds its rows as bytes (dimension
  * 128, every stored value an integer in [0, 255], as SIFT-like bases are; kept current by
- * vdx_gpu_hnsw_update and dropped once a row that is not so arrives). The plain search reads that
- * copy unless vdx_search_opts.row_format = 1: the same floats, so the same results, at a quarter
- * of the row traffic. */
+ * vdx_gpu_hnsw_update and dropped once a row that is not so arrives). The plain L2 search reads
+ * that copy unless vdx_search_opts.row_format = 1: the same floats, so the same results, at a
+ * quarter of the row traffic. */
 VDX_API int vdx_gpu_hnsw_u8_rows(const vdx_gpu_hnsw* index, uint32_t* kept);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 
@@ -260,9 +260,10 @@ typedef struct vdx_search_opts {
     float mlp_threshold;       /* stop when the model's probability exceeds this, in (0, 1) */
     float reserved_f;
     uint64_t* stats;           /* optional host array of VDX_STAT_COUNT counters, summed */
-    uint32_t row_format;       /* 0: the 8-bit row copy where the index keeps one (dimension 128,
-                                  every stored value an integer in [0, 255]: the same floats, so
-                                  the same distances and results); 1: always the FP32 rows */
+    uint32_t row_format;       /* 0: the plain L2 search reads the 8-bit row copy where the index
+                                  keeps one (dimension 128, every stored value an integer in
+                                  [0, 255]: the same floats, so the same distances and results);
+                                  1: always the FP32 rows */
     uint32_t reserved_u;
 } vdx_search_opts;
 
