@@ -114,3 +114,45 @@ def test_row_format_is_validated():
     opts.row_format = 2
     with pytest.raises(ValueError, match="row_format"):
         index.search_arrays(byte_base(3, 6), 5, 10, opts=opts)
+
+
+def test_captured_batch_follows_the_byte_rows_across_adds():
+    print("case: batch graph over 8-bit rows across adds, 3000 x 128", flush=True)
+    index = vdx.HnswIndex(128, vdx.Metric.L2, M=8, ef_construction=60, seed=11)
+    index.add(byte_base(3000, 1), num_threads=4)
+    queries = byte_base(256, 7) + np.float32(0.5)
+    ctx = nat.Context.get(0)
+    dq = nat.DeviceArray.from_numpy(queries, ctx)
+    out = nat.DeviceArray((len(queries), 10), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(len(queries), np.uint32, ctx)
+    batch = vdx.SearchBatch(index, len(queries), 10, 64)
+    extra = byte_base(1, 5)
+    extra[0, 3] = 300.0  # the last add drops the byte copy
+    for step, rows in enumerate((byte_base(4000, 2), extra, None)):
+        for replay in range(2):
+            batch.run(dq, len(queries), out, cnt, k=10, ef=64, graph=True)
+            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+            ref, ref_cnt = o.hnsw_search(index.graph_arrays(), queries, 10, 64)
+            np.testing.assert_array_equal(cnt.numpy(), ref_cnt)
+            assert out.numpy().tobytes() == ref.tobytes(), f"step {step}, replay {replay}"
+        if rows is not None:
+            index.add(rows, num_threads=8)
+            index.gpu_handle()  # the update; the next replay must see the new buffers
+            assert index.gpu_u8_rows() == (step == 0)
+
+
+@pytest.mark.parametrize("n", [5, 64, 70])
+@pytest.mark.parametrize("metric", [vdx.Metric.L2, vdx.Metric.Cosine])
+def test_bf16_lds_fed_filter_on_tiny_bases(n, metric):
+    """Fewer rows than one 64-row tile, exactly one, one and a bit: the LDS-fed BF16 kernel
+    repeats the last row past the end and tests the row bound in the last tile."""
+    print(f"case: bf16 128-d brute force, n {n}, metric {metric}", flush=True)
+    rng = np.random.default_rng(n)
+    base = rng.uniform(-1, 1, (n, 128)).astype(np.float32)
+    queries = rng.uniform(-1, 1, (33, 128)).astype(np.float32)
+    index = vdx.BruteForceIndex(128, metric)
+    index.add(base)
+    for k in (1, 5, n):
+        exact, _ = index.search_arrays(queries, k, "exact")
+        got, _ = index.search_arrays(queries, k, "bf16")
+        assert got.tobytes() == exact.tobytes(), (n, k)
