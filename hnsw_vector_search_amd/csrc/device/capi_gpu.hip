@@ -80,12 +80,13 @@ void upload_rows(DeviceBuffer& dst, const float* src, std::uint64_t rows, std::u
 }
 
 // Queues one beam launch over prepared device queries.
-// Keeps h's 8-bit row copy current after rows [first, h.n) changed: dimension 128 only, and only
-// while every stored value is an integer in [0, 255] (checked on the device as the rows are
-// packed); otherwise the copy is dropped and the search reads the FP32 rows. The buffer grows
-// with a quarter of headroom, keeping the rows before `first`. Synchronizes the stream.
+// Keeps h's 8-bit row copy current after rows [first, h.n) changed: squared L2 at dimension 128
+// only (the byte-row search's shape), and only while every stored value is an integer in
+// [0, 255] (checked on the device as the rows are packed); otherwise the copy is dropped and the
+// search reads the FP32 rows. The buffer grows with a quarter of headroom, keeping the rows
+// before `first`. Synchronizes the stream.
 void refresh_rows_u8(vdx_gpu_hnsw& h, std::uint64_t first, hipStream_t s) {
-    if (h.dim != 128 || h.metric == 2 || (first > 0 && !h.u8) || h.n == 0) {
+    if (h.dim != 128 || h.metric != 0 || (first > 0 && !h.u8) || h.n == 0) {
         h.u8 = false;
         h.rows_u8.release();
         return;

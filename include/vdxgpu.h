@@ -242,11 +242,11 @@ VDX_API int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw**
 /* The device copy's shape. */
 VDX_API int vdx_gpu_hnsw_info(const vdx_gpu_hnsw* index, uint32_t* metric, uint64_t* dim, uint64_t* n,
                               uint64_t* M);
-/* No vecdex counterpart: *kept = 1 if the device copy also holds its rows as bytes (dimension
- * 128, every stored value an integer in [0, 255], as SIFT-like bases are; kept current by
- * vdx_gpu_hnsw_update and dropped once a row that is not so arrives). The plain L2 search reads
- * that copy unless vdx_search_opts.row_format = 1: the same floats, so the same results, at a
- * quarter of the row traffic. */
+/* No vecdex counterpart: *kept = 1 if the device copy also holds its rows as bytes (squared
+ * L2, dimension 128, every stored value an integer in [0, 255], as SIFT-like bases are; kept
+ * current by vdx_gpu_hnsw_update and dropped once a row that is not so arrives). The plain search
+ * reads that copy unless vdx_search_opts.row_format = 1: the same floats, so the same results, at
+ * a quarter of the row traffic. */
 VDX_API int vdx_gpu_hnsw_u8_rows(const vdx_gpu_hnsw* index, uint32_t* kept);
 VDX_API void vdx_gpu_hnsw_free(vdx_gpu_hnsw* index);
 

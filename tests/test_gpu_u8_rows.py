@@ -44,7 +44,7 @@ def test_byte_rows_give_the_fp32_results(metric):
     base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 8000, 200)
     index = vdx.HnswIndex(128, metric, M=12, ef_construction=80, seed=3)
     index.add(base, num_threads=8)
-    assert index.gpu_u8_rows()
+    assert index.gpu_u8_rows() == (metric == vdx.Metric.L2)  # the byte-row search is L2's
     # fractional queries too: the rows convert exactly, whatever the query holds
     frac = queries + np.float32(0.37)
     for q in (queries, frac):
@@ -54,7 +54,8 @@ def test_byte_rows_give_the_fp32_results(metric):
             assert got.tobytes() == ref.tobytes()
 
 
-@pytest.mark.parametrize("case", ["fractional", "negative", "above 255", "dimension 100", "cosine"])
+@pytest.mark.parametrize("case", ["fractional", "negative", "above 255", "dimension 100", "cosine",
+                                  "inner product"])
 def test_other_bases_keep_fp32_rows_only(case):
     print(f"case: no 8-bit rows, {case}", flush=True)
     dim, metric = 128, vdx.Metric.L2
@@ -67,8 +68,10 @@ def test_other_bases_keep_fp32_rows_only(case):
         base[0, 0] = 256.0
     elif case == "dimension 100":
         dim, base = 100, base[:, :100].copy()
-    else:
+    elif case == "cosine":
         metric = vdx.Metric.Cosine
+    else:
+        metric = vdx.Metric.InnerProduct
     index = vdx.HnswIndex(dim, metric, M=8, ef_construction=40, seed=1)
     index.add(base, num_threads=4)
     assert not index.gpu_u8_rows()
