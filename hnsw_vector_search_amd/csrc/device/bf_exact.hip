@@ -34,7 +34,10 @@ __device__ __forceinline__ std::uint32_t insert_sorted(std::uint64_t* L, std::ui
 
 }  // namespace
 
-template <int NB, int N8, int TAIL>
+// L2: squared L2 (a.l2 != 0), else 1 - <q, x>, fixed at compile time so the per-row arithmetic
+// carries no metric branch; the group's partial sums are reduced with DPP moves (the same
+// additions in the same association as the reference's AVX2 horizontal sum).
+template <int NB, int N8, int TAIL, bool L2>
 __global__ __launch_bounds__(256) void bf_exact_scan_kernel(BfScanArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(256) void bf_exact_scan_kernel(BfScanArgs a) {
     const int n8 = static_cast<int>(a.n8);
     const int tail = static_cast<int>(a.tail);
     const int first_tail = static_cast<int>(a.first_tail);
-    const bool l2 = a.l2 != 0;
+    constexpr bool l2 = L2;
 
     float4 qb[NB > 0 ? NB : 1];
     float4 q8[N8 > 0 ? N8 : 1];
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(256) void bf_exact_scan_kernel(BfScanArgs a) {
             for (int e = 0; e < 8; ++e) {
                 xt[e] = (e < TAIL && e < tail) ? row[first_tail + e] : 0.f;
             }
-            const float d = finish_distance(l2, acc, qt, xt, TAIL > 0 ? tail : 0);
+            const float d = finish_distance_dpp(l2, acc, qt, xt, TAIL > 0 ? tail : 0);
             if (m == 0 && active) {
                 const std::uint64_t key = make_key(d, base + r);
                 if (count < cap || key < worst) {
@@ -195,17 +198,24 @@ hipError_t launch_bf_exact_scan(const BfScanArgs& args, unsigned qgroups, unsign
                                 hipStream_t stream, int shape) {
     const dim3 grid(qgroups, slabs);
     const dim3 block(waves_per_group * kWave);
-    switch (shape) {
-        case 0:
-            hipLaunchKernelGGL((bf_exact_scan_kernel<4, 0, 0>), grid, block, lds_bytes, stream, args);
-            break;
-        case 1:
-            hipLaunchKernelGGL((bf_exact_scan_kernel<3, 0, 4>), grid, block, lds_bytes, stream, args);
-            break;
-        default:
-            hipLaunchKernelGGL((bf_exact_scan_kernel<8, 3, 7>), grid, block, lds_bytes, stream, args);
-            break;
+#define VDX_SCAN_LAUNCH(L2)                                                                       \
+    switch (shape) {                                                                              \
+        case 0:                                                                                   \
+            hipLaunchKernelGGL((bf_exact_scan_kernel<4, 0, 0, L2>), grid, block, lds_bytes, stream, args); \
+            break;                                                                                \
+        case 1:                                                                                   \
+            hipLaunchKernelGGL((bf_exact_scan_kernel<3, 0, 4, L2>), grid, block, lds_bytes, stream, args); \
+            break;                                                                                \
+        default:                                                                                  \
+            hipLaunchKernelGGL((bf_exact_scan_kernel<8, 3, 7, L2>), grid, block, lds_bytes, stream, args); \
+            break;                                                                                \
     }
+    if (args.l2 != 0) {
+        VDX_SCAN_LAUNCH(true)
+    } else {
+        VDX_SCAN_LAUNCH(false)
+    }
+#undef VDX_SCAN_LAUNCH
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         return err;
