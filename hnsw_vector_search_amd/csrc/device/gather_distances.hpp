@@ -144,8 +144,8 @@ __device__ __forceinline__ float finish_distance_dpp(bool l2, float4 acc, const 
 // distances() for the register-resident beam: the candidates' ids are read from LDS first,
 // then every row's loads are issued under one predicate per row, then the arithmetic runs with
 // DPP reductions. FIXED: the runtime shape equals NB/N8/TAIL (dimensions 128 and 100), so
-// every block test folds away.
-template <int NB, int N8, int TAIL, int R, bool FIXED>
+// every block test folds away. MET: 1 squared L2, 0 the inner-product form, -1 a.l2 at run time.
+template <int NB, int N8, int TAIL, int R, bool FIXED, int MET = -1>
 __device__ __forceinline__ void distances_rows(const RowSource& a, const std::uint32_t* ids,
                                                std::uint32_t cnt, const float4 (&qb)[NB > 0 ? NB : 1],
                                                const float4 (&q8)[N8 > 0 ? N8 : 1],
@@ -206,7 +206,7 @@ __device__ __forceinline__ void distances_rows(const RowSource& a, const std::ui
             }
         }
     }
-    const bool l2 = a.l2 != 0;
+    const bool l2 = MET >= 0 ? MET == 1 : a.l2 != 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (static_cast<std::uint32_t>(r * 8) < cnt) {
