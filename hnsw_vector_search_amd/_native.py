@@ -2,18 +2,23 @@
 
 The library is loaded from this package directory only; there is no fallback. If it is missing
 the import fails with a message saying how to build it, so nothing ever runs on a CPU
-stand-in.
+stand-in. VDX_LIBRARY=sanitized selects the same library built with its host code under
+ASan + UBSan (`python -m hnsw_vector_search_amd.build --sanitize`), for CPU-side checks run
+with the sanitizer runtime preloaded.
 """
 
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libvdxgpu.so"
+if os.environ.get("VDX_LIBRARY") == "sanitized":
+    LIB_PATH = Path(__file__).resolve().parent / "_build_san" / "libvdxgpu.so"
 
 if not LIB_PATH.exists():
     raise ImportError(
@@ -29,7 +34,8 @@ VDX_OK, VDX_INVALID_ARGUMENT, VDX_RUNTIME_ERROR = 0, 1, 2
 VDX_LENGTH_ERROR, VDX_OUT_OF_MEMORY, VDX_DEVICE_ERROR = 3, 4, 5
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "exact": 3}
-STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates", "early_stops")
+STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates", "early_stops", "link_prefetched",
+              "clamped")
 MLP_PARAMS = 5345
 
 

@@ -68,7 +68,7 @@ struct BeamArgs {
     std::uint32_t min_steps;
 };
 
-constexpr int kBeamStats = 6;
+constexpr int kBeamStats = 7;
 
 enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
 

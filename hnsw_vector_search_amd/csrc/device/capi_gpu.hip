@@ -39,7 +39,7 @@ constexpr std::uint32_t kMaxGpuDim = 287;    // 8 full 32-blocks, 3 8-blocks, 7 
 constexpr std::uint32_t kMaxBeamEf = 8192;   // R capacity per wave in LDS
 constexpr std::uint32_t kMaxGpuM = 32;       // a layer-0 block fits one wave
 constexpr std::uint32_t kMaxBfK = 2048;
-constexpr std::uint32_t kRingSlots = 64;
+constexpr std::uint32_t kRingSlots = vdx_gpu_hnsw::kRingSlots;
 
 int bf_shape(std::uint64_t dim) { return dim == 128 ? 0 : (dim == 100 ? 1 : 2); }
 
@@ -655,7 +655,7 @@ int vdx_gpu_hnsw_build_add(vdx_gpu_hnsw* index, vdx_host_graph* graph, const flo
 int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out) {
     return guarded([&] {
         require(ctx != nullptr && path != nullptr && out != nullptr, "vdx_gpu_hnsw_load: null argument");
-        vdx::V2Map map(path);
+        vdx::V2File map(path);
         const vdx::V2Header& hd = map.header();
         if (hd.dim > kMaxGpuDim) {
             throw std::invalid_argument(vdx::fmt("vdx_gpu_hnsw_load: dimension %llu is above the GPU path's limit of %u",
@@ -691,8 +691,8 @@ int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out) {
             check(hipMemcpyAsync(h->upper_offset.as<void>(), verify.offsets().data(), hd.n * 4, hipMemcpyHostToDevice, s),
                   "hipMemcpyAsync(offsets)");
         }
-        // Two pinned chunks in turn: the file's pages are copied into one (and checked there,
-        // cache-hot) while the other's copy to HBM is in flight.
+        // Two pinned chunks in turn: the file is read into one (and checked there, cache-hot)
+        // while the other's copy to HBM is in flight.
         constexpr std::uint64_t kChunkBytes = std::uint64_t{64} << 20;
         vdx::gpu::PinnedBuffer pinned[2];
         hipEvent_t done[2] = {nullptr, nullptr};
@@ -722,7 +722,7 @@ int vdx_gpu_hnsw_load(vdx_gpu_ctx* ctx, const char* path, vdx_gpu_hnsw** out) {
                     check(hipEventSynchronize(done[slot]), "hipEventSynchronize");
                 }
                 pinned[slot].reserve(std::min(units, per) * unit_bytes);
-                std::memcpy(pinned[slot].as<void>(), map.section(sec) + u0 * unit_bytes, bytes);
+                map.read(sec, u0 * unit_bytes, pinned[slot].as<void>(), bytes);
                 check_chunk(pinned[slot].as<void>(), u0, m);
                 check(hipMemcpyAsync(dst + u0 * unit_bytes, pinned[slot].as<void>(), bytes, hipMemcpyHostToDevice, s),
                       "hipMemcpyAsync(section)");
@@ -817,7 +817,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         ws->queries.reserve(nq * h->dim * sizeof(float));
         ws->out.reserve(nq * kmax * sizeof(vdx_neighbor));
         ws->count.reserve(nq * sizeof(std::uint32_t));
-        ws->counters.reserve(64);
+        ws->counters.reserve(128);  // the queue counter, then the stats at byte 16
         check(hipMemcpyAsync(ws->queries.as<void>(), ws->h_queries.as<void>(), nq * h->dim * sizeof(float),
                              hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(queries)");
@@ -879,12 +879,23 @@ int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t
             return;
         }
         // max(ef, k) over the batch sizes R; with per-query arrays the caller's bound is max_k
-        // and the ef array must not exceed `ef`.
+        // and `ef` must bound the ef array (values above run at the launch's R capacity and are
+        // counted in VDX_STAT_CLAMPED).
+        require(d_efs == nullptr || ef >= 1,
+                "vdx_gpu_hnsw_search_device: with per-query efs, ef must bound them (it is 0)");
         const std::uint32_t efmax = std::max(ef, max_k);
         require(efmax <= kMaxBeamEf, "vdx_gpu_hnsw_search_device: ef above the GPU path's limit");
         DeviceGuard guard(h->ctx->device);
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->ctx->stream;
         const std::uint32_t slot = h->next_slot.fetch_add(1) % kRingSlots;
+        // the slot is reused only once the launch that last took it has completed
+        const std::lock_guard<std::mutex> slot_lock(h->ring_mutex[slot]);
+        hipEvent_t& slot_done = h->ring_event[slot];
+        if (slot_done == nullptr) {
+            check(hipEventCreateWithFlags(&slot_done, hipEventDisableTiming), "hipEventCreate(ring slot)");
+        } else {
+            check(hipEventSynchronize(slot_done), "hipEventSynchronize(ring slot)");
+        }
         char* base = h->ring.as<char>() + static_cast<std::size_t>(slot) * 256;
         std::uint32_t* counter = reinterpret_cast<std::uint32_t*>(base);
         unsigned long long* d_stats = reinterpret_cast<unsigned long long*>(base + 64);
@@ -899,6 +910,7 @@ int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t
         enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
                      reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
                      want_stats ? d_stats : nullptr, opts, s);
+        check(hipEventRecord(slot_done, s), "hipEventRecord(ring slot)");
         if (want_stats) {
             unsigned long long host_stats[vdx::dev::kBeamStats];
             check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),

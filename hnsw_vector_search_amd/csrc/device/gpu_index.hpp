@@ -33,9 +33,25 @@ struct vdx_gpu_hnsw {
     vdx::gpu::DeviceBuffer rows_u8, u8_flag;
     bool u8 = false;
     // Queue counters and stats of the device entry point: one 256-byte slot per call, taken in
-    // turn, so up to 64 calls in flight on different streams never share one.
+    // turn. A call holds its slot's lock while it waits for the slot's previous launch (its
+    // event) and enqueues its own, so calls in flight on different streams never share one.
+    static constexpr std::uint32_t kRingSlots = 64;
     vdx::gpu::DeviceBuffer ring;
     std::atomic<std::uint32_t> next_slot{0};
+    std::mutex ring_mutex[kRingSlots];
+    hipEvent_t ring_event[kRingSlots] = {};
+
+    vdx_gpu_hnsw() = default;
+    vdx_gpu_hnsw(const vdx_gpu_hnsw&) = delete;
+    vdx_gpu_hnsw& operator=(const vdx_gpu_hnsw&) = delete;
+    ~vdx_gpu_hnsw() {
+        for (hipEvent_t& e : ring_event) {
+            if (e != nullptr) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
+        }
+    }
 };
 
 namespace vdx::gpu {

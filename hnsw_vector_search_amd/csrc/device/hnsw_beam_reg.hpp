@@ -136,7 +136,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         end = uniform(a.range[1]);
     }
 
-    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0;
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0, n_clamp = 0;
 
     for (;;) {
         std::uint32_t t = 0;
@@ -151,6 +151,9 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
         const std::uint32_t ef = uniform(a.efs ? a.efs[q] : a.ef);
         const std::uint32_t efp = min(max(ef, k), static_cast<std::uint32_t>(64 * CH));
+        // a per-query ef above the launch's R capacity runs at the capacity: counted, so a
+        // caller whose `ef` bound was too small can see it
+        n_clamp += max(ef, k) > static_cast<std::uint32_t>(64 * CH) ? 1u : 0u;
         Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
 
         // The query, spread over the 8 lanes of each group (every group holds a full copy).
@@ -541,6 +544,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         atomicAdd(a.stats + 3, n_dup);
         atomicAdd(a.stats + 4, n_stop);
         atomicAdd(a.stats + 5, n_pref);
+        atomicAdd(a.stats + 6, n_clamp);
     }
 }
 

@@ -1,5 +1,6 @@
 // The host half of the C ABI: the restated graph build, FORMAT v1 and validation.
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -204,7 +205,7 @@ int vdx_host_graph_changes(const vdx_host_graph* g, uint64_t since_version, uint
         *count = changed.size();
         if (nodes != nullptr) {
             require(capacity >= changed.size(), "vdx_host_graph_changes: capacity is below the count");
-            std::memcpy(nodes, changed.data(), changed.size() * sizeof(uint32_t));
+            std::copy(changed.begin(), changed.end(), nodes);  // (no memcpy from an empty vector's null data)
         }
     });
 }

@@ -1,14 +1,15 @@
-// FORMAT v2 (format_v2.hpp): writer, mapped reader with streaming checks, and the host graph's
+// FORMAT v2 (format_v2.hpp): writer, reader with streaming checks, and the host graph's
 // v2 load. The checks are v1's (docs/FORMAT.md:104-117, src/hnsw_io.cpp:409-504) with v1's
 // messages where they coincide, plus the layout, checksum and padding rules of v2.
 
 #include "format_v2.hpp"
 
 #include <fcntl.h>
-#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -75,40 +76,34 @@ void v2_layout(V2Header& h, const std::string& path) {
     }
 }
 
-V2Map::V2Map(const std::string& path) : path_(path) {
-    const int fd = ::open(path.c_str(), O_RDONLY);
-    if (fd < 0) {
+V2File::V2File(const std::string& path) : path_(path) {
+    fd_ = ::open(path.c_str(), O_RDONLY);
+    if (fd_ < 0) {
         fail_path(path, std::strerror(errno));
     }
+    struct Close {
+        V2File* f;
+        bool armed = true;
+        ~Close() {
+            if (armed) {
+                ::close(f->fd_);
+                f->fd_ = -1;
+            }
+        }
+    } close_on_error{this};
     struct stat st {};
-    if (::fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
-        ::close(fd);
+    if (::fstat(fd_, &st) != 0 || !S_ISREG(st.st_mode)) {
         fail_path(path, "not a regular file");
     }
     size_ = static_cast<std::uint64_t>(st.st_size);
     if (size_ < kV2HeaderBytes) {
-        ::close(fd);
         fail_path(path, fmt("%llu bytes is too small for a version 2 vecdex index", static_cast<ull>(size_)));
     }
-    void* p = ::mmap(nullptr, size_, PROT_READ, MAP_PRIVATE, fd, 0);
-    ::close(fd);
-    if (p == MAP_FAILED) {
-        fail_path(path, std::string("cannot map the file: ") + std::strerror(errno));
-    }
-    base_ = static_cast<const unsigned char*>(p);
-    struct Unmap {
-        V2Map* m;
-        bool armed = true;
-        ~Unmap() {
-            if (armed) {
-                ::munmap(const_cast<unsigned char*>(m->base_), m->size_);
-                m->base_ = nullptr;
-            }
-        }
-    } unmap{this};
-    (void)::madvise(p, size_, MADV_SEQUENTIAL);
+    (void)::posix_fadvise(fd_, 0, 0, POSIX_FADV_SEQUENTIAL);
+    std::vector<unsigned char> header(kV2HeaderBytes);
+    read_bytes(0, header.data(), kV2HeaderBytes);
 
-    const unsigned char* b = base_;
+    const unsigned char* b = header.data();
     if (std::memcmp(b, kMagic, 8) != 0) {
         fail_path(path, "not a vecdex HNSW index (wrong magic bytes)");
     }
@@ -194,24 +189,52 @@ V2Map::V2Map(const std::string& path) : path_(path) {
         fail_path(path, fmt("the header describes %llu bytes but the file has %llu; it is truncated or corrupted",
                             static_cast<ull>(end), static_cast<ull>(size_)));
     }
-    unmap.armed = false;
+    close_on_error.armed = false;
 }
 
-V2Map::~V2Map() {
-    if (base_ != nullptr) {
-        ::munmap(const_cast<unsigned char*>(base_), size_);
+V2File::~V2File() {
+    if (fd_ >= 0) {
+        ::close(fd_);
     }
 }
 
-V2Checker::V2Checker(const V2Map& map) : map_(map), h_(map.header()) {}
+void V2File::read_bytes(std::uint64_t at, void* dst, std::uint64_t bytes) const {
+    auto* out = static_cast<unsigned char*>(dst);
+    while (bytes > 0) {
+        const std::size_t want = static_cast<std::size_t>(std::min<std::uint64_t>(bytes, std::uint64_t{1} << 30));
+        const ssize_t got = ::pread(fd_, out, want, static_cast<off_t>(at));
+        if (got < 0 && errno == EINTR) {
+            continue;
+        }
+        if (got < 0) {
+            fail_path(path_, std::string("read failed: ") + std::strerror(errno));
+        }
+        if (got == 0) {
+            fail_path(path_, "the file ended early; it is truncated or was changed while loading");
+        }
+        out += got;
+        at += static_cast<std::uint64_t>(got);
+        bytes -= static_cast<std::uint64_t>(got);
+    }
+}
+
+void V2File::read(int s, std::uint64_t at, void* dst, std::uint64_t bytes) const {
+    if (at > h_.bytes[s] || bytes > h_.bytes[s] - at) {
+        throw std::logic_error("V2File::read past the end of a section");
+    }
+    read_bytes(h_.offset[s] + at, dst, bytes);
+}
+
+V2Checker::V2Checker(const V2File& file) : map_(file), h_(file.header()) {}
 
 void V2Checker::levels_and_offsets() {
     const std::string& path = map_.path();
     const std::uint64_t n = h_.n;
-    levels_.assign(map_.section(kV2Levels), map_.section(kV2Levels) + n);
+    levels_.resize(n);
+    map_.read(kV2Levels, 0, levels_.data(), n);
     hash_[kV2Levels].update(levels_.data(), n);
     offsets_.resize(n);
-    std::memcpy(offsets_.data(), map_.section(kV2Offsets), n * 4);
+    map_.read(kV2Offsets, 0, offsets_.data(), n * 4);
     hash_[kV2Offsets].update(offsets_.data(), n * 4);
     std::uint64_t blocks = 0;
     for (std::uint64_t i = 0; i < n; ++i) {
@@ -383,28 +406,35 @@ void HnswGraph::save_v2(const std::string& path) const {
 }
 
 std::unique_ptr<HnswGraph> HnswGraph::load_v2(const std::string& path) {
-    V2Map map(path);
-    const V2Header& h = map.header();
-    V2Checker check(map);
+    // Every section is read into memory this load owns and checked there, so what is checked is
+    // exactly what the graph keeps, whatever happens to the file meanwhile.
+    V2File file(path);
+    const V2Header& h = file.header();
+    V2Checker check(file);
     check.levels_and_offsets();
-    check.layer0(reinterpret_cast<const std::uint32_t*>(map.section(kV2Layer0)), 0, h.n);
-    check.upper(reinterpret_cast<const std::uint32_t*>(map.section(kV2Upper)), 0, h.upper_blocks);
-    check.vectors(reinterpret_cast<const float*>(map.section(kV2Vectors)), 0, h.n);
-    check.finish();
     auto g = std::make_unique<HnswGraph>(h.dim, static_cast<Metric>(h.metric),
                                          GraphParams{h.M, h.ef_construction, h.seed});
+    g->layer0_.resize(h.bytes[kV2Layer0] / 4);
+    file.read(kV2Layer0, 0, g->layer0_.data(), h.bytes[kV2Layer0]);
+    check.layer0(g->layer0_.data(), 0, h.n);
+    g->upper_.resize(h.bytes[kV2Upper] / 4);
+    file.read(kV2Upper, 0, g->upper_.data(), h.bytes[kV2Upper]);
+    check.upper(g->upper_.data(), 0, h.upper_blocks);
+    g->vectors_.resize(h.n * h.dim);
+    const std::uint64_t chunk = std::max<std::uint64_t>(1, (std::uint64_t{16} << 20) / (h.ld * 4));
+    std::vector<float> rows(std::min(chunk, h.n) * h.ld);
+    for (std::uint64_t r0 = 0; r0 < h.n; r0 += chunk) {
+        const std::uint64_t m = std::min(chunk, h.n - r0);
+        file.read(kV2Vectors, r0 * h.ld * 4, rows.data(), m * h.ld * 4);
+        check.vectors(rows.data(), r0, m);
+        for (std::uint64_t i = 0; i < m; ++i) {
+            std::memcpy(g->vectors_.data() + (r0 + i) * h.dim, rows.data() + i * h.ld, h.dim * 4);
+        }
+    }
+    check.finish();
     g->levels_ = check.levels();
     g->upper_offset_ = check.offsets();
     g->touched_.assign(h.n, 0u);
-    const auto* l0 = reinterpret_cast<const Id*>(map.section(kV2Layer0));
-    g->layer0_.assign(l0, l0 + h.bytes[kV2Layer0] / 4);
-    const auto* up = reinterpret_cast<const Id*>(map.section(kV2Upper));
-    g->upper_.assign(up, up + h.bytes[kV2Upper] / 4);
-    g->vectors_.resize(h.n * h.dim);
-    const auto* vec = reinterpret_cast<const float*>(map.section(kV2Vectors));
-    for (std::uint64_t i = 0; i < h.n; ++i) {
-        std::memcpy(g->vectors_.data() + i * h.dim, vec + i * h.ld, h.dim * 4);
-    }
     g->entry_point_ = static_cast<Id>(h.entry_point);
     g->max_level_ = h.max_level;
     g->rng_.discard(h.n);  // each stored node drew its level once

@@ -3,8 +3,8 @@
 // FORMAT v2: this library's aligned, mappable index file (a new version of docs/FORMAT.md's
 // format under its versioning rule: same magic, version at offset 8). v1 packs the sections with
 // no padding and is meant to be streamed into memory the index owns; v2 lays every section out
-// as the device copy holds it, each at a 4 KiB-aligned offset, so a loader can map the file and
-// copy the sections to HBM as they are, checking them chunk by chunk on the way.
+// as the device copy holds it, each at a 4 KiB-aligned offset, so a file can be mapped, and a
+// loader copies the sections to HBM as they are, checking them chunk by chunk on the way.
 //
 //   offset 0     header (4096 bytes)
 //     0  magic \x89VDXHNSW        8  u32 version = 2     12 u32 metric
@@ -49,21 +49,25 @@ struct V2Header {
 // The layout a header with these fields must have (offsets and sizes filled in).
 void v2_layout(V2Header& h, const std::string& path);
 
-// The file mapped read-only, its header checked (magic, version, ranges, header checksum, the
-// exact layout, the file size). Section contents are checked by V2Checker.
-class V2Map {
+// The file open for reading, its header checked (magic, version, ranges, header checksum, the
+// exact layout, the file size). Section bytes are copied out with read() into memory the caller
+// owns (a file that shrinks meanwhile is a runtime error, not a fault); V2Checker checks those
+// copies, so what a loader keeps is what was checked.
+class V2File {
 public:
-    explicit V2Map(const std::string& path);
-    ~V2Map();
-    V2Map(const V2Map&) = delete;
-    V2Map& operator=(const V2Map&) = delete;
+    explicit V2File(const std::string& path);
+    ~V2File();
+    V2File(const V2File&) = delete;
+    V2File& operator=(const V2File&) = delete;
     const V2Header& header() const noexcept { return h_; }
-    const unsigned char* section(int s) const noexcept { return base_ + h_.offset[s]; }
+    // bytes [at, at + bytes) of section s into dst
+    void read(int s, std::uint64_t at, void* dst, std::uint64_t bytes) const;
     const std::string& path() const noexcept { return path_; }
 
 private:
+    void read_bytes(std::uint64_t at, void* dst, std::uint64_t bytes) const;
     std::string path_;
-    const unsigned char* base_ = nullptr;
+    int fd_ = -1;
     std::uint64_t size_ = 0;
     V2Header h_;
 };
@@ -74,7 +78,7 @@ private:
 // std::runtime_error starting with the path.
 class V2Checker {
 public:
-    explicit V2Checker(const V2Map& map);
+    explicit V2Checker(const V2File& file);
     void levels_and_offsets();
     void layer0(const std::uint32_t* blocks, std::uint64_t first_node, std::uint64_t nodes);
     void upper(const std::uint32_t* blocks, std::uint64_t first_block, std::uint64_t count);
@@ -87,7 +91,7 @@ private:
     void links(const std::uint32_t* b, std::uint64_t node, std::uint64_t layer, std::uint64_t cap);
     void advance(int s, std::uint64_t expect_first, std::uint64_t first, const void* data, std::uint64_t bytes);
 
-    const V2Map& map_;
+    const V2File& map_;
     const V2Header& h_;
     std::vector<std::uint8_t> levels_;
     std::vector<std::uint32_t> offsets_;

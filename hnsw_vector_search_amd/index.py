@@ -52,16 +52,19 @@ class GraphReport:
 
 
 class _RWLock:
-    """Many readers or one writer (the binding's std::shared_mutex, python/bindings.cpp:312)."""
+    """Many readers or one writer (the binding's std::shared_mutex, python/bindings.cpp:312).
+    A waiting writer holds off new readers: searches that re-take the lock back to back would
+    otherwise keep the reader count above zero and starve add_items forever."""
 
     def __init__(self):
         self._cond = threading.Condition()
         self._readers = 0
         self._writer = False
+        self._writers_waiting = 0
 
     def acquire_read(self):
         with self._cond:
-            while self._writer:
+            while self._writer or self._writers_waiting:
                 self._cond.wait()
             self._readers += 1
 
@@ -73,8 +76,12 @@ class _RWLock:
 
     def acquire_write(self):
         with self._cond:
-            while self._writer or self._readers:
-                self._cond.wait()
+            self._writers_waiting += 1
+            try:
+                while self._writer or self._readers:
+                    self._cond.wait()
+            finally:
+                self._writers_waiting -= 1
             self._writer = True
 
     def release_write(self):

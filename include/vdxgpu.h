@@ -274,6 +274,8 @@ enum vdx_stat {
     VDX_STAT_DUPLICATES = 3,   /* recomputed members of the result list the lossy visited cache let through */
     VDX_STAT_EARLY_STOPS = 4,  /* queries the learned model stopped before the fixed-ef stop */
     VDX_STAT_LINK_PREFETCHED = 5, /* layer-0 link blocks that were already prefetched when needed */
+    VDX_STAT_CLAMPED = 6,      /* queries whose max(ef, k) exceeded the launch's result-list capacity
+                                  (the device entry point's `ef` bound) and ran at that capacity */
     VDX_STAT_COUNT = 8
 };
 
@@ -290,7 +292,12 @@ VDX_API int vdx_gpu_hnsw_search(vdx_gpu_hnsw* index, const float* queries, uint6
 
 /* The same on device memory, asynchronous on `stream` (the context's stream if NULL):
  * d_queries are already prepared (validated; unit length under cosine), nq x dim floats.
- * d_ks / d_efs may be NULL. max_k bounds every k (it sizes nothing; it is checked). */
+ * d_ks / d_efs may be NULL. max_k bounds every k (it sizes nothing; it is checked). The launch's
+ * result-list capacity is sized from max(ef, max_k): with d_efs, `ef` must be at least every
+ * d_efs value (ef = 0 is rejected); a query whose max(ef, k) is above the capacity runs at the
+ * capacity and is counted in VDX_STAT_CLAMPED (opts.stats). Each call takes one of 64 queue
+ * counter slots in turn and first waits for the launch that last used that slot, so any number
+ * of calls may be in flight on different streams. */
 VDX_API int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* index, const float* d_queries, uint64_t nq,
                                const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs,
                                uint32_t ef, uint32_t max_k, vdx_neighbor* d_out,

@@ -157,3 +157,27 @@ def test_rows_added_after_the_upload_are_appended(metric, dim):
             assert (cnt == 10).all()
             assert got.tobytes() == ref.tobytes(), (step, precision)
         assert index.gpu_handle().value == handle
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp32", "bf16", "fp8"])
+@pytest.mark.parametrize("workload", ["gmm1024_l2", "aniso_cosine"])
+def test_modes_equal_the_oracle_on_the_config_generators(precision, workload):
+    """The precision modes on the brute-force workloads' own generators (the 1,024-component
+    mixture under L2, the anisotropic cosine base at dimension 100), 60,000 rows, k in
+    {1, 10, 100}: ids and distance bits equal the oracle's restatement of
+    BruteForceIndex::search directly (src/brute_force.cpp:20-37, 68-84)."""
+    print(f"case: bf {precision} on {workload}, 60000 rows", flush=True)
+    if workload == "gmm1024_l2":
+        mix = datasets.gmm1024()
+        base, queries, metric, dim = mix.sample(60_000, 1), mix.sample(200, 2), 0, 128
+    else:
+        gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
+        base, queries, metric, dim = gen.sample(60_000, 1), gen.sample(200, 2), 2, 100
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(base)
+    prepared = o.normalize(queries) if metric == 2 else queries
+    for k in (1, 10, 100):
+        ref = o.bruteforce_f32(index.vectors(), prepared, k, metric=metric, threads=16)
+        got, cnt = index.search_arrays(queries, k, precision)
+        assert (cnt == k).all()
+        assert got.tobytes() == ref.tobytes(), (precision, workload, k)

@@ -134,3 +134,78 @@ def test_training_and_calibrated_learned_search(gmm10k):
           f"(fixed ef=128: {fixed_stats[0] / nq:.1f})")
     assert r >= 0.93
     assert stats[0] < fixed_stats[0]
+
+
+# ---- the workloads' own generators: the 1,024-component mixture (L2) and the anisotropic
+#      cosine base, at 40,000 rows, judged against the oracle's brute force
+
+@pytest.fixture(scope="module", params=["gmm1024_l2", "aniso_cosine"])
+def config_base(request):
+    from oracle import oracle as o
+    if request.param == "gmm1024_l2":
+        mix = datasets.gmm1024()
+        base, queries, metric, dim = mix.sample(40_000, 1), mix.sample(3000, 2), vdx.Metric.L2, 128
+        stored = base
+    else:
+        gen = datasets.AnisotropicGaussian(100, 0.8, seed=0)
+        base, metric, dim = gen.sample(40_000, 1), vdx.Metric.Cosine, 100
+        queries, stored = o.normalize(gen.sample(3000, 2)), o.normalize(base)
+    print(f"case: learned stop on {request.param}, 40000 rows", flush=True)
+    index = vdx.HnswIndex(dim, metric, M=16, ef_construction=200, seed=0)
+    index.add(base, num_threads=16)
+    truth = o.bruteforce_f32(stored, queries, 10, metric=metric, threads=16)["id"].astype(np.uint32)
+    return request.param, index, queries, truth
+
+
+def _device_search(index, queries, k, ef, opts=None):
+    ctx = nat.Context.get(0)
+    nq = len(queries)
+    dq = nat.DeviceArray.from_numpy(np.ascontiguousarray(queries, dtype=np.float32), ctx)
+    out = nat.DeviceArray((nq, k), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(nq, np.uint32, ctx)
+    learned.search_device(index, dq, nq, k, ef, out, cnt, opts)
+    return out.numpy(), cnt.numpy()
+
+
+def test_never_stop_model_equals_the_oracle_on_the_config_generators(config_base):
+    """The learned kernel with a model that never stops is the fixed-ef search: identical to the
+    oracle's restatement of vecdex's search at the cap, for k 10 and 100."""
+    from oracle import oracle as o
+    name, index, queries, _ = config_base
+    params = learned.init_params(np.zeros(16), np.ones(16))
+    params[-1] = -100.0
+    never = learned.StopModel(params)
+    g = index.graph_arrays()
+    for k, ef in ((10, 64), (10, 128), (100, 128)):
+        got, cnt = _device_search(index, queries[:500], k, ef, learned.learned_opts(never, 0.5))
+        ref, ref_cnt = o.hnsw_search(g, queries[:500], k, ef, threads=16)
+        np.testing.assert_array_equal(cnt, ref_cnt)
+        assert got.tobytes() == ref.tobytes(), (name, k, ef)
+
+
+def test_learned_stop_on_the_config_generators(config_base):
+    """Traces of 2,000 training queries labelled by the oracle's brute force, a model trained on
+    them, a threshold calibrated on 500 held-out queries, then 500 evaluation queries: recall
+    against brute force within 0.02 of the calibration target, with fewer expansions than the
+    fixed-ef search it stops early."""
+    name, index, queries, truth = config_base
+    train, calib, evalq = queries[:2000], queries[2000:2500], queries[2500:]
+    cap = 128
+    f, labels, _, _ = learned.collect_traces(index, train, truth[:2000], k=10, ef=cap)
+    model, losses = learned.train_stop_model(f, labels, epochs=6, batch=4096, lr=1e-3)
+    assert losses[-1] < losses[0]
+    fixed_calib, _ = _device_search(index, calib, 10, cap)
+    target = min(0.95, learned.recall_of(fixed_calib["id"], truth[2000:2500]) - 0.01)
+    thr, rec, work, table = learned.calibrate(index, model, calib, truth[2000:2500], 10, cap, target)
+    stats = (ctypes.c_uint64 * 8)()
+    got, _ = _device_search(index, evalq, 10, cap, learned.learned_opts(model, thr, 1, stats))
+    fixed_stats = (ctypes.c_uint64 * 8)()
+    opts = nat.SearchOpts()
+    opts.stats = ctypes.cast(fixed_stats, ctypes.c_void_p)
+    fixed, _ = _device_search(index, evalq, 10, cap, opts)
+    r = learned.recall_of(got["id"], truth[2500:])
+    r_fixed = learned.recall_of(fixed["id"], truth[2500:])
+    print(f"  {name}: target {target:.4f}, threshold {thr}, eval recall {r:.4f} (fixed ef {cap}: "
+          f"{r_fixed:.4f}), expansions {stats[0] / 500:.1f} vs {fixed_stats[0] / 500:.1f}", flush=True)
+    assert r >= target - 0.02
+    assert stats[0] < fixed_stats[0]
