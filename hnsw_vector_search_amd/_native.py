@@ -37,6 +37,7 @@ PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "exact": 3}
 STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates", "early_stops", "link_prefetched",
               "clamped")
 MLP_PARAMS = 5345
+NOALLOC_MAX_QUERIES = 1 << 20
 
 
 class GraphView(ctypes.Structure):
@@ -159,6 +160,8 @@ _SIGS = {
                                        c_ptr, c_int, c_ptr]),
     "vdx_gpu_hnsw_batch_classes": (c_int, [c_ptr, ctypes.POINTER(BatchClass), c_u32,
                                            ctypes.POINTER(c_u32)]),
+    "vdx_gpu_hnsw_search_uneven_noalloc": (c_int, [c_ptr, c_ptr, c_u64, c_ptr, c_u32, c_ptr, c_u32, c_u32, c_u32,
+                                                   c_ptr, c_u64, c_ptr, ctypes.POINTER(SearchOpts), c_ptr]),
     "vdx_gpu_bf_upload": (c_int, [c_ptr, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
     "vdx_gpu_bf_append": (c_int, [c_ptr, c_ptr, c_u64, c_u64]),
     "vdx_gpu_bf_free": (None, [c_ptr]),

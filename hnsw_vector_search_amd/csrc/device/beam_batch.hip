@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <stdexcept>
 #include <vector>
 
 #include "../host/capi_common.hpp"
@@ -111,6 +113,57 @@ __global__ __launch_bounds__(1024) void beam_classify_kernel(ClassifyArgs a) {
 
 }  // namespace vdx::dev
 
+// The per-call scratch of vdx_gpu_hnsw_search_uneven_noalloc: one call's classification (the
+// work counters, the class ranges, the query order by class) for up to
+// VDX_NOALLOC_MAX_QUERIES queries, 4 MiB on every GPU the library runs on, held for the life
+// of the process. In host code `scratch` names a host-side stand-in: only the address that
+// hipGetSymbolAddress returns for a device (noalloc_scratch_attach) may reach a kernel.
+namespace {
+constexpr std::uint32_t kScratchQueues = 0;   // kMaxClasses work counters
+constexpr std::uint32_t kScratchRanges = 16;  // 2 x kMaxClasses words
+constexpr std::uint32_t kScratchOrder = 64;   // the query order, VDX_NOALLOC_MAX_QUERIES words
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) std::uint32_t scratch[kScratchOrder + VDX_NOALLOC_MAX_QUERIES];
+
+namespace vdx::gpu {
+namespace {
+std::mutex g_scratch_mutex;
+bool g_scratch_done[kScratchDevices] = {};
+hipError_t g_scratch_status[kScratchDevices] = {};
+std::uint32_t* g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+void noalloc_scratch_attach(int device) {
+    if (device < 0 || device >= kScratchDevices) return;  // the entry point reports it
+    const std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // on the current device
+    g_scratch_status[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? static_cast<std::uint32_t*>(p) : nullptr;
+    g_scratch_done[device] = true;
+}
+
+namespace {
+std::uint32_t* noalloc_scratch(int device) {
+    if (device < 0 || device >= kScratchDevices) {
+        throw std::runtime_error(vdx::fmt("vdx_gpu_hnsw_search_uneven_noalloc: device %d is beyond the %d this "
+                                          "build keeps scratch addresses for", device, kScratchDevices));
+    }
+    const std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_done[device] || g_scratch_ptr[device] == nullptr) {
+        throw std::runtime_error(vdx::fmt("vdx_gpu_hnsw_search_uneven_noalloc: the scratch array of device %d "
+                                          "could not be located (%s)", device,
+                                          g_scratch_done[device] ? hipGetErrorName(g_scratch_status[device])
+                                                                 : "the device was never opened"));
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+}  // namespace vdx::gpu
+
 using vdx::guarded;
 using vdx::require;
 using vdx::gpu::check;
@@ -177,36 +230,73 @@ void check_call(const vdx_gpu_batch& b, const BatchCall& c) {
     require(c.out_stride >= b.max_k, "vdx_gpu_hnsw_batch: out_stride is below max_k");
 }
 
-void enqueue_classify(vdx_gpu_batch& b, const BatchCall& c, hipStream_t s) {
+// Where one call's classification lives: the query indices grouped by class, two range words
+// and one work counter per class (a batch's own buffers, or the no-allocation scratch).
+struct ClassBuffers {
+    std::uint32_t* order;
+    std::uint32_t* ranges;
+    std::uint32_t* queues;
+};
+
+// The R-capacity classes of batches with k <= max_k and ef <= max_ef: 64, 128, 256, 512 (the
+// register-resident kernel, up to the first that holds both), then one LDS-kernel class.
+int class_caps(std::uint32_t max_k, std::uint32_t max_ef, std::uint32_t (&caps)[vdx::dev::kMaxClasses]) {
+    const std::uint32_t top = std::max(max_ef, max_k);
+    int n = 0;
+    for (std::uint32_t cap = 64; cap <= vdx::dev::kRegMaxEf; cap *= 2) {
+        caps[n++] = cap;
+        if (cap >= top) break;
+    }
+    if (top > vdx::dev::kRegMaxEf) {
+        caps[n++] = (top + 63u) / 64u * 64u;
+    }
+    return n;
+}
+
+void enqueue_classify_into(const BatchCall& c, std::uint32_t k_cap, int ncls,
+                           const std::uint32_t (&caps)[vdx::dev::kMaxClasses], const ClassBuffers& buf,
+                           hipStream_t s) {
     vdx::dev::ClassifyArgs a{};
     a.nq = static_cast<std::uint32_t>(c.nq);
     a.ks = c.d_ks;
     a.k = c.k;
     a.efs = c.d_efs;
     a.ef = c.ef;
-    a.k_cap = b.max_k;
-    a.ncls = static_cast<std::uint32_t>(b.ncls);
-    for (int i = 0; i < b.ncls; ++i) a.caps[i] = b.caps[i];
-    a.order = b.order.as<std::uint32_t>();
-    a.ranges = b.ranges.as<std::uint32_t>();
-    a.queues = b.queues.as<std::uint32_t>();
+    a.k_cap = k_cap;
+    a.ncls = static_cast<std::uint32_t>(ncls);
+    for (int i = 0; i < ncls; ++i) a.caps[i] = caps[i];
+    a.order = buf.order;
+    a.ranges = buf.ranges;
+    a.queues = buf.queues;
     hipLaunchKernelGGL(vdx::dev::beam_classify_kernel, dim3(1), dim3(1024), 0, s, a);
     check(hipGetLastError(), "beam_classify launch");
 }
 
-// The launch of class `cls` on stream s (its work counter must be zero).
+// The launch of class `cls` (R capacity `cap`) on stream s; its work counter must be zero.
+void enqueue_class_into(vdx_gpu_hnsw& h, int mode, const vdx_search_opts* opts, std::uint32_t cap,
+                        std::uint32_t k_cap, int cls, const vdx::gpu::BeamConfig& cfg, const BatchCall& c,
+                        const ClassBuffers& buf, hipStream_t s) {
+    const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, cap, c.nq, cfg, mode);
+    vdx::dev::BeamArgs a = vdx::gpu::beam_args(
+        h, plan, mode, opts, c.d_queries, c.nq, c.d_ks, c.k, c.d_efs, c.ef,
+        reinterpret_cast<vdx::dev::Neighbor*>(c.d_out), c.out_stride, c.d_count, buf.queues + cls, nullptr);
+    a.k_cap = std::min<std::uint32_t>(a.k_cap, k_cap);
+    a.order = buf.order;
+    a.range = buf.ranges + 2 * cls;
+    vdx::gpu::launch_beam(h, plan, mode, a, s);
+}
+
+ClassBuffers buffers_of(vdx_gpu_batch& b) {
+    return {b.order.as<std::uint32_t>(), b.ranges.as<std::uint32_t>(), b.queues.as<std::uint32_t>()};
+}
+
+void enqueue_classify(vdx_gpu_batch& b, const BatchCall& c, hipStream_t s) {
+    enqueue_classify_into(c, b.max_k, b.ncls, b.caps, buffers_of(b), s);
+}
+
 void enqueue_class(vdx_gpu_batch& b, int cls, const vdx::gpu::BeamConfig& cfg, const BatchCall& c,
                    hipStream_t s) {
-    vdx_gpu_hnsw& h = *b.index;
-    const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, b.caps[cls], c.nq, cfg, b.mode);
-    vdx::dev::BeamArgs a = vdx::gpu::beam_args(
-        h, plan, b.mode, b.opts_ptr(), c.d_queries, c.nq, c.d_ks, c.k, c.d_efs, c.ef,
-        reinterpret_cast<vdx::dev::Neighbor*>(c.d_out), c.out_stride, c.d_count,
-        b.queues.as<std::uint32_t>() + cls, nullptr);
-    a.k_cap = std::min<std::uint32_t>(a.k_cap, b.max_k);
-    a.order = b.order.as<std::uint32_t>();
-    a.range = b.ranges.as<std::uint32_t>() + 2 * cls;
-    vdx::gpu::launch_beam(h, plan, b.mode, a, s);
+    enqueue_class_into(*b.index, b.mode, b.opts_ptr(), b.caps[cls], b.max_k, cls, cfg, c, buffers_of(b), s);
 }
 
 // classify, then every class on its own stream, joined back to s.
@@ -244,13 +334,7 @@ int vdx_gpu_hnsw_batch_create(vdx_gpu_hnsw* index, uint64_t max_nq, uint32_t max
         b->max_nq = max_nq;
         b->max_k = max_k;
         b->max_ef = max_ef;
-        for (std::uint32_t cap = 64; cap <= vdx::dev::kRegMaxEf; cap *= 2) {
-            b->caps[b->ncls++] = cap;
-            if (cap >= top) break;
-        }
-        if (top > vdx::dev::kRegMaxEf) {
-            b->caps[b->ncls++] = (top + 63u) / 64u * 64u;
-        }
+        b->ncls = class_caps(max_k, max_ef, b->caps);
         const vdx::gpu::BeamConfig base = vdx::gpu::config_from(opts);
         for (int c = 0; c < b->ncls; ++c) {
             b->cfg[c] = base;
@@ -420,6 +504,45 @@ int vdx_gpu_hnsw_batch_classes(const vdx_gpu_batch* b, vdx_batch_class* out, uin
                 info.blocks_per_cu = b->cfg[c].blocks_per_cu;
             }
             out[c] = info;
+        }
+    });
+}
+
+int vdx_gpu_hnsw_search_uneven_noalloc(vdx_gpu_hnsw* index, const float* d_queries, uint64_t nq,
+                                       const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs, uint32_t ef,
+                                       uint32_t max_k, uint32_t max_ef, vdx_neighbor* d_out, uint64_t out_stride,
+                                       uint32_t* d_out_count, const vdx_search_opts* opts, void* stream) {
+    return guarded([&] {
+        require(index != nullptr, "vdx_gpu_hnsw_search_uneven_noalloc: index is null");
+        if (nq == 0) return;
+        if (nq > VDX_NOALLOC_MAX_QUERIES) {
+            throw std::invalid_argument(vdx::fmt("vdx_gpu_hnsw_search_uneven_noalloc: %llu queries is above the "
+                                                 "scratch limit of %u per call",
+                                                 static_cast<unsigned long long>(nq), VDX_NOALLOC_MAX_QUERIES));
+        }
+        require(max_k >= 1, "vdx_gpu_hnsw_search_uneven_noalloc: max_k must be positive");
+        require(std::max(max_ef, max_k) <= 8192, "vdx_gpu_hnsw_search_uneven_noalloc: ef above the GPU path's limit");
+        require(index->n > 0, "vdx_gpu_hnsw_search_uneven_noalloc: empty index");
+        require(opts == nullptr || opts->stats == nullptr,
+                "vdx_gpu_hnsw_search_uneven_noalloc: opts.stats is not collected by this entry point");
+        require(d_queries != nullptr && d_out != nullptr && d_out_count != nullptr,
+                "vdx_gpu_hnsw_search_uneven_noalloc: null buffer");
+        require(d_ks != nullptr || (k >= 1 && k <= max_k), "vdx_gpu_hnsw_search_uneven_noalloc: k outside [1, max_k]");
+        require(d_efs != nullptr || ef <= max_ef, "vdx_gpu_hnsw_search_uneven_noalloc: ef above max_ef");
+        require(out_stride >= max_k, "vdx_gpu_hnsw_search_uneven_noalloc: out_stride is below max_k");
+        const int mode = vdx::gpu::beam_mode(opts, false);
+        const vdx::gpu::BeamConfig cfg = vdx::gpu::config_from(opts);
+        std::uint32_t caps[vdx::dev::kMaxClasses] = {};
+        const int ncls = class_caps(max_k, max_ef, caps);
+        DeviceGuard guard(index->ctx->device);
+        std::uint32_t* sc = vdx::gpu::noalloc_scratch(index->ctx->device);
+        const ClassBuffers buf{sc + kScratchOrder, sc + kScratchRanges, sc + kScratchQueues};
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : index->ctx->stream;
+        const BatchCall c{d_queries, nq, d_ks, k, d_efs, ef, d_out, out_stride, d_out_count};
+        // the classify kernel zeroes the work counters itself, so nothing is cleared here
+        enqueue_classify_into(c, max_k, ncls, caps, buf, s);
+        for (int cls = 0; cls < ncls; ++cls) {
+            enqueue_class_into(*index, mode, opts, caps[cls], max_k, cls, cfg, c, buf, s);
         }
     });
 }

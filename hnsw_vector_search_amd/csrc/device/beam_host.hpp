@@ -50,4 +50,8 @@ void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vd
 
 BeamConfig config_from(const vdx_search_opts* opts);
 
+// Locates the no-allocation entry point's scratch array on `device` (which must be current),
+// once per device; vdx_gpu_open calls it.
+void noalloc_scratch_attach(int device);
+
 }  // namespace vdx::gpu

@@ -464,6 +464,7 @@ int vdx_gpu_open(int device, vdx_gpu_ctx** out) {
         ctx->cu_count = prop.multiProcessorCount;
         ctx->lds_per_block = prop.sharedMemPerBlock;
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        vdx::gpu::noalloc_scratch_attach(device);  // with the device current
         *out = ctx.release();
     });
 }

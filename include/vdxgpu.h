@@ -374,6 +374,26 @@ VDX_API int vdx_gpu_hnsw_batch_run(vdx_gpu_batch* batch, const float* d_queries,
 VDX_API int vdx_gpu_hnsw_batch_classes(const vdx_gpu_batch* batch, vdx_batch_class* out, uint32_t capacity,
                                        uint32_t* count);
 
+/* No counterpart in vecdex: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). vdx_gpu_hnsw_batch_run's uneven
+ * batch without a batch object: the same classes as a batch created with (nq, max_k, max_ef,
+ * opts), the same per-query results, enqueued on `stream` (the index context's stream if NULL)
+ * as one classify launch and one persistent launch per class, in turn on that stream, in default
+ * launch shapes. One call's scratch (work counters, class ranges, the query order) lives in a
+ * fixed device array of the library, 4 MiB per GPU, located once per device by vdx_gpu_open:
+ * the call makes no hipMalloc/hipFree and no other allocation on the device, and clears nothing
+ * with memset (the classify kernel zeroes the counters). nq above VDX_NOALLOC_MAX_QUERIES is
+ * rejected, as is opts.stats (not collected). Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (on different streams, or from different threads) race and must be
+ * serialized by the caller; calls queued on one stream are ordered by it. */
+#define VDX_NOALLOC_MAX_QUERIES 1048576u
+VDX_API int vdx_gpu_hnsw_search_uneven_noalloc(vdx_gpu_hnsw* index, const float* d_queries, uint64_t nq,
+                                               const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs,
+                                               uint32_t ef, uint32_t max_k, uint32_t max_ef,
+                                               vdx_neighbor* d_out, uint64_t out_stride,
+                                               uint32_t* d_out_count, const vdx_search_opts* opts,
+                                               void* stream);
+
 /* ---- Brute force: replaces BruteForceIndex::search (include/vecdex/brute_force.hpp:36;
  *      src/brute_force.cpp:68-84), batched over queries. */
 
