@@ -34,6 +34,7 @@ VDX_OK, VDX_INVALID_ARGUMENT, VDX_RUNTIME_ERROR = 0, 1, 2
 VDX_LENGTH_ERROR, VDX_OUT_OF_MEMORY, VDX_DEVICE_ERROR = 3, 4, 5
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2, "exact": 3}
+PREC_FP64_TIES, PREC_FP64_ERROR = 0x100, 0x200
 STAT_NAMES = ("expansions", "distances", "upper_blocks", "duplicates", "early_stops", "link_prefetched",
               "clamped")
 MLP_PARAMS = 5345
@@ -88,7 +89,9 @@ class BatchClass(ctypes.Structure):
 
 class BfStats(ctypes.Structure):
     _fields_ = [("shortlist", c_u64), ("widened", c_u64), ("exact_fallback", c_u64),
-                ("fp64_ties", c_u64), ("max_abs_error", c_double)]
+                ("fp64_ties", c_u64), ("max_abs_error", c_double), ("fp64_reordered", c_u64),
+                ("fp64_unresolved", c_u64), ("fp64_window", c_u64), ("max_err_fp64", c_double),
+                ("max_rel_err_fp64", c_double)]
 
 
 NEIGHBOR_DTYPE = np.dtype([("distance", np.float32), ("id", np.uint32)])

@@ -20,6 +20,7 @@
 #include "beam_args.hpp"
 #include "beam_host.hpp"
 #include "bf_args.hpp"
+#include "bf_fp64.hpp"
 #include "bf_host.hpp"
 #include "fill_copy.hpp"
 #include "gpu_index.hpp"
@@ -1101,20 +1102,78 @@ int vdx_gpu_bf_search_device(vdx_gpu_bf* b, const float* d_queries, uint64_t nq,
             check(hipStreamSynchronize(s), "bf search");
             return;
         }
-        switch (precision) {
-            case VDX_PREC_EXACT:
-                vdx::gpu::bf_exact(*b, d_queries, nq, k, reinterpret_cast<vdx::dev::Neighbor*>(d_out),
-                                   out_stride, d_out_count, *ws, s);
-                break;
-            case VDX_PREC_FP32:
-            case VDX_PREC_BF16:
-            case VDX_PREC_FP8:
-                vdx::gpu::bf_mfma(*b, static_cast<int>(precision), d_queries, nq, k,
-                                  reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride,
-                                  d_out_count, *ws, s, stats);
-                break;
-            default:
-                throw std::invalid_argument("BruteForceIndex::search: unknown precision mode");
+        const std::uint32_t mode = precision & 0xFFu;
+        const bool fp64_ties = (precision & VDX_PREC_FP64_TIES) != 0;
+        const bool fp64_error = (precision & VDX_PREC_FP64_ERROR) != 0;
+        if ((precision & ~(0xFFu | VDX_PREC_FP64_TIES | VDX_PREC_FP64_ERROR)) != 0 || mode > VDX_PREC_EXACT) {
+            throw std::invalid_argument("BruteForceIndex::search: unknown precision mode");
+        }
+        const auto run_mode = [&](std::uint32_t kk, vdx::dev::Neighbor* out, std::uint64_t stride, std::uint32_t* count) {
+            if (mode == VDX_PREC_EXACT) {
+                vdx::gpu::bf_exact(*b, d_queries, nq, kk, out, stride, count, *ws, s);
+            } else {
+                vdx::gpu::bf_mfma(*b, static_cast<int>(mode), d_queries, nq, kk, out, stride, count, *ws, s, stats);
+            }
+        };
+        if (!fp64_ties && !fp64_error) {
+            run_mode(k, reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count);
+            check(hipStreamSynchronize(s), "bf search");
+            return;
+        }
+        // The float64 pass over a window of each query's first m results: the k wanted, plus (for
+        // the tie re-rank) as many again (at least 8) so a tie group at the k-th result is whole;
+        // a query whose group reaches the window's end doubles the window, up to 1024.
+        const std::uint64_t keep = std::min<std::uint64_t>(k, b->n);
+        if (fp64_ties && keep > vdx::dev::kFp64MaxWindow / 2) {
+            throw std::invalid_argument(vdx::fmt("BruteForceIndex::search: the float64 tie re-rank takes k up to %u",
+                                                 vdx::dev::kFp64MaxWindow / 2));
+        }
+        std::uint64_t m = fp64_ties ? std::min<std::uint64_t>(b->n, keep + std::max<std::uint64_t>(keep, 8)) : keep;
+        unsigned long long counters[5] = {};
+        for (;;) {
+            ws->fp64[0].reserve(nq * m * sizeof(vdx::dev::Neighbor));
+            ws->fp64[1].reserve(nq * 4);
+            ws->fp64[2].reserve(nq * m * sizeof(double));
+            ws->fp64[3].reserve(sizeof(counters));
+            check(hipMemsetAsync(ws->fp64[3].as<void>(), 0, sizeof(counters), s), "hipMemsetAsync(fp64 counters)");
+            auto* window = ws->fp64[0].as<vdx::dev::Neighbor>();
+            run_mode(static_cast<std::uint32_t>(m), window, m, ws->fp64[1].as<std::uint32_t>());
+            vdx::dev::BfFp64Args a{};
+            a.vectors = b->vectors.as<float>();
+            a.ld = b->split.ld;
+            a.dim = static_cast<std::uint32_t>(b->dim);
+            a.metric = b->metric;
+            a.n = static_cast<std::uint32_t>(b->n);
+            a.queries = d_queries;
+            a.qld = static_cast<std::uint32_t>(b->dim);
+            a.nq = static_cast<std::uint32_t>(nq);
+            a.cand = window;
+            a.cand_count = ws->fp64[1].as<std::uint32_t>();
+            a.m = static_cast<std::uint32_t>(m);
+            a.d64 = ws->fp64[2].as<double>();
+            a.out = reinterpret_cast<vdx::dev::Neighbor*>(d_out);
+            a.out_stride = out_stride;
+            a.out_count = d_out_count;
+            a.keep = static_cast<std::uint32_t>(keep);
+            a.ties = fp64_ties ? 1u : 0u;
+            a.stats = ws->fp64[3].as<unsigned long long>();
+            check(vdx::dev::launch_bf_fp64(a, s), "bf_fp64 launch");
+            check(hipMemcpyAsync(counters, ws->fp64[3].as<void>(), sizeof(counters), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(fp64 counters)");
+            check(hipStreamSynchronize(s), "bf search (float64 pass)");
+            if (!fp64_ties || counters[0] == 0 || m >= b->n || m >= vdx::dev::kFp64MaxWindow) break;
+            m = std::min<std::uint64_t>({b->n, 2 * m, vdx::dev::kFp64MaxWindow});
+        }
+        if (stats) {
+            double err = 0.0, rel = 0.0;
+            std::memcpy(&err, &counters[3], sizeof(err));
+            std::memcpy(&rel, &counters[4], sizeof(rel));
+            stats->fp64_unresolved = fp64_ties ? counters[0] : 0;
+            stats->fp64_ties = counters[1];
+            stats->fp64_reordered = counters[2];
+            stats->fp64_window = m;
+            stats->max_err_fp64 = err;
+            stats->max_rel_err_fp64 = rel;
         }
         check(hipStreamSynchronize(s), "bf search");
     });

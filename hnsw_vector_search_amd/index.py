@@ -465,8 +465,8 @@ class BruteForceIndex:
     """vecdex::BruteForceIndex (include/vecdex/brute_force.hpp:21-43) with a batched GPU search.
 
     search(query, k) returns the exact top-k (bit-equal to vecdex's). `precision` picks the
-    GPU method: "exact" (every distance in the exact order), or an MFMA shortlist ("fp32",
-    "bf16", "fp8") re-ranked exactly.
+    GPU method: an MFMA shortlist ("bf16", the default, "fp32", "fp8") re-ranked exactly and
+    certified, or "exact" (every distance in the exact order); all return the same result.
     """
 
     def __init__(self, dim: int, metric: int = Metric.L2, *, device: int = 0):
@@ -532,8 +532,12 @@ class BruteForceIndex:
             nat.lib.vdx_gpu_bf_free(self._gpu)
             self._gpu = None
 
-    def search_arrays(self, queries, k: int, precision: str = "exact",
-                      where: str = "BruteForceIndex::search"):
+    def search_arrays(self, queries, k: int, precision: str = "bf16",
+                      where: str = "BruteForceIndex::search", fp64: str | None = None):
+        """fp64: None; "error" also measures the returned distances against float64 on the GPU
+        (last_stats.max_err_fp64); "ties" re-ranks candidates whose float32 distances chain
+        within one ulp by their float64 distances (and measures)."""
+        flags = {None: 0, "error": nat.PREC_FP64_ERROR, "ties": nat.PREC_FP64_TIES}[fp64]
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q.reshape(1, -1)
@@ -545,16 +549,16 @@ class BruteForceIndex:
             return out, count
         self.last_stats = nat.BfStats()
         nat.check(nat.lib.vdx_gpu_bf_search(
-            self.gpu_handle(), nat.ptr(q), nq, q.shape[1], int(k), nat.PRECISIONS[precision],
+            self.gpu_handle(), nat.ptr(q), nq, q.shape[1], int(k), nat.PRECISIONS[precision] | flags,
             nat.ptr(out), keep, nat.ptr(count), where.encode(), ctypes.byref(self.last_stats)))
         return out, count
 
-    def search(self, query, k: int, precision: str = "exact") -> list:
+    def search(self, query, k: int, precision: str = "bf16") -> list:
         out, count = self.search_arrays(np.asarray(query, dtype=np.float32).reshape(1, -1), k,
                                         precision)
         return [Neighbor(float(d), int(i)) for d, i in out[0][: count[0]]]
 
-    def search_batch(self, queries, k: int, precision: str = "exact") -> list:
+    def search_batch(self, queries, k: int, precision: str = "bf16") -> list:
         q = _as_matrix(queries, self._dim)
         out, count = self.search_arrays(q, k, precision)
         return [[Neighbor(float(d), int(i)) for d, i in out[r][: count[r]]]

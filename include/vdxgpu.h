@@ -403,7 +403,17 @@ enum vdx_precision {
     VDX_PREC_FP32 = 0, /* FP32 MFMA shortlist, exact re-rank */
     VDX_PREC_BF16 = 1, /* BF16 MFMA shortlist (FP32 accumulate), exact re-rank */
     VDX_PREC_FP8 = 2,  /* FP8-e4m3 MFMA with per-vector scales, exact re-rank */
-    VDX_PREC_EXACT = 3 /* no GEMM: every distance in the exact order (slow; for checks) */
+    VDX_PREC_EXACT = 3, /* no GEMM: every distance in the exact order (slow; for checks) */
+    /* Flags OR'ed into a mode above (no vecdex counterpart; vecdex's float64 yardstick is
+     * tests/brute_force_test.cpp:189-213):
+     * FP64_TIES: candidates whose float32 distances chain within one ulp of each other are
+     *   re-ranked on the GPU by their float64 distance (ties by smaller id) and the first k of
+     *   the re-ranked list returned, each with its own float32 distance (so results can differ
+     *   from BruteForceIndex::search exactly where float32 cannot order them); k <= 512.
+     * FP64_ERROR: the float64 distances of the returned neighbours are computed on the GPU and
+     *   the largest |float32 - float64| reported in vdx_bf_stats (results unchanged). */
+    VDX_PREC_FP64_TIES = 0x100,
+    VDX_PREC_FP64_ERROR = 0x200
 };
 
 /* Vectors as BruteForceIndex stores them (unit length under cosine). */
@@ -420,8 +430,14 @@ typedef struct vdx_bf_stats {
     uint64_t shortlist;        /* candidates kept per query in the first pass */
     uint64_t widened;          /* queries whose shortlist failed the bound and were redone */
     uint64_t exact_fallback;   /* queries finished by an exact scan */
-    uint64_t fp64_ties;        /* candidate pairs re-ranked in float64 (fp64_ties mode) */
+    uint64_t fp64_ties;        /* VDX_PREC_FP64_TIES: candidates in one-ulp groups re-ranked in float64 */
     double max_abs_error;      /* max |GEMM distance - exact distance| over re-ranked candidates */
+    uint64_t fp64_reordered;   /* VDX_PREC_FP64_TIES: queries whose order the re-rank changed */
+    uint64_t fp64_unresolved;  /* VDX_PREC_FP64_TIES: queries whose tie group at the k-th result
+                                  still reached the end of the largest window (1024 candidates) */
+    uint64_t fp64_window;      /* candidates per query the float64 pass examined */
+    double max_err_fp64;       /* FP64_TIES or FP64_ERROR: max |returned distance - float64 distance| */
+    double max_rel_err_fp64;   /* the same divided by 1 + |float64 distance| */
 } vdx_bf_stats;
 
 /* The min(k, n) closest stored vectors of each query, ascending by (distance, id); the

@@ -22,7 +22,7 @@ def pairs(result):
     return [(n.distance, n.id) for n in result]
 
 
-@pytest.mark.parametrize("precision", ["exact"])
+@pytest.mark.parametrize("precision", ["exact", "bf16", "fp32"])
 def test_hand_cases(precision):
     # tests/brute_force_test.cpp:46-73, 158-175
     idx = line_index([10, 0, 7, 3, -2])
