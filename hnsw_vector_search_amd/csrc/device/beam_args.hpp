@@ -27,6 +27,11 @@ struct BeamArgs {
     std::uint32_t nb, n8, tail, first_tail;
     // D128U8: the rows as bytes, 128 per row in the beam's lane order (pack_rows_u8)
     const unsigned char* rows_u8;
+    // D128U8: a device word that launch_query_bytes_check sets to qflag_epoch when some query of
+    // the launch has a component that is not an integer in [0, 255] (any other value: every
+    // query is byte-valued, the integer kernel serves the launch); null: the float arithmetic
+    const std::uint32_t* qflag;
+    std::uint32_t qflag_epoch;
     // the queries (prepared), per-query k and ef
     const float* queries;
     std::uint32_t qld, nq;
@@ -70,7 +75,8 @@ struct BeamArgs {
 
 constexpr int kBeamStats = 7;
 
-enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2 };
+enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2,
+                      kBeamU8Int = 3 /* kernel selector only: the plain search's integer kernel */ };
 
 // D128U8: dimension 128 under squared L2 with the rows read from the 8-bit copy
 // (BeamArgs::rows_u8); only the register-resident plain search has that variant, the other

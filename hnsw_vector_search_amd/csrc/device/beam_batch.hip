@@ -44,6 +44,7 @@ struct ClassifyArgs {
     std::uint32_t* order;             // nq: query indices grouped by class
     std::uint32_t* ranges;            // 2 per class: [begin, end) in order
     std::uint32_t* queues;            // 1 per class: zeroed
+    std::uint32_t* qflag;             // the byte-query flag of the launches (zeroed), or null
 };
 
 __device__ __forceinline__ std::uint32_t class_of(const ClassifyArgs& a, std::uint32_t q) {
@@ -88,6 +89,9 @@ __global__ __launch_bounds__(1024) void beam_classify_kernel(ClassifyArgs a) {
             a.ranges[2 * c + 1] = run;
             a.queues[c] = 0;
         }
+        if (a.qflag != nullptr) {
+            *a.qflag = 0;
+        }
     }
     __syncthreads();
     for (std::uint32_t q0 = wave * 64; q0 < a.nq; q0 += waves * 64) {
@@ -120,6 +124,7 @@ __global__ __launch_bounds__(1024) void beam_classify_kernel(ClassifyArgs a) {
 // hipGetSymbolAddress returns for a device (noalloc_scratch_attach) may reach a kernel.
 namespace {
 constexpr std::uint32_t kScratchQueues = 0;   // kMaxClasses work counters
+constexpr std::uint32_t kScratchQflag = 8;    // the byte-query flag
 constexpr std::uint32_t kScratchRanges = 16;  // 2 x kMaxClasses words
 constexpr std::uint32_t kScratchOrder = 64;   // the query order, VDX_NOALLOC_MAX_QUERIES words
 constexpr int kScratchDevices = 64;
@@ -236,6 +241,7 @@ struct ClassBuffers {
     std::uint32_t* order;
     std::uint32_t* ranges;
     std::uint32_t* queues;
+    std::uint32_t* qflag;  // the byte-query flag word the integer kernel reads
 };
 
 // The R-capacity classes of batches with k <= max_k and ef <= max_ef: 64, 128, 256, 512 (the
@@ -253,9 +259,11 @@ int class_caps(std::uint32_t max_k, std::uint32_t max_ef, std::uint32_t (&caps)[
     return n;
 }
 
-void enqueue_classify_into(const BatchCall& c, std::uint32_t k_cap, int ncls,
-                           const std::uint32_t (&caps)[vdx::dev::kMaxClasses], const ClassBuffers& buf,
-                           hipStream_t s) {
+// The classify launch, then (where the plain search reads the 8-bit rows) the check of every
+// query for byte values, into the flag the classify kernel zeroed.
+void enqueue_classify_into(const vdx_gpu_hnsw& h, int mode, const vdx::gpu::BeamConfig& cfg, const BatchCall& c,
+                           std::uint32_t k_cap, int ncls, const std::uint32_t (&caps)[vdx::dev::kMaxClasses],
+                           const ClassBuffers& buf, hipStream_t s) {
     vdx::dev::ClassifyArgs a{};
     a.nq = static_cast<std::uint32_t>(c.nq);
     a.ks = c.d_ks;
@@ -268,8 +276,15 @@ void enqueue_classify_into(const BatchCall& c, std::uint32_t k_cap, int ncls,
     a.order = buf.order;
     a.ranges = buf.ranges;
     a.queues = buf.queues;
+    const bool bytes = h.u8 && h.metric == 0 && h.dim == 128 && mode == vdx::dev::kBeamPlain && !cfg.f32_rows;
+    a.qflag = bytes ? buf.qflag : nullptr;
     hipLaunchKernelGGL(vdx::dev::beam_classify_kernel, dim3(1), dim3(1024), 0, s, a);
     check(hipGetLastError(), "beam_classify launch");
+    if (bytes) {
+        // the classify kernel zeroed the flag (a replayed graph repeats the same epoch, 1)
+        check(vdx::dev::launch_query_bytes_check(c.d_queries, c.nq * h.dim, buf.qflag, 1u, nullptr, s),
+              "query_bytes_check launch");
+    }
 }
 
 // The launch of class `cls` (R capacity `cap`) on stream s; its work counter must be zero.
@@ -283,15 +298,21 @@ void enqueue_class_into(vdx_gpu_hnsw& h, int mode, const vdx_search_opts* opts, 
     a.k_cap = std::min<std::uint32_t>(a.k_cap, k_cap);
     a.order = buf.order;
     a.range = buf.ranges + 2 * cls;
+    if (plan.shape == vdx::dev::BeamShape::D128U8) {
+        a.qflag = buf.qflag;
+        a.qflag_epoch = 1u;
+    }
     vdx::gpu::launch_beam(h, plan, mode, a, s);
 }
 
 ClassBuffers buffers_of(vdx_gpu_batch& b) {
-    return {b.order.as<std::uint32_t>(), b.ranges.as<std::uint32_t>(), b.queues.as<std::uint32_t>()};
+    return {b.order.as<std::uint32_t>(), b.ranges.as<std::uint32_t>(), b.queues.as<std::uint32_t>(),
+            b.queues.as<std::uint32_t>() + vdx::dev::kMaxClasses};
 }
 
 void enqueue_classify(vdx_gpu_batch& b, const BatchCall& c, hipStream_t s) {
-    enqueue_classify_into(c, b.max_k, b.ncls, b.caps, buffers_of(b), s);
+    enqueue_classify_into(*b.index, b.mode, vdx::gpu::config_from(b.opts_ptr()), c, b.max_k, b.ncls, b.caps,
+                          buffers_of(b), s);
 }
 
 void enqueue_class(vdx_gpu_batch& b, int cls, const vdx::gpu::BeamConfig& cfg, const BatchCall& c,
@@ -343,7 +364,7 @@ int vdx_gpu_hnsw_batch_create(vdx_gpu_hnsw* index, uint64_t max_nq, uint32_t max
         DeviceGuard guard(index->ctx->device);
         b->order.resize(max_nq * sizeof(std::uint32_t));
         b->ranges.resize(2 * vdx::dev::kMaxClasses * sizeof(std::uint32_t));
-        b->queues.resize(vdx::dev::kMaxClasses * sizeof(std::uint32_t));
+        b->queues.resize((vdx::dev::kMaxClasses + 1) * sizeof(std::uint32_t));  // the counters, the byte-query flag
         check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
         for (int c = 0; c < b->ncls; ++c) {
             check(hipStreamCreateWithFlags(&b->side[c], hipStreamNonBlocking), "hipStreamCreate");
@@ -536,11 +557,11 @@ int vdx_gpu_hnsw_search_uneven_noalloc(vdx_gpu_hnsw* index, const float* d_queri
         const int ncls = class_caps(max_k, max_ef, caps);
         DeviceGuard guard(index->ctx->device);
         std::uint32_t* sc = vdx::gpu::noalloc_scratch(index->ctx->device);
-        const ClassBuffers buf{sc + kScratchOrder, sc + kScratchRanges, sc + kScratchQueues};
+        const ClassBuffers buf{sc + kScratchOrder, sc + kScratchRanges, sc + kScratchQueues, sc + kScratchQflag};
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : index->ctx->stream;
         const BatchCall c{d_queries, nq, d_ks, k, d_efs, ef, d_out, out_stride, d_out_count};
         // the classify kernel zeroes the work counters itself, so nothing is cleared here
-        enqueue_classify_into(c, max_k, ncls, caps, buf, s);
+        enqueue_classify_into(*index, mode, cfg, c, max_k, ncls, caps, buf, s);
         for (int cls = 0; cls < ncls; ++cls) {
             enqueue_class_into(*index, mode, opts, caps[cls], max_k, cls, cfg, c, buf, s);
         }

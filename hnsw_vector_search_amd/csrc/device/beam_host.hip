@@ -102,6 +102,20 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     const std::uint64_t resident = static_cast<std::uint64_t>(per_cu) * h.ctx->cu_count;
     const std::uint64_t needed = (nq + p.waves - 1) / p.waves;
     p.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, std::min(resident, needed)));
+    p.grid_int = 0;
+    if (p.shape == vdx::dev::BeamShape::D128U8) {
+        int per_cu_int = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu_int, vdx::dev::hnsw_beam_reg_kernel_ptr(p.shape, vdx::dev::kBeamU8Int, p.chunks),
+                  p.waves * 64, p.lds_bytes),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu_int = std::max(per_cu_int, 1);
+        if (cfg.blocks_per_cu != 0) {
+            per_cu_int = std::min<int>(per_cu_int, static_cast<int>(cfg.blocks_per_cu));
+        }
+        const std::uint64_t resident_int = static_cast<std::uint64_t>(per_cu_int) * h.ctx->cu_count;
+        p.grid_int = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, std::min(resident_int, needed)));
+    }
     return p;
 }
 
@@ -159,6 +173,12 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
 void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vdx::dev::BeamArgs& a,
                  hipStream_t stream) {
     if (plan.chunks > 0) {
+        if (plan.shape == vdx::dev::BeamShape::D128U8 && mode == vdx::dev::kBeamPlain && a.qflag != nullptr) {
+            // the integer kernel and the float kernel, in turn: the flag lets exactly one work
+            check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid_int, plan.waves, plan.lds_bytes, stream,
+                                                 plan.shape, vdx::dev::kBeamU8Int, plan.chunks),
+                  "hnsw_beam_reg (integer) launch");
+        }
         check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,
                                              plan.shape, mode, plan.chunks),
               "hnsw_beam_reg launch");

@@ -23,6 +23,7 @@ struct BeamConfig {
 
 struct BeamPlan {
     std::uint32_t ef_cap, visited_log2, waves, grid;
+    std::uint32_t grid_int;  // D128U8: the grid of the integer kernel (hnsw_beam_reg_u8int_kernel)
     int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
     std::size_t lds_per_wave, lds_bytes;
     vdx::dev::BeamShape shape;  // D128U8: the plain register kernel over the 8-bit row copy

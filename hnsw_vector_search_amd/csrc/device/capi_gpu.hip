@@ -136,7 +136,19 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     }
     a.entries = d_entries;
     a.layer = layer;
-    check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
+    if (plan.shape == vdx::dev::BeamShape::D128U8) {
+        // the byte-query check also clears the work counter; its flag is the word after the
+        // counter (every caller's counter word has a free word after it), stamped with this
+        // launch's epoch, so it needs no clearing
+        std::uint32_t epoch = h.query_epoch.fetch_add(1) + 1;
+        if (epoch == 0) epoch = h.query_epoch.fetch_add(1) + 1;
+        check(vdx::dev::launch_query_bytes_check(d_queries, nq * h.dim, d_queue + 1, epoch, d_queue, stream),
+              "query_bytes_check launch");
+        a.qflag = d_queue + 1;
+        a.qflag_epoch = epoch;
+    } else {
+        check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
+    }
     vdx::gpu::launch_beam(h, plan, mode, a, stream);
 }
 

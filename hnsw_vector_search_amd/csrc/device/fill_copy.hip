@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "fill_copy.hpp"
@@ -94,6 +96,31 @@ unsigned blocks_for(std::uint64_t items) {
 }
 }  // namespace
 
+// A wave with any float that is not an integer in [0, 255] stamps the flag (one store per wave;
+// every writer writes the same value, so the flag needs no clearing between launches).
+__global__ __launch_bounds__(256) void query_bytes_check_kernel(const float* q, std::uint64_t count,
+                                                                std::uint32_t* flag, std::uint32_t epoch,
+                                                                std::uint32_t* zero) {
+    if (zero != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        *zero = 0u;
+    }
+    bool bad = false;
+    const std::uint64_t step = static_cast<std::uint64_t>(gridDim.x) * blockDim.x * 4;
+    for (std::uint64_t i = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; i < count;
+         i += step) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (i + j < count) {
+                const float v = q[i + j];
+                bad = bad || !(v >= 0.0f && v <= 255.0f && v == __builtin_floorf(v));
+            }
+        }
+    }
+    if (__ballot(bad) != 0 && __lane_id() == 0) {
+        *flag = epoch;
+    }
+}
+
 hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(count)), dim3(256), 0, s, dst, value, count);
@@ -132,6 +159,14 @@ hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hi
     if (!aligned) return hipErrorInvalidValue;
     hipLaunchKernelGGL(copy_bytes_kernel, dim3(blocks_for(bytes / 16 + 1)), dim3(256), 0, s,
                        static_cast<unsigned char*>(dst), static_cast<const unsigned char*>(src), bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_bytes_check(const float* queries, std::uint64_t count, std::uint32_t* flag,
+                                    std::uint32_t epoch, std::uint32_t* zero, hipStream_t s) {
+    const std::uint64_t blocks = std::max<std::uint64_t>(1, std::min<std::uint64_t>(1024, (count + 1023) / 1024));
+    hipLaunchKernelGGL(query_bytes_check_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, queries, count,
+                       flag, epoch, zero);
     return hipGetLastError();
 }
 

@@ -23,5 +23,11 @@ hipError_t launch_pack_rows_u8(unsigned char* dst, const float* src, std::uint32
                                std::uint64_t rows, std::uint32_t* bad, hipStream_t s);
 // dst and src 16-byte aligned (device allocations are).
 hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
+// Writes `epoch` to *flag if any of the `count` floats at `queries` is not an integer in
+// [0, 255] (the exactness condition of the byte rows' integer arithmetic), leaving it alone
+// otherwise; with `zero` non-null it also clears that word (a work counter of the launches after
+// it).
+hipError_t launch_query_bytes_check(const float* queries, std::uint64_t count, std::uint32_t* flag,
+                                    std::uint32_t epoch, std::uint32_t* zero, hipStream_t s);
 
 }  // namespace vdx::dev

@@ -296,4 +296,84 @@ __device__ __forceinline__ void distances_rows_u8_l2(const unsigned char* rows, 
     }
 }
 
+// Integer DPP partner moves within an 8-lane group (as dpp_xor1/dpp_xor2/dpp_next4).
+__device__ __forceinline__ int dpp_xor1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_xor2_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_next4_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, true); }
+
+// The query as byte words in the row copy's lane order (byte j of word i is element
+// 32i + 4m + j of lane m's slice), when every component of it is an integer in [0, 255]:
+// returns false (wave-uniform) otherwise. sum_sq: the query's sum of squares (every lane).
+__device__ __forceinline__ bool query_as_bytes(const float4 (&qb)[4], std::uint32_t (&qw)[4], std::uint32_t& sum_sq) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float v[4] = {qb[i].x, qb[i].y, qb[i].z, qb[i].w};
+        std::uint32_t w = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            ok = ok && v[j] >= 0.0f && v[j] <= 255.0f && v[j] == __builtin_floorf(v[j]);
+            w |= static_cast<std::uint32_t>(__builtin_fminf(__builtin_fmaxf(v[j], 0.0f), 255.0f)) << (8 * j);
+        }
+        qw[i] = w;
+    }
+    std::uint32_t qq = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        qq = __builtin_amdgcn_udot4(qw[i], qw[i], qq, false);
+    }
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 1, 8));
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 2, 8));
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 4, 8));
+    sum_sq = qq;
+    return __ballot(!ok) == 0;
+}
+
+// distances_rows_u8_l2 for a query whose components are all integers in [0, 255]
+// (query_as_bytes): with both operands bytes, every partial sum of the reference's float32
+// order is an integer of at most 128 * 255^2 = 8,323,200 < 2^24, so that order computes the
+// exact integer sum of (q - x)^2 — which equals |q|^2 + |x|^2 - 2 q.x, computed here exactly in
+// integers (v_dot4_u32_u8: four byte products per instruction) and converted once. Every
+// distance is therefore bit-identical to the float path's, with an eighth of its arithmetic.
+template <int R>
+__device__ __forceinline__ void distances_rows_u8_l2_int(const unsigned char* rows, const std::uint32_t* ids,
+                                                         std::uint32_t cnt, const std::uint32_t (&qw)[4],
+                                                         std::uint32_t qq, float* out) {
+    const int lane = lane_id();
+    const int g = lane >> 3;
+    const int m = lane & 7;
+    std::uint32_t rid[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const std::uint32_t c = static_cast<std::uint32_t>(r * 8 + g);
+        rid[r] = ids[c < cnt ? c : 0u];
+    }
+    uint4 w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        w[r] = *reinterpret_cast<const uint4*>(rows + static_cast<std::size_t>(rid[r]) * 128 + 16 * m);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every row load in flight before the arithmetic
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (static_cast<std::uint32_t>(r * 8) < cnt) {
+            const std::uint32_t wi[4] = {w[r].x, w[r].y, w[r].z, w[r].w};
+            std::uint32_t qx = 0, xx = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                qx = __builtin_amdgcn_udot4(qw[i], wi[i], qx, false);
+                xx = __builtin_amdgcn_udot4(wi[i], wi[i], xx, false);
+            }
+            int p = static_cast<int>(xx) - 2 * static_cast<int>(qx);  // |p| < 2^25: no overflow
+            p += dpp_xor1_i(p);
+            p += dpp_xor2_i(p);
+            p += dpp_next4_i(p);  // lane m = 0: the group's 8 lanes
+            const std::uint32_t c = static_cast<std::uint32_t>(r * 8 + g);
+            if (m == 0 && c < cnt) {
+                out[c] = static_cast<float>(static_cast<int>(qq) + p);
+            }
+        }
+    }
+}
+
 }  // namespace vdx::dev

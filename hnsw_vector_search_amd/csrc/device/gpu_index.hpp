@@ -36,6 +36,8 @@ struct vdx_gpu_hnsw {
     // turn. A call holds its slot's lock while it waits for the slot's previous launch (its
     // event) and enqueues its own, so calls in flight on different streams never share one.
     static constexpr std::uint32_t kRingSlots = 64;
+    // stamps of the byte-query flag, one per launch that checks its queries (never 0)
+    std::atomic<std::uint32_t> query_epoch{0};
     vdx::gpu::DeviceBuffer ring;
     std::atomic<std::uint32_t> next_slot{0};
     std::mutex ring_mutex[kRingSlots];

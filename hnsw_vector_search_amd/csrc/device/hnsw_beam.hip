@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
     const int first_tail = static_cast<int>(a.first_tail);
     const RowSource src{a.vectors, a.ld, a.l2};
 
-    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_clamp = 0;
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0;
 
     std::uint32_t begin = 0, end = a.nq;
     if (a.range != nullptr) {
@@ -204,7 +204,9 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
         const std::uint32_t ef = uniform(a.efs ? a.efs[q] : a.ef);
         const std::uint32_t efp = min(max(ef, k), a.ef_cap);
-        n_clamp += max(ef, k) > a.ef_cap ? 1u : 0u;  // ran at the launch's R capacity
+        if (max(ef, k) > a.ef_cap && lane == 0 && a.stats != nullptr) {
+            atomicAdd(a.stats + 6, 1ull);  // runs at the launch's R capacity
+        }
         Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
         if (a.n == 0) {
             if (lane == 0) {
@@ -461,7 +463,6 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         atomicAdd(a.stats + 2, n_up);
         atomicAdd(a.stats + 3, n_dup);
         atomicAdd(a.stats + 4, n_stop);
-        atomicAdd(a.stats + 6, n_clamp);
     }
 }
 

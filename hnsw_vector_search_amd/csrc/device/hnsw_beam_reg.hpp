@@ -97,9 +97,14 @@ __device__ __forceinline__ void first_open(const std::uint64_t (&Rk)[CH], int fr
 
 // U8 (dimension 128, squared L2 only): rows come from the 8-bit copy a.rows_u8
 // (distances_rows_u8_l2). MET: 1 squared L2, 0 the inner-product form, -1 read from a.l2.
-template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE, bool U8 = false, int MET = -1>
+// QI (with U8): every query of the launch has all its components integers in [0, 255]
+// (a.qflag, a.qflag_epoch), so the rows' distances take exact integer arithmetic (distances_rows_u8_l2_int)
+// with the query held as 4 byte words.
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE, bool U8 = false, int MET = -1,
+          bool QI = false>
 __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
     static_assert(!U8 || (NB == 4 && N8 == 0 && TAIL == 0 && FIXED), "8-bit rows are dimension 128");
+    static_assert(!QI || (U8 && MODE == kBeamPlain), "integer queries: the plain search over the 8-bit rows");
     using namespace reg;
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -136,7 +141,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         end = uniform(a.range[1]);
     }
 
-    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0, n_clamp = 0;
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0;
 
     for (;;) {
         std::uint32_t t = 0;
@@ -151,9 +156,12 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         const std::uint32_t k = min(uniform(a.ks ? a.ks[q] : a.k), a.k_cap);
         const std::uint32_t ef = uniform(a.efs ? a.efs[q] : a.ef);
         const std::uint32_t efp = min(max(ef, k), static_cast<std::uint32_t>(64 * CH));
-        // a per-query ef above the launch's R capacity runs at the capacity: counted, so a
-        // caller whose `ef` bound was too small can see it
-        n_clamp += max(ef, k) > static_cast<std::uint32_t>(64 * CH) ? 1u : 0u;
+        // a per-query ef above the launch's R capacity runs at the capacity: counted at once
+        // (rare; no counter kept live through the search), so a caller whose `ef` bound was
+        // too small can see it
+        if (max(ef, k) > static_cast<std::uint32_t>(64 * CH) && lane == 0 && a.stats != nullptr) {
+            atomicAdd(a.stats + 6, 1ull);
+        }
         Neighbor* out = a.out + static_cast<std::size_t>(q) * a.out_stride;
 
         // The query, spread over the 8 lanes of each group (every group holds a full copy).
@@ -169,6 +177,12 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 qb[i] = make_float4(p[0], p[1], p[2], p[3]);
             }
         }
+        // QI: the query as byte words (the floats are dead from here)
+        std::uint32_t qw[4] = {0u, 0u, 0u, 0u};
+        std::uint32_t qq = 0;
+        if constexpr (QI) {
+            (void)query_as_bytes(qb, qw, qq);
+        }
 #pragma unroll
         for (int s = 0; s < N8; ++s) {
             q8[s] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -183,7 +197,9 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         }
         // Distances of cnt candidates (ids in LDS) to the query, into LDS.
         const auto row_distances = [&](const std::uint32_t* ids, std::uint32_t cnt, float* dst) {
-            if constexpr (U8) {
+            if constexpr (QI) {
+                distances_rows_u8_l2_int<RR>(a.rows_u8, ids, cnt, qw, qq, dst);
+            } else if constexpr (U8) {
                 distances_rows_u8_l2<RR>(a.rows_u8, ids, cnt, qb, dst);
             } else {
                 distances_rows<NB, N8, TAIL, RR, FIXED, MET>(src, ids, cnt, qb, q8, qt, nb, n8, tail, first_tail, dst);
@@ -201,6 +217,12 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 top = 0;
             }
         }
+        // cur's first upper block (loaded with its row): with each hop's neighbours' first
+        // blocks loaded together with their rows, a hop costs two dependent loads (its block,
+        // then the neighbours' rows and first blocks), not three. Not in the float search over
+        // the byte rows, where the extra register would cost a wave per SIMD.
+        constexpr bool kOffsets = QI || !U8;
+        std::uint32_t cur_off = top > a.layer ? uniform(a.upper_offset[cur]) : 0u;
         if (lane == 0) {
             fid[0] = cur;
         }
@@ -211,13 +233,18 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         ++n_dist;
         for (std::uint32_t layer = top; layer > a.layer; --layer) {
             for (;;) {
-                const std::uint32_t* blk =
-                    a.upper + (static_cast<std::size_t>(a.upper_offset[cur]) + (layer - 1)) * a.bu;
+                const std::uint32_t* blk = a.upper + (static_cast<std::size_t>(cur_off) + (layer - 1)) * a.bu;
                 const std::uint32_t cnt = uniform(blk[0]);
                 std::uint32_t id = 0;
                 if (static_cast<std::uint32_t>(lane) < cnt) {
                     id = blk[1 + lane];
                     fid[lane] = id;
+                }
+                // a neighbour on this layer has upper blocks down to layer 1 (the upload's link
+                // rules), so its first block index is what the next hop or layer needs
+                std::uint32_t nb_off = 0;
+                if constexpr (kOffsets) {
+                    nb_off = static_cast<std::uint32_t>(lane) < cnt ? a.upper_offset[id] : 0u;
                 }
                 ++n_up;
                 n_dist += cnt;
@@ -236,6 +263,11 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 const std::uint64_t at = ballot(static_cast<std::uint32_t>(lane) < cnt && d == best);
                 const int first = __ffsll(static_cast<long long>(at)) - 1;
                 cur = uniform(broadcast(id, first));
+                if constexpr (kOffsets) {
+                    cur_off = uniform(broadcast(nb_off, first));
+                } else {
+                    cur_off = uniform(a.upper_offset[cur]);
+                }
                 cur_d = best;
             }
         }
@@ -544,7 +576,6 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         atomicAdd(a.stats + 3, n_dup);
         atomicAdd(a.stats + 4, n_stop);
         atomicAdd(a.stats + 5, n_pref);
-        atomicAdd(a.stats + 6, n_clamp);
     }
 }
 
@@ -562,7 +593,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
     // the metric fixed per copy of the body (one uniform branch per launch instead of one per
     // row block): 2-3% faster over the FP32 rows (measured, L2 and cosine)
     if constexpr (U8) {
-        beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, true, 1>(a);
+        // a launch whose queries are all byte-valued (*qflag != qflag_epoch) is served by
+        // hnsw_beam_reg_u8int_kernel, enqueued with this one
+        if (a.qflag != nullptr && *a.qflag != a.qflag_epoch) {
+            return;
+        }
+        beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, true, 1, false>(a);
     } else if (a.l2 != 0) {
         beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, false, 1>(a);
     } else {
@@ -570,7 +606,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
     }
 }
 
-// Kernel pointers of one chunk count, by shape and mode.
+// The plain search over the 8-bit rows when every query of the launch has all its components
+// integers in [0, 255] (*qflag != qflag_epoch after query_bytes_check): exact integer
+// arithmetic with the query held as 4 byte words, so the same distances in a kernel of its own
+// (sharing one kernel with the float path cost both bodies registers and a wave per SIMD).
+// Held to 64 VGPRs (8 waves per SIMD) up to R capacity 256, a few spilled registers cheaper
+// than the wave they buy (measured: ef 36 / 64 0.3875 / 0.5304 -> 0.3848 / 0.5150 ms).
+// Enqueued together with hnsw_beam_reg_plain_kernel<U8>: whichever the flag does not select
+// returns at once.
+template <int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 8 : 4))) void hnsw_beam_reg_u8int_kernel(
+    BeamArgs a) {
+    if (a.qflag == nullptr || *a.qflag == a.qflag_epoch) {
+        return;
+    }
+    beam_reg_body<4, 0, 0, 4, true, CH, kBeamPlain, true, 1, true>(a);
+}
+
+// Kernel pointers of one chunk count, by shape and mode (kBeamU8Int: the integer kernel above).
 template <int CH>
 const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
 #define VDX_REG_KERNEL(MODE)                                                                     \
@@ -582,6 +635,9 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
             return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<3, 0, 4, 4, true, CH, MODE>); \
         default:                                                                                 \
             return reinterpret_cast<const void*>(&hnsw_beam_reg_kernel<8, 3, 7, 1, false, CH, MODE>); \
+    }
+    if (mode == kBeamU8Int) {
+        return reinterpret_cast<const void*>(&hnsw_beam_reg_u8int_kernel<CH>);
     }
     if (mode == kBeamTrace) {
         VDX_REG_KERNEL(kBeamTrace)

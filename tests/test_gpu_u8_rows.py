@@ -159,3 +159,47 @@ def test_bf16_lds_fed_filter_on_tiny_bases(n, metric):
         exact, _ = index.search_arrays(queries, k, "exact")
         got, _ = index.search_arrays(queries, k, "bf16")
         assert got.tobytes() == exact.tobytes(), (n, k)
+
+
+@pytest.mark.parametrize("case", ["all bytes", "one fraction", "one negative", "one above 255", "one -0.0"])
+def test_integer_arithmetic_only_when_every_query_is_byte_valued(case):
+    """A launch whose queries all have every component an integer in [0, 255] takes the exact
+    integer arithmetic over the byte rows; one query outside that sends the launch to the float
+    arithmetic. Either way every result equals the oracle's and the FP32-row search's, in one
+    launch and in an uneven batch."""
+    print(f"case: integer-vs-float byte-row kernel, {case}", flush=True)
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 8000, 300)
+    index = vdx.HnswIndex(128, vdx.Metric.L2, M=16, ef_construction=100, seed=4)
+    index.add(base, num_threads=8)
+    assert index.gpu_u8_rows()
+    q = queries.copy()
+    if case == "one fraction":
+        q[137, 5] += 0.5
+    elif case == "one negative":
+        q[0, 0] = -1.0
+    elif case == "one above 255":
+        q[299, 127] = 256.0
+    elif case == "one -0.0":
+        q[10, :] = np.float32(-0.0)
+    for k, ef in ((1, 16), (10, 64), (100, 200), (10, 500)):
+        got = assert_matches_oracle(index, q, k, ef)
+        ref, _ = index.search_arrays(q, k, ef, opts=f32_rows())
+        assert got.tobytes() == ref.tobytes()
+    # uneven batch: the classify kernel zeroes the flag, the check sets it
+    ctx = nat.Context.get(0)
+    rng = np.random.default_rng(3)
+    efs = np.array([16, 64, 200, 500], dtype=np.uint32)[rng.integers(0, 4, len(q))]
+    dq = nat.DeviceArray.from_numpy(q, ctx)
+    defs = nat.DeviceArray.from_numpy(efs, ctx)
+    out = nat.DeviceArray((len(q), 10), nat.NEIGHBOR_DTYPE, ctx)
+    cnt = nat.DeviceArray(len(q), np.uint32, ctx)
+    batch = vdx.SearchBatch(index, len(q), 10, 500)
+    for graph in (False, True, True):
+        batch.run(dq, len(q), out, cnt, k=10, efs=defs, graph=graph)
+        nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+        got = out.numpy()
+        for ef in np.unique(efs):
+            sel = np.nonzero(efs == ef)[0]
+            ref, _ = index.search_arrays(q[sel], 10, int(ef), opts=f32_rows())
+            assert got[sel].tobytes() == ref.tobytes(), (case, ef, graph)
+    del batch
