@@ -50,9 +50,13 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_TFLOPS = 157.3
 BF16_MFMA_TFLOPS = 2500.0
+FP8_MFMA_TFLOPS = 5000.0
 EF_SWEEP = (16, 32, 64, 128, 512)
 EF_FINE = (34, 36, 38, 40, 44, 48, 56)
 TARGET_RECALL = 0.95
+CALIB_MARGIN = 0.002  # about the sampling error of recall@10 over 10,000 held-out queries
+THRESHOLDS = (0.5, 0.6, 0.65, 0.7, 0.72, 0.75, 0.78, 0.8, 0.83, 0.85, 0.88, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99,
+              0.995, 0.999)
 K = 10
 CACHE = Path(os.environ.get("VDX_BENCH_CACHE", "/tmp/vdx_bench_cache"))
 
@@ -183,8 +187,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--serving-nq", type=int, default=65536)
-    ap.add_argument("--base10m", action="store_true",
-                    help="also run the 10M x 128 M=32 base (host graph build takes minutes)")
+    ap.add_argument("--no-base10m", action="store_true",
+                    help="skip the 10M x 128 M=32 base (its GPU-assisted graph build takes about half a minute)")
     ap.add_argument("--n10m", type=int, default=10_000_000)
     ap.add_argument("--no-gpu-build", action="store_true", help="skip the GPU-assisted build leg")
     ap.add_argument("--no-load", action="store_true", help="skip the FORMAT v1/v2 load leg")
@@ -212,31 +216,41 @@ def main(argv=None) -> int:
     log(f"generated {args.n} x 128 base and {args.nq} queries in {time.perf_counter() - t:.1f} s")
     index, build_s = build_or_load_graph(vdx, "gmm1m", base, vdx.Metric.L2, threads)
 
-    # Ground truth (and the brute-force leg): exact scan, FP32 vs BF16 MFMA shortlist.
+    # Ground truth (and the brute-force leg): exact scan, FP32 vs BF16 MFMA shortlist. The
+    # held-out calibration queries (same mixture, their own seed) choose the fixed ef; the
+    # evaluation queries then measure it.
     t = time.perf_counter()
     truth, bf = ground_truth(vdx, base, vdx.Metric.L2, queries)
     gt_s = time.perf_counter() - t
+    calib_queries = mix.sample(args.nq, 4)
+    calib_truth, _ = bf.search_arrays(calib_queries, K, "bf16")
     bf_leg = bruteforce_leg(args, nat, bf, queries, vdx.Metric.L2, threads)
     del bf
 
-    # Fixed-ef sweep.
+    # Fixed-ef sweep: recall on both query sets, time on the evaluation set.
     run = Runner(nat, index, queries)
+    run_calib = Runner(nat, index, calib_queries)
     sweep = []
     for ef in sorted(set(map(int, args.efs.split(",")))):
         run.launch(ef)
         res = run.results()
+        run_calib.launch(ef)
         ms, _ = run.timed(ef, args.steps, args.warmup)
         row = {"ef": ef, "recall": round(recall(res, truth), 4),
                "recall_threshold": round(threshold_recall(res, truth), 4),
+               "calib_recall": round(recall(run_calib.results(), calib_truth), 4),
                "ms": round(ms, 4), "qps": round(args.nq / ms * 1000, 1)}
         sweep.append(row)
-        log(f"fixed ef {ef}: recall {row['recall']:.4f}, {ms:.3f} ms/pass, {row['qps']:,.0f} QPS")
+        log(f"fixed ef {ef}: recall {row['recall']:.4f} (held-out {row['calib_recall']:.4f}), {ms:.3f} ms/pass, "
+            f"{row['qps']:,.0f} QPS")
+    del run_calib
     gpu_build_leg = None if args.no_gpu_build else run_gpu_build(vdx, nat, base, queries, truth, threads,
                                                                   build_s, sweep)
     load_leg = None if args.no_load else run_load(vdx, index, queries)
-    config_points = [s for s in sweep if s["ef"] in EF_SWEEP and s["recall"] >= TARGET_RECALL]
+    # the operating points: the smallest ef whose recall on the held-out queries reaches 0.95
+    config_points = [s for s in sweep if s["ef"] in EF_SWEEP and s["calib_recall"] >= TARGET_RECALL]
     fixed = min(config_points, key=lambda s: s["ef"]) if config_points else sweep[-1]
-    fine_points = [s for s in sweep if s["recall"] >= TARGET_RECALL]
+    fine_points = [s for s in sweep if s["calib_recall"] >= TARGET_RECALL]
     fixed_best = min(fine_points, key=lambda s: s["ef"]) if fine_points else fixed
 
     # Learned termination.
@@ -247,7 +261,7 @@ def main(argv=None) -> int:
         learned_leg, headline = run_learned(args, vdx, nat, datasets, learned, index, base, queries,
                                             truth, mix, threads, fixed, fixed_best, headline)
     serving_leg = None if args.no_serving else run_serving(args, vdx, nat, datasets, threads)
-    base10m_leg = run_10m(args, vdx, nat, datasets, threads) if args.base10m else None
+    base10m_leg = None if args.no_base10m else run_10m(args, vdx, nat, datasets, threads)
 
     if args.copy_probe_mb > 0:
         nbytes = args.copy_probe_mb << 20
@@ -305,6 +319,8 @@ def main(argv=None) -> int:
     host_ms = sorted(host_times)[2] * 1000
     log(f"host entry point (PCIe inclusive), ef {headline['ef']}: {host_ms:.3f} ms, "
         f"{nq / host_ms * 1000:,.0f} QPS")
+
+    single = run_single_query(args, nat, index, queries, threads)
 
     cpu = None
     identical = None
@@ -365,6 +381,8 @@ def main(argv=None) -> int:
         "base_10m": base10m_leg,
         "host_entry_point": {"ms": round(host_ms, 4), "qps": round(nq / host_ms * 1000, 1),
                              "note": "queries from and results to host memory (PCIe inclusive)"},
+        "single_query": single,
+        "float_data": float_figures(f32_leg, learned_leg),
         "gpu_cpu_identical_fraction": identical,
         "wall_ms_per_step": round(wall_ms, 4),
         "ground_truth_s": round(gt_s, 3),
@@ -372,6 +390,48 @@ def main(argv=None) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_single_query(args, nat, index, queries, threads, efs=(36, 64)):
+    """HnswIndex::search of one query per call (src/hnsw.cpp:484-488): through the host entry
+    point (validation, copies both ways, launch; mean over 300 calls), and one device-resident
+    query per launch (HIP events over 200 launches); beside it the oracle's search of one query
+    at a time on one host thread (vecdex's single-thread figure) over 2,000 queries."""
+    out = {}
+    for ef in efs:
+        index.search_arrays(queries[:1], K, ef)
+        t = time.perf_counter()
+        for i in range(300):
+            index.search_arrays(queries[i:i + 1], K, ef)
+        host_us = (time.perf_counter() - t) / 300 * 1e6
+        one = Runner(nat, index, queries[:1])
+        ms1, _ = one.timed(ef, 200, 10)
+        row = {"host_entry_us": round(host_us, 1), "device_launch_us": round(ms1 * 1000, 1)}
+        if not args.no_cpu:
+            from oracle import oracle as o
+            g = index.graph_arrays()
+            o.hnsw_search(g, queries[:200], K, ef, threads=1)
+            t = time.perf_counter()
+            o.hnsw_search(g, queries[:2000], K, ef, threads=1)
+            row["cpu_one_thread_us"] = round((time.perf_counter() - t) / 2000 * 1e6, 1)
+        out[f"ef{ef}"] = row
+        log(f"single query, ef {ef}: host entry {host_us:.0f} us, device launch {ms1 * 1000:.0f} us, "
+            f"CPU one thread {row.get('cpu_one_thread_us')} us")
+        del one
+    return out
+
+
+def float_figures(f32_leg, learned_leg):
+    """The figures that hold for any float data (no byte-row copy): the headline ef over the
+    1M base's FP32 rows, and the 1.2M x 100 cosine base (float vectors) at recall 0.95."""
+    out = {}
+    if f32_leg is not None:
+        out["fp32_rows_1M_x_128_same_ef"] = {"qps": f32_leg["qps"], "ms": f32_leg["ms"], "frac": f32_leg["frac"]}
+    if learned_leg is not None:
+        at = [r for r in learned_leg["cos_fixed_sweep"] if r["calib_recall"] >= TARGET_RECALL]
+        if at:
+            out["cosine_1_2M_x_100_at_0_95"] = at[0]
+    return out
 
 
 _COS = {}
@@ -542,55 +602,84 @@ def run_10m(args, vdx, nat, datasets, threads):
 
 
 def bruteforce_leg(args, nat, bf, queries, metric, threads):
-    """BruteForceIndex::search on the GPU over all (prepared) queries at k=10: the exact scan (the
-    reference's result), then the FP32 and BF16 MFMA shortlists with exact re-rank, each checked
-    byte for byte against the exact scan; plus the host's exact scan (the oracle's restatement of
-    the reference, on all host threads) over a bounded sample of the queries."""
+    """BruteForceIndex::search on the GPU over all (prepared) queries at k in {1, 10, 100}: the
+    exact scan (the reference's result), then the FP32, BF16 and FP8 MFMA shortlists with exact
+    re-rank, each checked byte for byte against the exact scan; the float64 pass on the GPU
+    (the largest |float32 - float64| over the returned neighbours, and the one-ulp tie re-rank's
+    agreement with the float64 oracle on a sample); plus the host's exact scan (the oracle's
+    restatement of the reference, on all host threads) over a bounded sample of the queries."""
+    from oracle import oracle as o
     base = bf.vectors()  # as stored: normalized under cosine
     nq, (n, d) = len(queries), base.shape
     ctx = nat.Context.get(0)
     dq = nat.DeviceArray.from_numpy(np.ascontiguousarray(queries, dtype=np.float32), ctx)
-    bout = nat.DeviceArray((nq, K), nat.NEIGHBOR_DTYPE, ctx)
-    bcnt = nat.DeviceArray(nq, np.uint32, ctx)
-    leg, exact = {}, None
+    leg = {}
     flops = 2.0 * nq * n * d
-    for prec in ("exact", "fp32", "bf16"):
+    sample = 200
+    for k in (1, 10, 100):
+        bout = nat.DeviceArray((nq, k), nat.NEIGHBOR_DTYPE, ctx)
+        bcnt = nat.DeviceArray(nq, np.uint32, ctx)
+        row, exact = {}, None
+
+        def call(prec_code, st):
+            nat.check(nat.lib.vdx_gpu_bf_search_device(bf.gpu_handle(), dq.ptr, nq, k, prec_code, bout.ptr, k,
+                                                       bcnt.ptr, ctypes.byref(st)))
+
+        for prec in ("exact", "fp32", "bf16", "fp8"):
+            st = nat.BfStats()
+            times = []
+            for _ in range(1 if prec in ("exact", "fp8") else 3):
+                nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+                t0 = time.perf_counter()
+                call(nat.PRECISIONS[prec], st)
+                times.append((time.perf_counter() - t0) * 1000)
+            res = bout.numpy()
+            ms = min(times)
+            if prec == "exact":
+                exact = res
+                row["exact"] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1)}
+                continue
+            peak = FP32_MFMA_TFLOPS if prec == "fp32" else (BF16_MFMA_TFLOPS if prec == "bf16" else FP8_MFMA_TFLOPS)
+            row[prec] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1),
+                         "identical_to_exact": bool(res.tobytes() == exact.tobytes()),
+                         "exact_fallbacks": int(st.exact_fallback), "shortlist": int(st.shortlist),
+                         "end_to_end_tflops": round(flops / ms / 1e9, 1),
+                         "frac_of_mfma_peak": round(flops / ms / 1e9 / peak, 4)}
+        row["bf16_speedup_vs_fp32"] = round(row["fp32"]["ms"] / row["bf16"]["ms"], 2)
+        # the float64 pass (GPU): the error of the returned float32 distances, then the re-rank
         st = nat.BfStats()
-        times = []
-        for _ in range(1 if prec == "exact" else 3):
-            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
-            t0 = time.perf_counter()
-            nat.check(nat.lib.vdx_gpu_bf_search_device(bf.gpu_handle(), dq.ptr, nq, K, nat.PRECISIONS[prec],
-                                                       bout.ptr, K, bcnt.ptr, ctypes.byref(st)))
-            times.append((time.perf_counter() - t0) * 1000)
-        res = bout.numpy()
-        ms = min(times)
-        if prec == "exact":
-            exact = res
-            leg["exact"] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1)}
-            log(f"brute force exact scan ({n:,} x {d}): {ms:.1f} ms per {nq:,} queries")
-            continue
-        peak = FP32_MFMA_TFLOPS if prec == "fp32" else BF16_MFMA_TFLOPS
-        leg[prec] = {"ms": round(ms, 2), "qps": round(nq / ms * 1000, 1),
-                     "identical_to_exact": bool(res.tobytes() == exact.tobytes()),
-                     "exact_fallbacks": int(st.exact_fallback), "shortlist": int(st.shortlist),
-                     "end_to_end_tflops": round(flops / ms / 1e9, 1),
-                     "frac_of_mfma_peak": round(flops / ms / 1e9 / peak, 4)}
-        log(f"brute force {prec}: {ms:.1f} ms per {nq:,} queries, identical to the exact scan "
-            f"{leg[prec]['identical_to_exact']}, fallbacks {st.exact_fallback}")
-    leg["bf16_speedup_vs_fp32"] = round(leg["fp32"]["ms"] / leg["bf16"]["ms"], 2)
+        call(nat.PRECISIONS["bf16"] | nat.PREC_FP64_ERROR, st)
+        same = bool(bout.numpy().tobytes() == exact.tobytes())
+        row["fp64"] = {"max_abs_err": float(st.max_err_fp64), "max_rel_err": float(st.max_rel_err_fp64),
+                       "results_unchanged": same}
+        st = nat.BfStats()
+        t0 = time.perf_counter()
+        call(nat.PRECISIONS["bf16"] | nat.PREC_FP64_TIES, st)
+        ties_ms = (time.perf_counter() - t0) * 1000
+        tied = bout.numpy()
+        s64 = 50  # the float64 oracle is a plain double loop: a small sample
+        ids64, _ = o.bruteforce_f64(base, np.ascontiguousarray(queries[:s64]), k, metric=metric, threads=threads)
+        row["fp64"].update({"ties_ms": round(ties_ms, 2), "tie_candidates": int(st.fp64_ties),
+                            "queries_reordered": int(st.fp64_reordered), "window": int(st.fp64_window),
+                            "fp64_oracle_sample": s64,
+                            "sample_ids_equal_fp64_oracle_fp32": float((exact["id"][:s64] == ids64).all(axis=1).mean()),
+                            "sample_ids_equal_fp64_oracle_reranked": float((tied["id"][:s64] == ids64).all(axis=1).mean())})
+        log(f"brute force k={k} ({n:,} x {d}): exact {row['exact']['ms']:.1f} ms, fp32 {row['fp32']['ms']:.2f}, "
+            f"bf16 {row['bf16']['ms']:.2f}, fp8 {row['fp8']['ms']:.1f} ms (fallbacks {row['fp8']['exact_fallbacks']}); "
+            f"identical {[row[p]['identical_to_exact'] for p in ('fp32', 'bf16', 'fp8')]}; max |fp32 - fp64| "
+            f"{st.max_err_fp64:.3g}")
+        leg[f"k{k}"] = row
+        del bout, bcnt
     if not args.no_cpu:
-        from oracle import oracle as o
-        sample = 200
         t0 = time.perf_counter()
         host = o.bruteforce_f32(base, np.ascontiguousarray(queries[:sample]), K, metric=metric, threads=threads)
         secs = time.perf_counter() - t0
+        gpu10 = leg["k10"]
         leg["cpu_exact"] = {"qps": round(sample / secs, 1), "cores": threads, "kind": "port",
-                            "sample": f"{sample} of the {nq:,} queries, oracle exact scan (FP32, AVX2 order)",
-                            "identical_ids": bool(np.array_equal(host["id"], exact["id"][:sample]))}
+                            "sample": f"{sample} of the {nq:,} queries at k=10, oracle exact scan (FP32, AVX2 order)"}
         log(f"host exact scan: {sample / secs:,.0f} QPS on {threads} threads (sample of {sample}); GPU/host "
-            f"speed-up of the BF16 path {leg['bf16']['qps'] / (sample / secs):,.0f}x")
-    del dq, bout, bcnt
+            f"speed-up of the BF16 path {gpu10['bf16']['qps'] / (sample / secs):,.0f}x")
+    del dq
     return leg
 
 
@@ -650,29 +739,35 @@ def run_gpu_build(vdx, nat, base, queries, truth, threads, host_build_s, sweep):
 
 def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, mix, threads,
                 fixed, fixed_best, headline):
-    """Traces, training, calibration and evaluation of the learned stop on both bases.
+    """Traces, training, calibration and evaluation of the learned stop on both bases, and the
+    oracle-stop ceiling of each.
 
-    Traces: half the training queries of each base at each of its two trace caps (L2: ef 64
-    and 128; cosine: 128 and 256), labelled by brute-force top-10 hits. One model for both.
-    Calibration (held-out queries, never the evaluation set): the recall target is 0.95 on the
-    L2 base and, on the cosine base (where 0.95 needs ef far above 512), the recall fixed ef=128
-    reaches; for each ef cap the threshold with the fewest expansions reaching target + 0.005,
-    then the cap with the highest calibration QPS. The evaluation set then measures recall and
-    QPS against the fixed-ef operating point."""
+    Recall target 0.95 on both bases. Cosine 1.2M x 100: a fixed-ef sweep up to 2048 on the
+    evaluation queries (above 512 the LDS beam kernel); the cap is the smallest ef whose
+    held-out recall reaches the target. Traces: training queries of each base at its trace caps
+    (L2: ef 64 and 128, half the queries each; cosine: the cap, on as many queries as 400 MB of
+    trace holds), labelled by brute-force top-10 hits; one model for both (20 epochs, Adam
+    lr 1e-3, batch 4096, up to --max-train-steps sampled steps). Calibration on held-out queries
+    (never the evaluation set): per ef cap and consult interval (every 4th or 8th expansion) the
+    threshold with the fewest expansions reaching target + 0.002, then the setting with the
+    highest calibration QPS.
+
+    Ceiling: the evaluation queries' traced searches at each cap, stopped by an oracle at the
+    first expansion whose top-10 equals the final top-10, and by a greedy recall-budgeted oracle
+    (the fewest expansions whose mean recall still reaches 0.95), against the expansions fixed ef
+    needs for 0.95: the most any stopping rule inside these searches could save."""
     t_all = time.perf_counter()
     cos_gen, cos_base, cos_index = cosine_setup(args, vdx, datasets, threads)
     ntr = args.train_queries
     sets = {
         "l2": {"index": index, "base": base, "metric": vdx.Metric.L2, "trace_efs": (64, 128),
-               "caps": (48, 64, 96, 128),
+               "caps": (40, 48, 64, 96, 128),
                "train": mix.sample(ntr, 3), "calib": mix.sample(args.nq, 4), "eval": queries},
         "cos": {"index": cos_index, "base": cos_base, "metric": vdx.Metric.Cosine,
-                "trace_efs": (128, 256), "caps": (128, 192, 256),
                 "train": prepared(nat, cos_gen.sample(ntr, 3), True),
                 "calib": prepared(nat, cos_gen.sample(args.nq, 4), True),
                 "eval": prepared(nat, cos_gen.sample(args.nq, 2), True)},
     }
-    feats, labels = [], []
     for name, s in sets.items():
         allq = np.concatenate([s["train"], s["calib"], s["eval"]])
         t = time.perf_counter()
@@ -681,14 +776,41 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
         s["truth_calib"] = tr[ntr:ntr + args.nq]
         s["truth_eval"] = truth if name == "l2" else tr[ntr + args.nq:]
         log(f"learned/{name}: ground truth for {len(allq)} queries in {time.perf_counter() - t:.1f} s")
-        half = ntr // 2
-        for part, ef in enumerate(s["trace_efs"]):
-            lo, hi = part * half, (part + 1) * half
+
+    # the cosine base's fixed-ef sweep; its cap for recall 0.95 from the held-out queries
+    s = sets["cos"]
+    rc, re_ = Runner(nat, s["index"], s["calib"]), Runner(nat, s["index"], s["eval"])
+    sweep_cos, fixed_cos = [], None
+    for ef in (64, 128, 256, 320, 384, 448, 512, 640, 768, 1024, 1536, 2048):
+        re_.launch(ef)
+        r_e = recall(re_.results(), s["truth_eval"])
+        rc.launch(ef)
+        r_c = recall(rc.results(), s["truth_calib"])
+        ms, _ = re_.timed(ef, max(3, args.steps // 2), 2)
+        sweep_cos.append({"ef": ef, "recall": round(r_e, 4), "calib_recall": round(r_c, 4), "ms": round(ms, 4),
+                          "qps": round(len(s["eval"]) / ms * 1000, 1)})
+        log(f"cos fixed ef {ef}: recall {r_e:.4f} (held-out {r_c:.4f}), {sweep_cos[-1]['qps']:,.0f} QPS")
+        if fixed_cos is None and r_c >= TARGET_RECALL:
+            fixed_cos = sweep_cos[-1]
+    del rc, re_
+    cap_cos = fixed_cos["ef"] if fixed_cos else 2048
+    s["trace_efs"] = (cap_cos,)
+    s["caps"] = tuple(sorted({cap_cos, int(cap_cos * 1.5), 2 * cap_cos} & set(range(1, 8193))))
+
+    # traces and training
+    feats, labels = [], []
+    for name, s in sets.items():
+        efs = s["trace_efs"]
+        per = ntr // len(efs)
+        for part, ef in enumerate(efs):
+            max_steps = 2 * ef
+            nq_tr = min(per, max(1000, (400 << 20) // (max_steps * 16 * 4)))
+            lo = part * per
             t = time.perf_counter()
-            f, lab, steps, final = learned.collect_traces(s["index"], s["train"][lo:hi],
-                                                          s["truth_train"]["id"][lo:hi], k=K, ef=ef,
-                                                          max_steps=2 * ef)
-            log(f"learned/{name}: traces at ef {ef}: {len(f):,} steps ({steps.mean():.1f} "
+            f, lab, steps, final = learned.collect_traces(s["index"], s["train"][lo:lo + nq_tr],
+                                                          s["truth_train"]["id"][lo:lo + nq_tr], k=K, ef=ef,
+                                                          max_steps=max_steps)
+            log(f"learned/{name}: traces at ef {ef} over {nq_tr} queries: {len(f):,} steps ({steps.mean():.1f} "
                 f"expansions/query, label mean {lab.mean():.3f}) in {time.perf_counter() - t:.1f} s")
             feats.append(f)
             labels.append(lab)
@@ -697,69 +819,87 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
     if len(f) > args.max_train_steps:
         pick = np.random.default_rng(0).choice(len(f), args.max_train_steps, replace=False)
         f, lab = f[pick], lab[pick]
+    n_train = len(f)
     t = time.perf_counter()
     model, losses = learned.train_stop_model(f, lab, epochs=20, batch=4096, lr=1e-3, seed=0)
     train_s = time.perf_counter() - t
     log(f"learned: trained on {len(f):,} steps, 20 epochs in {train_s:.1f} s; loss "
         f"{losses[0]:.4f} -> {losses[-1]:.4f}")
+    del feats, labels, f, lab
 
     results = {}
+    fixed_eval = {"l2": {"ef": fixed_best["ef"], "recall": fixed_best["recall"], "qps": fixed_best["qps"]},
+                  "cos": fixed_cos}
     for name, s in sets.items():
+        target = TARGET_RECALL
         rc = Runner(nat, s["index"], s["calib"])
         re_ = Runner(nat, s["index"], s["eval"])
-        if name == "l2":
-            target = TARGET_RECALL
-            fixed_eval = {"ef": fixed_best["ef"], "recall": fixed_best["recall"], "qps": fixed_best["qps"]}
-        else:
-            rc.launch(128)
-            target = recall(rc.results(), s["truth_calib"])
-            fixed_eval = None
-            sweep_cos = []
-            for ef in (64, 96, 128, 192, 256):
-                re_.launch(ef)
-                r_e = recall(re_.results(), s["truth_eval"])
-                ms, _ = re_.timed(ef, args.steps, args.warmup)
-                sweep_cos.append({"ef": ef, "recall": round(r_e, 4), "qps": round(len(s["eval"]) / ms * 1000, 1)})
-                log(f"cos fixed ef {ef}: recall {r_e:.4f}, {sweep_cos[-1]['qps']:,.0f} QPS")
-                if fixed_eval is None and r_e >= target:
-                    fixed_eval = sweep_cos[-1]
-            results["cos_fixed_sweep"] = sweep_cos
+        # the ceiling, from the evaluation queries' traced searches
+        ceiling = []
+        for cap in ((36, 48, 64, 128) if name == "l2" else (cap_cos,)):
+            hits, steps, final = learned.trace_hits(s["index"], s["eval"], s["truth_eval"]["id"], K, cap,
+                                                    max_steps=2 * cap)
+            c = learned.stop_ceiling(hits, steps, final, target, K)
+            c["cap"] = cap
+            ceiling.append(c)
+            log(f"learned/{name} ceiling at cap {cap}: fixed {c['fixed_expansions']} expansions (recall "
+                f"{c['fixed_recall']}), oracle stop {c['oracle_stop_expansions']}, budgeted to {target}: "
+                f"{c.get('budgeted_expansions')}")
         best = None
         for ef_cap in s["caps"]:
-            thr, rec_c, work, table = learned.calibrate(s["index"], model, s["calib"],
-                                                        s["truth_calib"]["id"], K, ef_cap, target + 0.005)
-            if rec_c < target + 0.005:
-                log(f"learned/{name} ef cap {ef_cap}: no threshold reaches {target + 0.005:.4f} "
-                    f"(best {rec_c:.4f})")
-                continue
-            opts = learned.learned_opts(model, thr, 1)
-            ms_c, _ = rc.timed(ef_cap, args.steps, args.warmup, opts)
-            log(f"learned/{name} ef cap {ef_cap}: threshold {thr}, calib recall {rec_c:.4f}, "
-                f"{work:.1f} expansions, {len(s['calib']) / ms_c * 1000:,.0f} QPS on calibration")
-            if best is None or ms_c < best["_ms_c"]:
-                best = {"ef_cap": ef_cap, "threshold": thr, "calib_recall": round(rec_c, 4),
-                        "_ms_c": ms_c, "_opts": opts}
+            for every in (4, 8):  # consult the model every 4th or 8th expansion
+                thr, rec_c, work, table = learned.calibrate(s["index"], model, s["calib"], s["truth_calib"]["id"],
+                                                            K, ef_cap, target + CALIB_MARGIN, every=every,
+                                                            grid=THRESHOLDS)
+                if rec_c < target + CALIB_MARGIN:
+                    log(f"learned/{name} ef cap {ef_cap}, every {every}: no threshold reaches "
+                        f"{target + CALIB_MARGIN:.4f} (best {rec_c:.4f})")
+                    continue
+                opts = learned.learned_opts(model, thr, 1, every=every)
+                ms_c, _ = rc.timed(ef_cap, max(3, args.steps // 2), 2, opts)
+                log(f"learned/{name} ef cap {ef_cap}, every {every}: threshold {thr}, calib recall {rec_c:.4f}, "
+                    f"{work:.1f} expansions, {len(s['calib']) / ms_c * 1000:,.0f} QPS on calibration")
+                if best is None or ms_c < best["_ms_c"]:
+                    best = {"ef_cap": ef_cap, "every": every, "threshold": thr, "calib_recall": round(rec_c, 4),
+                            "_ms_c": ms_c, "_opts": opts}
+        fe = fixed_eval[name]
+        if fe is not None:
+            # the fixed search's expansions at its operating point, on the evaluation queries
+            fe = dict(fe)
+            fe["expansions"] = round(re_.counters(fe["ef"])[0] / len(s["eval"]), 1)
         if best is not None:
             opts = best["_opts"]
             re_.launch(best["ef_cap"], opts)
             rec_e = recall(re_.results(), s["truth_eval"])
             ms, _ = re_.timed(best["ef_cap"], args.steps, args.warmup, opts)
             exp = re_.counters(best["ef_cap"], opts)[0] / len(s["eval"])
-            best.update({"target": round(target, 4), "eval_recall": round(rec_e, 4), "ms": round(ms, 4),
+            best.update({"target": target, "eval_recall": round(rec_e, 4), "ms": round(ms, 4),
                          "qps": round(len(s["eval"]) / ms * 1000, 1), "expansions": round(exp, 1),
-                         "fixed_ef_at_target": fixed_eval})
-            if fixed_eval is not None:
-                best["speedup_vs_fixed_ef"] = round(best["qps"] / fixed_eval["qps"], 3)
-            log(f"learned/{name}: eval recall {rec_e:.4f} (target {target:.4f}), {ms:.3f} ms, "
-                f"{best['qps']:,.0f} QPS, {exp:.1f} expansions; fixed ef at target: {fixed_eval}")
+                         "fixed_ef_at_target": fe})
+            if fe is not None:
+                best["speedup_vs_fixed_ef"] = round(best["qps"] / fe["qps"], 3)
+            log(f"learned/{name}: eval recall {rec_e:.4f} (target {target}), {ms:.3f} ms, "
+                f"{best['qps']:,.0f} QPS, {exp:.1f} expansions; fixed ef at target: {fe}")
+        if fe is not None:
+            floor = min((c["budgeted_expansions"] for c in ceiling if c.get("budgeted_expansions")), default=None)
+            if floor:
+                for c in ceiling:
+                    c["fixed_expansions_at_target"] = fe["expansions"]
+                results[name + "_ceiling_speedup"] = round(fe["expansions"] / floor, 3)
         results[name] = best
+        results[name + "_ceiling"] = ceiling
+        del rc, re_
     l2 = results["l2"]
     leg = {
-        "train_steps": len(f), "train_s": round(train_s, 1),
+        "train_steps": n_train,
+        "train_s": round(train_s, 1),
         "loss_first_last": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
         "l2": {k: v for k, v in (l2 or {}).items() if not k.startswith("_")},
         "cos": {k: v for k, v in (results["cos"] or {}).items() if not k.startswith("_")},
-        "cos_fixed_sweep": results["cos_fixed_sweep"],
+        "oracle_stop_ceiling": {"l2": results["l2_ceiling"], "cos": results["cos_ceiling"],
+                                "l2_max_speedup_in_expansions": results.get("l2_ceiling_speedup"),
+                                "cos_max_speedup_in_expansions": results.get("cos_ceiling_speedup")},
+        "cos_fixed_sweep": sweep_cos,
         "total_s": round(time.perf_counter() - t_all, 1),
     }
     if l2 is not None and l2["eval_recall"] >= TARGET_RECALL and l2["qps"] > headline["qps"]:

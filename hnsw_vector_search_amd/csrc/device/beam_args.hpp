@@ -71,12 +71,14 @@ struct BeamArgs {
     const void* mlp;
     float stop_logit;
     std::uint32_t min_steps;
+    std::uint32_t mlp_every;  // consult the model every this many expansions (>= 1)
 };
 
 constexpr int kBeamStats = 7;
 
 enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2,
-                      kBeamU8Int = 3 /* kernel selector only: the plain search's integer kernel */ };
+                      // kernel selectors only: the plain and learned searches' integer kernels
+                      kBeamU8Int = 3, kBeamU8IntLearned = 4 };
 
 // D128U8: dimension 128 under squared L2 with the rows read from the 8-bit copy
 // (BeamArgs::rows_u8); only the register-resident plain search has that variant, the other

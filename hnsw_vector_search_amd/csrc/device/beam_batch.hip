@@ -276,7 +276,8 @@ void enqueue_classify_into(const vdx_gpu_hnsw& h, int mode, const vdx::gpu::Beam
     a.order = buf.order;
     a.ranges = buf.ranges;
     a.queues = buf.queues;
-    const bool bytes = h.u8 && h.metric == 0 && h.dim == 128 && mode == vdx::dev::kBeamPlain && !cfg.f32_rows;
+    const bool bytes = h.u8 && h.metric == 0 && h.dim == 128 &&
+                       (mode == vdx::dev::kBeamPlain || mode == vdx::dev::kBeamLearned) && !cfg.f32_rows;
     a.qflag = bytes ? buf.qflag : nullptr;
     hipLaunchKernelGGL(vdx::dev::beam_classify_kernel, dim3(1), dim3(1024), 0, s, a);
     check(hipGetLastError(), "beam_classify launch");
@@ -469,7 +470,7 @@ int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* b, const float* d_queries, uint64_t n
                 } else if (b->caps[cls] <= 128) {
                     vl2s = {10, 11, 12};
                 } else {
-                    vl2s = {11, 12, 13};
+                    vl2s = {9, 10, 11};
                 }
                 wavess = fixed.waves ? std::vector<std::uint32_t>{fixed.waves} : std::vector<std::uint32_t>{2, 4};
                 float best_ms = -1.f;

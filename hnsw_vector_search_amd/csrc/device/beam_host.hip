@@ -13,13 +13,14 @@ namespace vdx::gpu {
 
 namespace {
 
-// 2^11 slots up to R capacity 512: a larger cache costs more in LDS occupancy (at capacity 512 a
-// 2^12 cache leaves one 4-wave workgroup per CU) than its fewer recomputed distances save. Over
-// the 8-bit rows, where a recomputed distance costs a quarter of the bytes, 2^10 (five
-// workgroups per CU instead of four) is faster (measured on the 1M x 128 base at ef 36 and 64).
+// A larger visited cache saves recomputed distances but costs LDS, hence waves per CU. Measured
+// (1M x 128 L2, 1.2M x 100 cosine; results identical throughout): 2^11 slots up to R capacity
+// 256 over the FP32 rows; 2^10 over the 8-bit rows (a recomputed row costs a quarter of the
+// bytes) and from capacity 512 up, where R itself takes LDS: cosine ef 512 / 640 / 1024 ran
+// 10.8 / 40.9 / 91.1 ms per 10k queries with 2^11 / 2^13 / 2^13 slots and 10.0 / 18.0 / 30.5 ms
+// with 2^10 (2^13 slots had left one wave per SIMD above capacity 512).
 std::uint32_t default_visited_log2(std::uint32_t ef_cap, bool u8_rows) {
-    if (ef_cap <= 512) return u8_rows ? 10 : 11;
-    return 13;
+    return (u8_rows || ef_cap >= 512) ? 10 : 11;
 }
 
 }  // namespace
@@ -72,8 +73,8 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
         p.ef_cap = need * 64u;
     }
     p.shape = beam_shape(h.dim);
-    if (p.shape == vdx::dev::BeamShape::D128 && h.u8 && h.metric == 0 && p.chunks > 0 && mode == vdx::dev::kBeamPlain &&
-        !cfg.f32_rows) {
+    if (p.shape == vdx::dev::BeamShape::D128 && h.u8 && h.metric == 0 && p.chunks > 0 &&
+        (mode == vdx::dev::kBeamPlain || mode == vdx::dev::kBeamLearned) && !cfg.f32_rows) {
         p.shape = vdx::dev::BeamShape::D128U8;
     }
     p.visited_log2 = cfg.visited_log2 ? cfg.visited_log2
@@ -106,7 +107,10 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     if (p.shape == vdx::dev::BeamShape::D128U8) {
         int per_cu_int = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu_int, vdx::dev::hnsw_beam_reg_kernel_ptr(p.shape, vdx::dev::kBeamU8Int, p.chunks),
+                  &per_cu_int,
+                  vdx::dev::hnsw_beam_reg_kernel_ptr(
+                      p.shape, mode == vdx::dev::kBeamLearned ? vdx::dev::kBeamU8IntLearned : vdx::dev::kBeamU8Int,
+                      p.chunks),
                   p.waves * 64, p.lds_bytes),
               "hipOccupancyMaxActiveBlocksPerMultiprocessor");
         per_cu_int = std::max(per_cu_int, 1);
@@ -166,6 +170,7 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
         const float t = opts->mlp_threshold;
         a.stop_logit = std::log(t / (1.0f - t));
         a.min_steps = opts->min_steps;
+        a.mlp_every = std::max<std::uint32_t>(1, opts->mlp_every);
     }
     return a;
 }
@@ -173,10 +178,11 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
 void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vdx::dev::BeamArgs& a,
                  hipStream_t stream) {
     if (plan.chunks > 0) {
-        if (plan.shape == vdx::dev::BeamShape::D128U8 && mode == vdx::dev::kBeamPlain && a.qflag != nullptr) {
+        if (plan.shape == vdx::dev::BeamShape::D128U8 && a.qflag != nullptr) {
             // the integer kernel and the float kernel, in turn: the flag lets exactly one work
-            check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid_int, plan.waves, plan.lds_bytes, stream,
-                                                 plan.shape, vdx::dev::kBeamU8Int, plan.chunks),
+            check(vdx::dev::launch_hnsw_beam_reg(
+                      a, plan.grid_int, plan.waves, plan.lds_bytes, stream, plan.shape,
+                      mode == vdx::dev::kBeamLearned ? vdx::dev::kBeamU8IntLearned : vdx::dev::kBeamU8Int, plan.chunks),
                   "hnsw_beam_reg (integer) launch");
         }
         check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,

@@ -305,8 +305,10 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         std::uint32_t size = 1;
         std::uint32_t hint = 0;
         // search-state features (trace and learned modes)
-        const float d_scale = fmaxf(fabsf(cur_d), 1e-6f);
-        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0;
+        const float inv_scale = 1.0f / fmaxf(fabsf(cur_d), 1e-6f);
+        const float inv_efp = 1.0f / static_cast<float>(efp);
+        const float inv_m2 = 1.0f / static_cast<float>(a.M2);
+        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = 0xFFFFFFFFu;
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
@@ -377,35 +379,20 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
                 const bool improved = inserted > 0 && first_pos < kk;
                 improved_hist = ((improved_hist << 1) | (improved ? 1u : 0u)) & 0xFFu;
                 since_improve = improved ? 0u : since_improve + 1u;
-                // Lane i computes feature i (all inputs are wave-uniform).
-                if (lane < kFeatures) {
-                    const float d_best = key_distance(Rl[0]);
-                    const float d_k = key_distance(Rl[kk - 1]);
-                    const float d_worst = key_distance(Rl[size - 1]);
-                    const float d_cur = key_distance(kp);
-                    const float sk = fmaxf(fabsf(d_k), 1e-6f);
-                    float v;
-                    switch (lane) {
-                        case 0: v = d_best / d_scale; break;
-                        case 1: v = d_k / d_scale; break;
-                        case 2: v = d_cur / sk; break;
-                        case 3: v = d_best / sk; break;
-                        case 4: v = static_cast<float>(steps) / efp; break;
-                        case 5: v = __log2f(1.0f + steps) * 0.125f; break;
-                        case 6: v = static_cast<float>(size) / efp; break;
-                        case 7: v = __popc(improved_hist) * 0.125f; break;
-                        case 8: v = static_cast<float>(p) / efp; break;
-                        case 9: v = static_cast<float>(inserted) / a.M2; break;
-                        case 10: v = static_cast<float>(nf) / a.M2; break;
-                        case 11: v = min(since_improve, 64u) * 0.0625f; break;
-                        case 12: v = efp * (1.0f / 512.0f); break;
-                        case 13: v = k * 0.01f; break;
-                        case 14: v = d_worst / sk; break;
-                        default: v = a.l2 ? 1.0f : 0.0f; break;
+                // the learned stop consults the model every mlp_every expansions once allowed
+                bool consult = true;
+                if constexpr (MODE == kBeamLearned) {
+                    consult = false;
+                    if (steps >= a.min_steps && size >= k && --countdown == 0) {
+                        consult = true;
+                        countdown = a.mlp_every;
                     }
-                    feat[lane] = v;
                 }
-                wave_lds_fence();
+                if (consult) {
+                const float feature = step_feature(lane, key_distance(Rl[0]), key_distance(Rl[kk - 1]),
+                                                   key_distance(Rl[size - 1]), key_distance(kp), inv_scale, steps,
+                                                   size, improved_hist, static_cast<std::uint32_t>(p), inserted, nf,
+                                                   since_improve, inv_efp, efp, inv_m2, k, a.l2 != 0);
                 if constexpr (MODE == kBeamTrace) {
                     // top-10 hits of the current result against the ground truth
                     const std::uint32_t kt = min(10u, size);
@@ -418,7 +405,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
                     if (steps <= a.trace_cap) {
                         const std::size_t row = static_cast<std::size_t>(q) * a.trace_cap + (steps - 1);
                         if (lane < kFeatures) {
-                            a.trace_feat[row * kFeatures + lane] = feat[lane];
+                            a.trace_feat[row * kFeatures + lane] = feature;
                         }
                         if (lane == 0) {
                             a.trace_hits[row] = static_cast<std::uint8_t>(hits);
@@ -429,16 +416,14 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
                     }
                 }
                 if constexpr (MODE == kBeamLearned) {
-                    if (steps >= a.min_steps && size >= k) {
-                        float h1, h2;
-                        const float z = mlp_forward_raw(*model, feat, lane, h1, h2,
-                                                        reinterpret_cast<std::uint16_t*>(fd));
-                        if (z > a.stop_logit) {
-                            ++n_stop;
-                            break;  // stop: the model judges further expansions useless
-                        }
+                    const float z = mlp_logit_lanes(*model, feature, lane, reinterpret_cast<std::uint16_t*>(feat),
+                                                    reinterpret_cast<std::uint16_t*>(fd));
+                    if (z > a.stop_logit) {
+                        ++n_stop;
+                        break;  // stop: the model judges further expansions useless
                     }
                 }
+                }  // consult
             }
         }
         if constexpr (MODE == kBeamTrace) {

@@ -104,7 +104,7 @@ template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE, bool U
           bool QI = false>
 __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
     static_assert(!U8 || (NB == 4 && N8 == 0 && TAIL == 0 && FIXED), "8-bit rows are dimension 128");
-    static_assert(!QI || (U8 && MODE == kBeamPlain), "integer queries: the plain search over the 8-bit rows");
+    static_assert(!QI || (U8 && MODE != kBeamTrace), "integer queries: the search over the 8-bit rows");
     using namespace reg;
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
@@ -303,8 +303,10 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         std::uint32_t pref_node = kNone;
         std::uint32_t pref_cnt = 0, pref_id = 0;
         // search-state features (trace and learned modes)
-        const float d_scale = fmaxf(fabsf(cur_d), 1e-6f);
-        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0;
+        const float inv_scale = 1.0f / fmaxf(fabsf(cur_d), 1e-6f);
+        const float inv_efp = 1.0f / static_cast<float>(efp);
+        const float inv_m2 = 1.0f / static_cast<float>(a.M2);
+        std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = kNone;
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
@@ -485,35 +487,21 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 const bool improved = inserted > 0 && fmin_pos < kk;
                 improved_hist = ((improved_hist << 1) | (improved ? 1u : 0u)) & 0xFFu;
                 since_improve = improved ? 0u : since_improve + 1u;
-                const float d_best = key_distance(r_at<CH>(Rk, 0));
-                const float d_k = key_distance(r_at<CH>(Rk, kk - 1));
-                const float d_worst = key_distance(r_at<CH>(Rk, size - 1));
-                const float d_cur = key_distance(kp);
-                // Lane i computes feature i (all inputs are wave-uniform).
-                if (lane < kFeatures) {
-                    const float sk = fmaxf(fabsf(d_k), 1e-6f);
-                    float v;
-                    switch (lane) {
-                        case 0: v = d_best / d_scale; break;
-                        case 1: v = d_k / d_scale; break;
-                        case 2: v = d_cur / sk; break;
-                        case 3: v = d_best / sk; break;
-                        case 4: v = static_cast<float>(steps) / efp; break;
-                        case 5: v = __log2f(1.0f + steps) * 0.125f; break;
-                        case 6: v = static_cast<float>(size) / efp; break;
-                        case 7: v = __popc(improved_hist) * 0.125f; break;
-                        case 8: v = static_cast<float>(p) / efp; break;
-                        case 9: v = static_cast<float>(inserted) / a.M2; break;
-                        case 10: v = static_cast<float>(nf) / a.M2; break;
-                        case 11: v = min(since_improve, 64u) * 0.0625f; break;
-                        case 12: v = efp * (1.0f / 512.0f); break;
-                        case 13: v = k * 0.01f; break;
-                        case 14: v = d_worst / sk; break;
-                        default: v = a.l2 ? 1.0f : 0.0f; break;
+                // the learned stop consults the model every mlp_every expansions once allowed
+                // (min_steps taken, R holding k); the features are computed only then
+                bool consult = true;
+                if constexpr (MODE == kBeamLearned) {
+                    consult = false;
+                    if (steps >= a.min_steps && size >= k && --countdown == 0) {
+                        consult = true;
+                        countdown = a.mlp_every;
                     }
-                    feat[lane] = v;
                 }
-                wave_lds_fence();
+                if (consult) {
+                const float feature =
+                    step_feature(lane, key_distance(r_at<CH>(Rk, 0)), key_distance(r_at<CH>(Rk, kk - 1)),
+                                 key_distance(r_at<CH>(Rk, size - 1)), key_distance(kp), inv_scale, steps, size,
+                                 improved_hist, p, inserted, nf, since_improve, inv_efp, efp, inv_m2, k, a.l2 != 0);
                 if constexpr (MODE == kBeamTrace) {
                     // top-10 hits of the current result against the ground truth
                     const std::uint32_t kt = min(10u, size);
@@ -527,7 +515,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                     if (steps <= a.trace_cap) {
                         const std::size_t row = static_cast<std::size_t>(q) * a.trace_cap + (steps - 1);
                         if (lane < kFeatures) {
-                            a.trace_feat[row * kFeatures + lane] = feat[lane];
+                            a.trace_feat[row * kFeatures + lane] = feature;
                         }
                         if (lane == 0) {
                             a.trace_hits[row] = static_cast<std::uint8_t>(hits);
@@ -538,16 +526,14 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                     }
                 }
                 if constexpr (MODE == kBeamLearned) {
-                    if (steps >= a.min_steps && size >= k) {
-                        float h1, h2;
-                        const float z = mlp_forward_raw(*model, feat, lane, h1, h2,
-                                                        reinterpret_cast<std::uint16_t*>(fd));
-                        if (z > a.stop_logit) {
-                            ++n_stop;
-                            break;  // stop: the model judges further expansions useless
-                        }
+                    const float z = mlp_logit_lanes(*model, feature, lane, reinterpret_cast<std::uint16_t*>(feat),
+                                                    reinterpret_cast<std::uint16_t*>(fd));
+                    if (z > a.stop_logit) {
+                        ++n_stop;
+                        break;  // stop: the model judges further expansions useless
                     }
                 }
+                }  // consult
             }
         }
         if constexpr (MODE == kBeamTrace) {
@@ -580,8 +566,9 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
 }
 
 // The kernels. The plain search is held to 128 VGPRs (4 waves per SIMD): the occupancy it gains
-// outweighs the few spilled registers at R capacities above 64 (measured), while the trace and
-// learned modes keep their registers.
+// outweighs the few spilled registers at R capacities above 64 (measured). The trace and learned
+// modes over the FP32 rows keep their registers (the stop model's LDS bounds their occupancy
+// near there anyway).
 template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, int MODE>
 __global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
     beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, MODE>(a);
@@ -623,7 +610,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 8
     beam_reg_body<4, 0, 0, 4, true, CH, kBeamPlain, true, 1, true>(a);
 }
 
-// Kernel pointers of one chunk count, by shape and mode (kBeamU8Int: the integer kernel above).
+// The learned stop over the 8-bit rows: integer arithmetic when every query is byte-valued, the
+// float arithmetic otherwise, enqueued together like the plain pair. The model staged in LDS per
+// workgroup (11 KB) leaves room for 5 four-wave workgroups per CU, so the kernels are held to the
+// registers of 5 waves per SIMD up to R capacity 256.
+template <int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 5 : 4))) void
+hnsw_beam_reg_u8int_learned_kernel(BeamArgs a) {
+    if (a.qflag == nullptr || *a.qflag == a.qflag_epoch) {
+        return;
+    }
+    beam_reg_body<4, 0, 0, 4, true, CH, kBeamLearned, true, 1, true>(a);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 5 : 4))) void
+hnsw_beam_reg_u8_learned_kernel(BeamArgs a) {
+    if (a.qflag != nullptr && *a.qflag != a.qflag_epoch) {
+        return;
+    }
+    beam_reg_body<4, 0, 0, 4, true, CH, kBeamLearned, true, 1, false>(a);
+}
+
+// Kernel pointers of one chunk count, by shape and mode (kBeamU8Int, kBeamU8IntLearned: the
+// integer kernels above).
 template <int CH>
 const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
 #define VDX_REG_KERNEL(MODE)                                                                     \
@@ -638,6 +648,12 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
     }
     if (mode == kBeamU8Int) {
         return reinterpret_cast<const void*>(&hnsw_beam_reg_u8int_kernel<CH>);
+    }
+    if (mode == kBeamU8IntLearned) {
+        return reinterpret_cast<const void*>(&hnsw_beam_reg_u8int_learned_kernel<CH>);
+    }
+    if (mode == kBeamLearned && shape == BeamShape::D128U8) {
+        return reinterpret_cast<const void*>(&hnsw_beam_reg_u8_learned_kernel<CH>);
     }
     if (mode == kBeamTrace) {
         VDX_REG_KERNEL(kBeamTrace)

@@ -133,4 +133,62 @@ __device__ __forceinline__ float mlp_forward_raw(const MlpWeights& w, const floa
         lane, h1, h2, hbuf);
 }
 
+// Lane i's search-state feature i after an expansion (lanes 16 and up: 0). Every argument is
+// wave-uniform, so every lane computes all 16 values and keeps its own (no branch per lane); the
+// quotients multiply by reciprocals of the per-query scales (inv_scale: of the entry distance;
+// inv_efp: of ef'; inv_m2: of 2M) and of the k-th distance. The trace mode records exactly these
+// values, so the model is trained on the arithmetic it later sees.
+//   0 best / entry   1 k-th / entry   2 expanded / k-th   3 best / k-th   4 steps / ef'
+//   5 log2(1 + steps) / 8   6 |R| / ef'   7 improvements among the last 8 expansions / 8
+//   8 expanded position in R / ef'   9 inserted / 2M   10 fresh neighbours / 2M
+//   11 min(expansions since the top-k improved, 64) / 16   12 ef' / 512   13 k / 100
+//   14 worst / k-th   15 1 for squared L2
+__device__ __forceinline__ float step_feature(int lane, float d_best, float d_k, float d_worst, float d_cur,
+                                              float inv_scale, std::uint32_t steps, std::uint32_t size,
+                                              std::uint32_t hist, std::uint32_t pos, std::uint32_t inserted,
+                                              std::uint32_t nf, std::uint32_t since, float inv_efp,
+                                              std::uint32_t efp, float inv_m2, std::uint32_t k, bool l2) {
+    const float inv_sk = __builtin_amdgcn_rcpf(fmaxf(fabsf(d_k), 1e-6f));
+    const float f[kFeatures] = {d_best * inv_scale,
+                                d_k * inv_scale,
+                                d_cur * inv_sk,
+                                d_best * inv_sk,
+                                static_cast<float>(steps) * inv_efp,
+                                __log2f(1.0f + static_cast<float>(steps)) * 0.125f,
+                                static_cast<float>(size) * inv_efp,
+                                static_cast<float>(__popc(hist)) * 0.125f,
+                                static_cast<float>(pos) * inv_efp,
+                                static_cast<float>(inserted) * inv_m2,
+                                static_cast<float>(nf) * inv_m2,
+                                static_cast<float>(min(since, 64u)) * 0.0625f,
+                                static_cast<float>(efp) * (1.0f / 512.0f),
+                                static_cast<float>(k) * 0.01f,
+                                d_worst * inv_sk,
+                                l2 ? 1.0f : 0.0f};
+    float v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) {
+        v = lane == i ? f[i] : v;
+    }
+    return v;
+}
+
+// The stop model's logit from the feature each lane holds (step_feature): lane i standardises
+// and rounds its own input (mlp_input), the 16 BF16 inputs meet in `xbuf` (32 bytes of the
+// wave's LDS, 16-byte aligned), and mlp_forward_core runs on them — the values
+// mlp_forward_raw computes from the raw features, with one LDS round trip instead of 16 reads.
+__device__ __forceinline__ float mlp_logit_lanes(const MlpWeights& w, float feature, int lane, std::uint16_t* xbuf,
+                                                 std::uint16_t* hbuf) {
+    if (lane < kFeatures) {
+        xbuf[lane] = float_to_bf16_bits(mlp_input(w, lane, feature));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint4 x0 = reinterpret_cast<const uint4*>(xbuf)[0];
+    const uint4 x1 = reinterpret_cast<const uint4*>(xbuf)[1];
+    const std::uint32_t words[kFeatures / 2] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    float h1, h2;
+    return mlp_forward_core(w, [&](int p) { return words[p]; }, lane, h1, h2, hbuf);
+}
+
 }  // namespace vdx::dev

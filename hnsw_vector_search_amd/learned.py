@@ -110,6 +110,69 @@ def collect_traces(index, queries: np.ndarray, truth_ids: np.ndarray, k: int = 1
             np.concatenate(all_final))
 
 
+def trace_hits(index, queries: np.ndarray, truth_ids: np.ndarray, k: int = 10, ef: int = 128,
+               max_steps: int = 256, chunk: int = 16384):
+    """The traced fixed-ef search's per-step top-10 hits: (hits uint8 [nq, max_steps], steps [nq],
+    final_hits [nq]); hits[q, s] is the hits after expansion s + 1 (valid for s < steps[q])."""
+    ctx = nat.Context.get(0)
+    handle = index.gpu_handle()
+    hits, all_steps, all_final = [], [], []
+    for c0 in range(0, len(queries), chunk):
+        q = np.ascontiguousarray(queries[c0:c0 + chunk], dtype=np.float32)
+        nq = q.shape[0]
+        t = np.ascontiguousarray(truth_ids[c0:c0 + chunk, :10], dtype=np.uint32)
+        dq = nat.DeviceArray.from_numpy(q, ctx)
+        dt = nat.DeviceArray.from_numpy(t, ctx)
+        df = nat.DeviceArray((nq, max_steps, FEATURES), np.float32, ctx)
+        dh = nat.DeviceArray((nq, max_steps), np.uint8, ctx)
+        ds = nat.DeviceArray(nq, np.uint32, ctx)
+        dfin = nat.DeviceArray(nq, np.uint32, ctx)
+        nat.check(nat.lib.vdx_gpu_hnsw_trace_device(handle, dq.ptr, nq, k, ef, dt.ptr, max_steps,
+                                                    df.ptr, dh.ptr, ds.ptr, dfin.ptr))
+        hits.append(dh.numpy())
+        all_steps.append(ds.numpy())
+        all_final.append(dfin.numpy())
+    return np.concatenate(hits), np.concatenate(all_steps), np.concatenate(all_final)
+
+
+def stop_ceiling(hits: np.ndarray, steps: np.ndarray, final: np.ndarray, target: float, k: int = 10):
+    """What an oracle stop could save on these traced searches: each query stopped at the first
+    expansion whose top-k hits equal its final hits (the training label's event), and the fewest
+    total expansions over per-query stops whose mean recall@k still reaches `target` — stopping
+    a query at the first expansion reaching h hits, h chosen per query, solved exactly as a
+    multiple-choice knapsack over the hits given up (dynamic programming). Returns mean
+    expansions per query."""
+    nq, cap = hits.shape
+    recorded = np.minimum(steps, cap)
+    # first[q][h]: the first expansion (1-based) with >= h hits; 0 for h = 0
+    first = np.zeros((nq, k + 1), dtype=np.int64)
+    for h in range(1, k + 1):
+        reach = (hits >= h) & (np.arange(cap)[None, :] < recorded[:, None])
+        first[:, h] = np.where(reach.any(axis=1), reach.argmax(axis=1) + 1, steps)
+    level = np.minimum(final, k).astype(np.int64)
+    at_final = first[np.arange(nq), level]
+    out = {"fixed_expansions": round(float(steps.mean()), 2),
+           "fixed_recall": round(float(final.mean() / k), 4),
+           "oracle_stop_expansions": round(float(at_final.mean()), 2)}
+    need = int(np.ceil(target * k * nq - 1e-9))
+    budget = int(level.sum()) - need  # hits that may be given up
+    if budget < 0:
+        out["budgeted_expansions"] = None
+        return out
+    # best[b]: the most expansions saved giving up at most b hits over the queries so far
+    best = np.zeros(budget + 1, dtype=np.int64)
+    for q in np.nonzero(level > 0)[0]:
+        f = int(level[q])
+        nxt = best.copy()
+        for d in range(1, min(f, budget) + 1):
+            save = int(at_final[q] - first[q, f - d])
+            np.maximum(nxt[d:], best[:budget + 1 - d] + save, out=nxt[d:])
+        best = nxt
+    out["budgeted_expansions"] = round(float((at_final.sum() - best[budget]) / nq), 2)
+    out["budgeted_recall"] = round(need / (k * nq), 4)
+    return out
+
+
 def train_stop_model(features: np.ndarray, labels: np.ndarray, epochs: int = 20, batch: int = 4096,
                      lr: float = 1e-3, seed: int = 0) -> tuple[StopModel, np.ndarray]:
     mean = features.mean(axis=0)
@@ -122,12 +185,14 @@ def train_stop_model(features: np.ndarray, labels: np.ndarray, epochs: int = 20,
     return model, losses
 
 
-def learned_opts(model: StopModel, threshold: float, min_steps: int = 1, stats=None) -> nat.SearchOpts:
+def learned_opts(model: StopModel, threshold: float, min_steps: int = 1, stats=None,
+                 every: int = 1) -> nat.SearchOpts:
     opts = nat.SearchOpts()
     opts.termination = 1
     opts.mlp = model.handle
     opts.mlp_threshold = float(threshold)
     opts.min_steps = int(min_steps)
+    opts.mlp_every = int(every)
     if stats is not None:
         opts.stats = ctypes.cast(stats, ctypes.c_void_p)
     return opts
@@ -148,7 +213,7 @@ def recall_of(found_ids: np.ndarray, truth_ids: np.ndarray, k: int = 10) -> floa
 
 def calibrate(index, model: StopModel, queries: np.ndarray, truth_ids: np.ndarray, k: int, ef: int,
               target: float = 0.95, min_steps: int = 1,
-              grid=(0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999)):
+              grid=(0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999), every: int = 1):
     """The threshold with the fewest expansions whose recall@k meets `target` on these queries.
     Returns (threshold, recall, expansions per query, table)."""
     ctx = nat.Context.get(0)
@@ -159,7 +224,7 @@ def calibrate(index, model: StopModel, queries: np.ndarray, truth_ids: np.ndarra
     table = []
     for t in grid:
         stats = (ctypes.c_uint64 * 8)()
-        search_device(index, dq, nq, k, ef, out, cnt, learned_opts(model, t, min_steps, stats))
+        search_device(index, dq, nq, k, ef, out, cnt, learned_opts(model, t, min_steps, stats, every))
         r = recall_of(out.numpy()["id"], truth_ids, k)
         table.append((t, r, stats[0] / nq))
     ok = [row for row in table if row[1] >= target]

@@ -260,11 +260,12 @@ typedef struct vdx_search_opts {
     float mlp_threshold;       /* stop when the model's probability exceeds this, in (0, 1) */
     float reserved_f;
     uint64_t* stats;           /* optional host array of VDX_STAT_COUNT counters, summed */
-    uint32_t row_format;       /* 0: the plain L2 search reads the 8-bit row copy where the index
-                                  keeps one (dimension 128, every stored value an integer in
-                                  [0, 255]: the same floats, so the same distances and results);
+    uint32_t row_format;       /* 0: the plain and learned L2 searches read the 8-bit row copy where
+                                  the index keeps one (dimension 128, every stored value an integer
+                                  in [0, 255]: the same floats, so the same distances and results);
                                   1: always the FP32 rows */
-    uint32_t reserved_u;
+    uint32_t mlp_every;        /* learned stop: consult the model every this many expansions once
+                                  allowed (0 or 1: after every expansion) */
 } vdx_search_opts;
 
 enum vdx_stat {

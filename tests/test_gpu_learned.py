@@ -209,3 +209,33 @@ def test_learned_stop_on_the_config_generators(config_base):
           f"{r_fixed:.4f}), expansions {stats[0] / 500:.1f} vs {fixed_stats[0] / 500:.1f}", flush=True)
     assert r >= target - 0.02
     assert stats[0] < fixed_stats[0]
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_learned_over_byte_rows_equals_fp32_rows(gmm10k, every):
+    """The learned stop over the 8-bit rows (the integer kernel for byte-valued queries, the float
+    kernel for others) computes the same distances as over the FP32 rows, hence the same features
+    and stop decisions: identical results and counters, for a model consulted every `every`
+    expansions. (One feature, the fresh neighbours of an expansion, counts the nodes the lossy
+    visited cache forgot, so both searches use one cache size.)"""
+    index, queries, truth = gmm10k
+    assert index.gpu_u8_rows()
+    f, labels, _, _ = learned.collect_traces(index, queries[:800], truth[:800], k=10, ef=64)
+    model, _ = learned.train_stop_model(f, labels, epochs=3, batch=4096, lr=1e-3)
+    q_int = queries[800:1600]
+    q_frac = q_int + np.float32(0.25)
+    for q in (q_int, q_frac):
+        ctx = nat.Context.get(0)
+        dq = nat.DeviceArray.from_numpy(q, ctx)
+        results = []
+        for rows in (0, 1):
+            stats = (ctypes.c_uint64 * 8)()
+            opts = learned.learned_opts(model, 0.8, 4, stats, every)
+            opts.row_format = rows
+            opts.visited_log2 = 11
+            out = nat.DeviceArray((len(q), 10), nat.NEIGHBOR_DTYPE, ctx)
+            cnt = nat.DeviceArray(len(q), np.uint32, ctx)
+            learned.search_device(index, dq, len(q), 10, 64, out, cnt, opts)
+            results.append((out.numpy().tobytes(), cnt.numpy().tobytes(), list(stats)[:5]))
+        assert results[0] == results[1]
+        assert results[0][2][4] > 0  # the model stopped some queries
