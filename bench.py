@@ -145,16 +145,17 @@ def threshold_recall(found, truth):
     return r.mean_threshold_recall_at_k(found, truth["distance"][:, K - 1], K)
 
 
-def measured_traffic(mode, ef, nq, rows):
+def measured_traffic(mode, ef, nq, rows, max_expansions=0):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), when it covers this
-    workload (rows: "u8" for the 8-bit row copy, "f32"); else None."""
+    workload (rows: "u8" for the 8-bit row copy, "f32"; the expansion budget, 0 for none);
+    else None."""
     path = ROOT / "profiles" / "hnsw_beam_traffic.json"
     if not path.exists():
         return None
     for row in json.loads(path.read_text()).get("workloads", []):
         if (row.get("mode") == mode and row.get("ef") == ef and row.get("nq") == nq
-                and row.get("rows", "f32") == rows):
+                and row.get("rows", "f32") == rows and row.get("max_expansions", 0) == max_expansions):
             return row.get("traffic_bytes")
     return None
 
@@ -195,6 +196,7 @@ def main(argv=None) -> int:
     ap.add_argument("--host-build-10m", action="store_true",
                     help="build the 10M graph on the host alone (default: GPU-assisted)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-bruteforce", action="store_true", help="skip the brute-force leg (profiling runs)")
     ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run the library's copy kernel once over this many MiB (counter calibration)")
@@ -224,7 +226,7 @@ def main(argv=None) -> int:
     gt_s = time.perf_counter() - t
     calib_queries = mix.sample(args.nq, 4)
     calib_truth, _ = bf.search_arrays(calib_queries, K, "bf16")
-    bf_leg = bruteforce_leg(args, nat, bf, queries, vdx.Metric.L2, threads)
+    bf_leg = None if args.no_bruteforce else bruteforce_leg(args, nat, bf, queries, vdx.Metric.L2, threads)
     del bf
 
     # Fixed-ef sweep: recall on both query sets, time on the evaluation set.
@@ -253,13 +255,15 @@ def main(argv=None) -> int:
     fine_points = [s for s in sweep if s["calib_recall"] >= TARGET_RECALL]
     fixed_best = min(fine_points, key=lambda s: s["ef"]) if fine_points else fixed
 
-    # Learned termination.
+    # Adaptive termination: the expansion budget, then the learned stop (with and without it).
     learned_leg = None
     headline = {"mode": "fixed-ef", "ef": fixed_best["ef"], "opts": None, "recall": fixed_best["recall"],
                 "qps": fixed_best["qps"]}
+    budget_leg, headline = run_budget(args, nat, index, queries, truth, calib_queries, calib_truth,
+                                      fixed_best, headline)
     if not args.no_learned:
         learned_leg, headline = run_learned(args, vdx, nat, datasets, learned, index, base, queries,
-                                            truth, mix, threads, fixed, fixed_best, headline)
+                                            truth, mix, threads, fixed, fixed_best, headline, budget_leg)
     serving_leg = None if args.no_serving else run_serving(args, vdx, nat, datasets, threads)
     base10m_leg = None if args.no_base10m else run_10m(args, vdx, nat, datasets, threads)
 
@@ -279,9 +283,10 @@ def main(argv=None) -> int:
     counts = run.counters(headline["ef"], headline["opts"], visited_log2=13)
     expansions, distances, upper_blocks, dups = counts[:4]
     d, M, nq = 128, 16, args.nq
-    # The plain search reads the 8-bit row copy when the index keeps one (every stored value an
-    # integer in [0, 255]: the same floats, so the same results): d bytes per row, else 4d.
-    rows = "u8" if index.gpu_u8_rows() and headline["mode"] == "fixed-ef" else "f32"
+    # The plain and learned searches read the 8-bit row copy when the index keeps one (every
+    # stored value an integer in [0, 255]: the same floats, so the same results): d bytes per
+    # row, else 4d.
+    rows = "u8" if index.gpu_u8_rows() else "f32"
     row_bytes = d if rows == "u8" else 4 * d
     alg_bytes = (nq * 4 * d + upper_blocks * 4 * (1 + M) + expansions * 4 * (1 + 2 * M)
                  + (distances - dups) * row_bytes + nq * 8 * K)
@@ -292,7 +297,8 @@ def main(argv=None) -> int:
     f32_leg = None
     if rows == "u8":
         # the same launches over the FP32 rows: the kernel the row copy replaces
-        f32 = nat.SearchOpts()
+        # (the headline's own options, the row format forced)
+        f32 = nat.SearchOpts.from_buffer_copy(headline["opts"]) if headline["opts"] is not None else nat.SearchOpts()
         f32.row_format = 1
         ms32, _ = run.timed(headline["ef"], args.steps, args.warmup, f32)
         res32 = run.results()
@@ -303,7 +309,7 @@ def main(argv=None) -> int:
                    "identical_to_u8_rows": bool(res32.tobytes() == res.tobytes()),
                    "achieved_gbs": round(bytes32 / (ms32 / 1000.0) / 1e9, 1),
                    "frac": round(bytes32 / (ms32 / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
-                   "traffic": measured_traffic(headline["mode"], headline["ef"], nq, "f32"),
+                   "traffic": measured_traffic(headline["mode"], headline["ef"], nq, "f32", f32.max_expansions),
                    "algorithmic_bytes_per_launch": bytes32}
         log(f"same ef over the FP32 rows: {ms32:.3f} ms, {nq / ms32 * 1000:,.0f} QPS "
             f"({ms32 / ms:.2f}x the 8-bit rows' time), results identical: {f32_leg['identical_to_u8_rows']}")
@@ -362,16 +368,19 @@ def main(argv=None) -> int:
                                f"mixture (1,024 components, integer-valued), {nq:,} queries, graph "
                                f"M=16 efConstruction=200 seed 0, k=10",
                    "ef": headline["ef"], "termination": headline["mode"],
+                   "max_expansions": headline["opts"].max_expansions if headline["opts"] is not None else 0,
                    "recall@10": round(rec, 4), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(headline["mode"], headline["ef"], nq, rows),
+                     "traffic": measured_traffic(headline["mode"], headline["ef"], nq, rows,
+                                                 headline["opts"].max_expansions if headline["opts"] else 0),
                      "algorithmic_bytes_per_launch": alg_bytes, "rows": rows},
         "cpu_baseline": cpu,
         "fixed_ef": {"ef": fixed["ef"], "recall": fixed["recall"], "qps": fixed["qps"],
                      "best_fine_ef": fixed_best["ef"], "best_fine_qps": fixed_best["qps"],
                      "best_fine_recall": fixed_best["recall"]},
         "headline_over_f32_rows": f32_leg,
+        "budget": budget_leg,
         "learned": learned_leg,
         "sweep": sweep,
         "bruteforce": bf_leg,
@@ -737,8 +746,56 @@ def run_gpu_build(vdx, nat, base, queries, truth, threads, host_build_s, sweep):
             "search": rows}
 
 
+BUDGET_EFS = (36, 38, 40, 42, 44, 48, 56)
+BUDGETS = (44, 48, 52, 56, 60, 64, 72, 96)
+
+
+def run_budget(args, nat, index, queries, truth, calib, calib_truth, fixed_best, headline):
+    """Fixed ef with an expansion budget (vdx_search_opts.max_expansions): every query stops after
+    at most B layer-0 expansions. A batch takes as long as its slowest queries, and a budget cuts
+    exactly those, so it can gain more time than the expansions it saves. (ef, B) is chosen on the
+    held-out queries (recall >= target + CALIB_MARGIN, fastest there), then measured on the
+    evaluation queries."""
+    rc, re_ = Runner(nat, index, calib), Runner(nat, index, queries)
+    table, best = [], None
+    for budget in BUDGETS:
+        for ef in BUDGET_EFS:
+            o = nat.SearchOpts()
+            o.max_expansions = budget
+            rc.launch(ef, o)
+            r_c = recall(rc.results(), calib_truth)
+            if r_c < TARGET_RECALL + CALIB_MARGIN:
+                continue
+            ms_c, _ = rc.timed(ef, max(3, args.steps // 2), 2, o)
+            table.append({"ef": ef, "max_expansions": budget, "calib_recall": round(r_c, 4), "calib_ms": round(ms_c, 4)})
+            if best is None or ms_c < best[0]:
+                best = (ms_c, ef, budget, o, r_c)
+    del rc
+    if best is None:
+        return {"table": table}, headline
+    _, ef, budget, o, r_c = best
+    re_.launch(ef, o)
+    rec = recall(re_.results(), truth)
+    ms, _ = re_.timed(ef, args.steps, args.warmup, o)
+    counts = re_.counters(ef, o)
+    leg = {"ef": ef, "max_expansions": budget, "calib_recall": round(r_c, 4), "eval_recall": round(rec, 4),
+           "ms": round(ms, 4), "qps": round(len(queries) / ms * 1000, 1),
+           "expansions": round(counts[0] / len(queries), 1),
+           "stopped_by_budget": round(counts[4] / len(queries), 4),
+           "fixed_ef_at_target": {k: fixed_best[k] for k in ("ef", "recall", "qps")},
+           "speedup_vs_fixed_ef": round(len(queries) / ms * 1000 / fixed_best["qps"], 3),
+           "table": table}
+    log(f"budget: ef {ef}, at most {budget} expansions (held-out recall {r_c:.4f}): eval recall {rec:.4f}, "
+        f"{ms:.3f} ms, {leg['qps']:,.0f} QPS ({leg['speedup_vs_fixed_ef']}x fixed ef {fixed_best['ef']}); "
+        f"{leg['expansions']} expansions, {leg['stopped_by_budget'] * 100:.1f}% of queries stopped by the budget")
+    if rec >= TARGET_RECALL and leg["qps"] > headline["qps"]:
+        headline = {"mode": "budget", "ef": ef, "opts": o, "qps": leg["qps"], "recall": rec}
+    del re_
+    return leg, headline
+
+
 def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, mix, threads,
-                fixed, fixed_best, headline):
+                fixed, fixed_best, headline, budget_leg=None):
     """Traces, training, calibration and evaluation of the learned stop on both bases, and the
     oracle-stop ceiling of each.
 
@@ -846,22 +903,29 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
                 f"{c['fixed_recall']}), oracle stop {c['oracle_stop_expansions']}, budgeted to {target}: "
                 f"{c.get('budgeted_expansions')}")
         best = None
+        # expansion budgets tried with the model: none, and (L2) the budget leg's and a looser one
+        budgets = [0]
+        if name == "l2" and budget_leg and budget_leg.get("max_expansions"):
+            budgets += [budget_leg["max_expansions"], budget_leg["max_expansions"] + 8]
         for ef_cap in s["caps"]:
             for every in (4, 8):  # consult the model every 4th or 8th expansion
-                thr, rec_c, work, table = learned.calibrate(s["index"], model, s["calib"], s["truth_calib"]["id"],
-                                                            K, ef_cap, target + CALIB_MARGIN, every=every,
-                                                            grid=THRESHOLDS)
-                if rec_c < target + CALIB_MARGIN:
-                    log(f"learned/{name} ef cap {ef_cap}, every {every}: no threshold reaches "
-                        f"{target + CALIB_MARGIN:.4f} (best {rec_c:.4f})")
-                    continue
-                opts = learned.learned_opts(model, thr, 1, every=every)
-                ms_c, _ = rc.timed(ef_cap, max(3, args.steps // 2), 2, opts)
-                log(f"learned/{name} ef cap {ef_cap}, every {every}: threshold {thr}, calib recall {rec_c:.4f}, "
-                    f"{work:.1f} expansions, {len(s['calib']) / ms_c * 1000:,.0f} QPS on calibration")
-                if best is None or ms_c < best["_ms_c"]:
-                    best = {"ef_cap": ef_cap, "every": every, "threshold": thr, "calib_recall": round(rec_c, 4),
-                            "_ms_c": ms_c, "_opts": opts}
+                for budget in budgets:
+                    thr, rec_c, work, table = learned.calibrate(s["index"], model, s["calib"],
+                                                                s["truth_calib"]["id"], K, ef_cap,
+                                                                target + CALIB_MARGIN, every=every, grid=THRESHOLDS,
+                                                                max_expansions=budget)
+                    if rec_c < target + CALIB_MARGIN:
+                        log(f"learned/{name} ef cap {ef_cap}, every {every}, budget {budget}: no threshold "
+                            f"reaches {target + CALIB_MARGIN:.4f} (best {rec_c:.4f})")
+                        continue
+                    opts = learned.learned_opts(model, thr, 1, every=every, max_expansions=budget)
+                    ms_c, _ = rc.timed(ef_cap, max(3, args.steps // 2), 2, opts)
+                    log(f"learned/{name} ef cap {ef_cap}, every {every}, budget {budget}: threshold {thr}, "
+                        f"calib recall {rec_c:.4f}, {work:.1f} expansions, "
+                        f"{len(s['calib']) / ms_c * 1000:,.0f} QPS on calibration")
+                    if best is None or ms_c < best["_ms_c"]:
+                        best = {"ef_cap": ef_cap, "every": every, "threshold": thr, "max_expansions": budget,
+                                "calib_recall": round(rec_c, 4), "_ms_c": ms_c, "_opts": opts}
         fe = fixed_eval[name]
         if fe is not None:
             # the fixed search's expansions at its operating point, on the evaluation queries
@@ -878,6 +942,8 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
                          "fixed_ef_at_target": fe})
             if fe is not None:
                 best["speedup_vs_fixed_ef"] = round(best["qps"] / fe["qps"], 3)
+            if name == "l2" and budget_leg and budget_leg.get("qps"):
+                best["speedup_vs_budget"] = round(best["qps"] / budget_leg["qps"], 3)
             log(f"learned/{name}: eval recall {rec_e:.4f} (target {target}), {ms:.3f} ms, "
                 f"{best['qps']:,.0f} QPS, {exp:.1f} expansions; fixed ef at target: {fe}")
         if fe is not None:

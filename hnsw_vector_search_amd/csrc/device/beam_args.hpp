@@ -72,6 +72,7 @@ struct BeamArgs {
     float stop_logit;
     std::uint32_t min_steps;
     std::uint32_t mlp_every;  // consult the model every this many expansions (>= 1)
+    std::uint32_t max_exp;  // expansion budget: layer-0 link blocks per query at most (0xFFFFFFFF: none)
 };
 
 constexpr int kBeamStats = 7;

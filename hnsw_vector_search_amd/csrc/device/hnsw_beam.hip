@@ -310,6 +310,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         const float inv_m2 = 1.0f / static_cast<float>(a.M2);
         std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = 0xFFFFFFFFu;
+        std::uint32_t qexp = 0;
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
                 truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
@@ -330,6 +331,14 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
             if (p < 0) {
                 break;
             }
+            if (qexp >= a.max_exp) {
+                // the expansion budget ends the search with an open member left
+                if (lane == 0 && a.stats != nullptr) {
+                    atomicAdd(a.stats + 4, 1ull);
+                }
+                break;
+            }
+            ++qexp;
             const std::uint64_t kp = Rl[p];
             const std::uint32_t node = uniform(key_id(kp));
             wave_lds_fence();

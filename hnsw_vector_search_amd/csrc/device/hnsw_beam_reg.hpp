@@ -308,6 +308,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         const float inv_m2 = 1.0f / static_cast<float>(a.M2);
         std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = kNone;
+        std::uint32_t qexp = 0;
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
                 truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
@@ -321,6 +322,15 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             if (pc < 0) {
                 break;
             }
+            if (qexp >= a.max_exp) {
+                // the expansion budget ends the search with an open member left (counted at once:
+                // no counter kept live through the search)
+                if (lane == 0 && a.stats != nullptr) {
+                    atomicAdd(a.stats + 4, 1ull);
+                }
+                break;
+            }
+            ++qexp;
             std::uint64_t kp = 0;
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
@@ -574,23 +584,30 @@ __global__ __launch_bounds__(256) void hnsw_beam_reg_kernel(BeamArgs a) {
     beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, MODE>(a);
 }
 
-template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH, bool U8 = false>
+template <int NB, int N8, int TAIL, int RR, bool FIXED, int CH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void hnsw_beam_reg_plain_kernel(
     BeamArgs a) {
     // the metric fixed per copy of the body (one uniform branch per launch instead of one per
     // row block): 2-3% faster over the FP32 rows (measured, L2 and cosine)
-    if constexpr (U8) {
-        // a launch whose queries are all byte-valued (*qflag != qflag_epoch) is served by
-        // hnsw_beam_reg_u8int_kernel, enqueued with this one
-        if (a.qflag != nullptr && *a.qflag != a.qflag_epoch) {
-            return;
-        }
-        beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, true, 1, false>(a);
-    } else if (a.l2 != 0) {
+    if (a.l2 != 0) {
         beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, false, 1>(a);
     } else {
         beam_reg_body<NB, N8, TAIL, RR, FIXED, CH, kBeamPlain, false, 0>(a);
     }
+}
+
+// The plain search over the 8-bit rows for a launch with a query that is not byte-valued
+// (*qflag == qflag_epoch after query_bytes_check; the float arithmetic over the converted bytes).
+// Held to 80 VGPRs (6 waves per SIMD, the occupancy its LDS allows) at R capacity 64.
+template <int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 1 ? 6 : 4))) void hnsw_beam_reg_u8_kernel(
+    BeamArgs a) {
+    // a launch whose queries are all byte-valued is served by hnsw_beam_reg_u8int_kernel,
+    // enqueued with this one
+    if (a.qflag != nullptr && *a.qflag != a.qflag_epoch) {
+        return;
+    }
+    beam_reg_body<4, 0, 0, 4, true, CH, kBeamPlain, true, 1, false>(a);
 }
 
 // The plain search over the 8-bit rows when every query of the launch has all its components
@@ -666,7 +683,7 @@ const void* hnsw_beam_reg_ptr(BeamShape shape, int mode) {
         case BeamShape::D128:
             return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<4, 0, 0, 4, true, CH>);
         case BeamShape::D128U8:
-            return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<4, 0, 0, 4, true, CH, true>);
+            return reinterpret_cast<const void*>(&hnsw_beam_reg_u8_kernel<CH>);
         case BeamShape::D100:
             return reinterpret_cast<const void*>(&hnsw_beam_reg_plain_kernel<3, 0, 4, 4, true, CH>);
         default:

@@ -186,13 +186,15 @@ def train_stop_model(features: np.ndarray, labels: np.ndarray, epochs: int = 20,
 
 
 def learned_opts(model: StopModel, threshold: float, min_steps: int = 1, stats=None,
-                 every: int = 1) -> nat.SearchOpts:
+                 every: int = 1, max_expansions: int = 0) -> nat.SearchOpts:
+    """Search options of the learned stop; max_expansions adds the expansion budget (0: none)."""
     opts = nat.SearchOpts()
     opts.termination = 1
     opts.mlp = model.handle
     opts.mlp_threshold = float(threshold)
     opts.min_steps = int(min_steps)
     opts.mlp_every = int(every)
+    opts.max_expansions = int(max_expansions)
     if stats is not None:
         opts.stats = ctypes.cast(stats, ctypes.c_void_p)
     return opts
@@ -213,9 +215,10 @@ def recall_of(found_ids: np.ndarray, truth_ids: np.ndarray, k: int = 10) -> floa
 
 def calibrate(index, model: StopModel, queries: np.ndarray, truth_ids: np.ndarray, k: int, ef: int,
               target: float = 0.95, min_steps: int = 1,
-              grid=(0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999), every: int = 1):
-    """The threshold with the fewest expansions whose recall@k meets `target` on these queries.
-    Returns (threshold, recall, expansions per query, table)."""
+              grid=(0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999), every: int = 1,
+              max_expansions: int = 0):
+    """The threshold with the fewest expansions whose recall@k meets `target` on these queries
+    (under the expansion budget, if any). Returns (threshold, recall, expansions per query, table)."""
     ctx = nat.Context.get(0)
     nq = len(queries)
     dq = nat.DeviceArray.from_numpy(np.ascontiguousarray(queries, dtype=np.float32), ctx)
@@ -224,7 +227,8 @@ def calibrate(index, model: StopModel, queries: np.ndarray, truth_ids: np.ndarra
     table = []
     for t in grid:
         stats = (ctypes.c_uint64 * 8)()
-        search_device(index, dq, nq, k, ef, out, cnt, learned_opts(model, t, min_steps, stats, every))
+        search_device(index, dq, nq, k, ef, out, cnt, learned_opts(model, t, min_steps, stats, every,
+                                                                   max_expansions))
         r = recall_of(out.numpy()["id"], truth_ids, k)
         table.append((t, r, stats[0] / nq))
     ok = [row for row in table if row[1] >= target]

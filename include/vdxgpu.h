@@ -266,6 +266,10 @@ typedef struct vdx_search_opts {
                                   1: always the FP32 rows */
     uint32_t mlp_every;        /* learned stop: consult the model every this many expansions once
                                   allowed (0 or 1: after every expansion) */
+    uint32_t max_expansions;   /* expansion budget: a query stops after this many layer-0 link
+                                  blocks (0: no budget, vecdex's search). A stopped query returns
+                                  the closest k found so far; under either termination. No vecdex
+                                  counterpart: a serving bound on the slowest queries of a batch */
 } vdx_search_opts;
 
 enum vdx_stat {
@@ -273,7 +277,8 @@ enum vdx_stat {
     VDX_STAT_DISTANCES = 1,    /* distances computed (layer 0 and above) */
     VDX_STAT_UPPER_BLOCKS = 2, /* upper-layer link blocks read */
     VDX_STAT_DUPLICATES = 3,   /* recomputed members of the result list the lossy visited cache let through */
-    VDX_STAT_EARLY_STOPS = 4,  /* queries the learned model stopped before the fixed-ef stop */
+    VDX_STAT_EARLY_STOPS = 4,  /* queries stopped before the fixed-ef stop, by the learned model or
+                                  by the expansion budget */
     VDX_STAT_LINK_PREFETCHED = 5, /* layer-0 link blocks that were already prefetched when needed */
     VDX_STAT_CLAMPED = 6,      /* queries whose max(ef, k) exceeded the launch's result-list capacity
                                   (the device entry point's `ef` bound) and ran at that capacity */

@@ -54,6 +54,9 @@ class _Graph(ctypes.Structure):
 _lib.vdxo_hnsw_search_batch.restype = _int
 _lib.vdxo_hnsw_search_batch.argtypes = [ctypes.POINTER(_Graph), _p, _u64, _u64, _u64, _p, _u64,
                                         _p, _int, _int]
+_lib.vdxo_hnsw_search_batch_bounded.restype = _int
+_lib.vdxo_hnsw_search_batch_bounded.argtypes = [ctypes.POINTER(_Graph), _p, _u64, _u64, _u64, _u64, _p, _u64,
+                                                _p, _p, _int, _int]
 _lib.vdxo_hnsw_search_batch_from.restype = _int
 _lib.vdxo_hnsw_search_batch_from.argtypes = [ctypes.POINTER(_Graph), _p, _p, _u64, _u64, _u64, _u64, _p,
                                              _u64, _p, _int, _int]
@@ -109,13 +112,15 @@ def bruteforce_f32(base, queries, k: int, metric: int = 0, threads: int = 0) -> 
 
 
 def hnsw_search(graph: dict, queries, k: int, ef: int, threads: int = 0, fast: bool = True,
-                entries=None, layer: int = 0):
+                entries=None, layer: int = 0, max_expansions: int | None = None):
     """HnswIndex::search for every row of prepared queries (normalized under cosine) over a
     graph given as FORMAT v1 sections (dict of numpy arrays and header fields).
     layer: the layer the beam searches (0 for queries; insert searches each of its layers).
     entries: per query, an entry on that layer that replaces the greedy descent (insert's
     searches start from the layer above's result, src/hnsw.cpp:324-335), or 0xFFFFFFFF.
-    Returns (structured [nq, k] array, counts[nq])."""
+    max_expansions (layer 0, no entries; not vecdex's): the expansion budget, 0 for none; then
+    the result also carries each query's expansions taken.
+    Returns (structured [nq, k] array, counts[nq]) [, expansions[nq]]."""
     queries = _c(queries, np.float32)
     keep = [graph[name] for name in ("levels", "layer0", "upper", "vectors")]
     g = _Graph(graph["metric"], graph["dim"], graph["n"], graph["M"], graph["max_level"],
@@ -123,6 +128,14 @@ def hnsw_search(graph: dict, queries, k: int, ef: int, threads: int = 0, fast: b
     nq = queries.shape[0]
     out = np.zeros((nq, k), dtype=NEIGHBOR)
     count = np.zeros(nq, dtype=np.uint32)
+    if max_expansions is not None:
+        assert entries is None and layer == 0
+        taken = np.zeros(nq, dtype=np.uint32)
+        _lib.vdxo_hnsw_search_batch_bounded(ctypes.byref(g), queries.ctypes.data, nq, k, ef, int(max_expansions),
+                                            out.ctypes.data, k, count.ctypes.data, taken.ctypes.data, threads,
+                                            1 if fast else 0)
+        del keep
+        return out, count, taken
     ent = None if entries is None else _c(entries, np.uint32)
     _lib.vdxo_hnsw_search_batch_from(ctypes.byref(g), queries.ctypes.data,
                                      None if ent is None else ent.ctypes.data, layer, nq, k, ef, out.ctypes.data,

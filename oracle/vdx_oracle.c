@@ -427,6 +427,8 @@ typedef struct {
     vdxo_neighbor* out;
     uint64_t out_stride;
     uint32_t* out_count;
+    uint64_t max_expansions; /* expansions of the searched layer per query at most (UINT64_MAX: none) */
+    uint32_t* expansions;    /* per query: expansions taken; may be NULL */
     dist_fn l2, ip;
 } hnsw_ctx;
 
@@ -488,12 +490,16 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
         s->epoch = 1;
     }
     size_t ncand = 0, nres = 0;
+    uint64_t taken = 0;
     s->marks[entry.id] = s->epoch;
     heap_push(s->cand, &ncand, entry, closer);
     heap_push(s->res, &nres, entry, farther);
     while (ncand > 0) {
         vdxo_neighbor cur = heap_pop(s->cand, &ncand, closer);
         if (nres >= ef && closer(s->res[0], cur)) break;
+        /* the expansion budget (no vecdex counterpart): stop with the results found so far */
+        if (taken >= c->max_expansions) break;
+        ++taken;
         const uint32_t* b = links_of(c, cur.id, c->layer);
         uint32_t nf = 0;
         for (uint32_t i = 0; i < b[0]; ++i) {
@@ -525,6 +531,7 @@ static void hnsw_one(void* vctx, uint64_t q, void* local) {
     uint64_t count = nres < c->k ? nres : c->k;
     memcpy(c->out + q * c->out_stride, s->res, count * sizeof(vdxo_neighbor));
     c->out_count[q] = (uint32_t)count;
+    if (c->expansions != NULL) c->expansions[q] = (uint32_t)taken;
 }
 
 typedef struct {
@@ -543,10 +550,13 @@ static void hnsw_cleanup(void* local) {
     free(s->fresh);
 }
 
-int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const uint32_t* entries,
-                                uint64_t layer, uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out,
-                                uint64_t out_stride, uint32_t* out_count, int threads, int use_fast_kernel) {
+static int search_batch(const vdxo_graph* g, const float* queries, const uint32_t* entries, uint64_t layer,
+                        uint64_t nq, uint64_t k, uint64_t ef, uint64_t max_expansions, vdxo_neighbor* out,
+                        uint64_t out_stride, uint32_t* out_count, uint32_t* expansions, int threads,
+                        int use_fast_kernel) {
     hnsw_ctx c;
+    c.max_expansions = max_expansions == 0 ? UINT64_MAX : max_expansions;
+    c.expansions = expansions;
     c.g = g;
     c.queries = queries;
     c.entries = entries;
@@ -571,11 +581,26 @@ int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const
     return 0;
 }
 
+int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const uint32_t* entries,
+                                uint64_t layer, uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out,
+                                uint64_t out_stride, uint32_t* out_count, int threads, int use_fast_kernel) {
+    return search_batch(g, queries, entries, layer, nq, k, ef, 0, out, out_stride, out_count, NULL, threads,
+                        use_fast_kernel);
+}
+
 int vdxo_hnsw_search_batch(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
                            uint64_t ef, vdxo_neighbor* out, uint64_t out_stride,
                            uint32_t* out_count, int threads, int use_fast_kernel) {
-    return vdxo_hnsw_search_batch_from(g, queries, NULL, 0, nq, k, ef, out, out_stride, out_count, threads,
-                                       use_fast_kernel);
+    return search_batch(g, queries, NULL, 0, nq, k, ef, 0, out, out_stride, out_count, NULL, threads,
+                        use_fast_kernel);
+}
+
+int vdxo_hnsw_search_batch_bounded(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
+                                   uint64_t ef, uint64_t max_expansions, vdxo_neighbor* out,
+                                   uint64_t out_stride, uint32_t* out_count, uint32_t* expansions,
+                                   int threads, int use_fast_kernel) {
+    return search_batch(g, queries, NULL, 0, nq, k, ef, max_expansions, out, out_stride, out_count, expansions,
+                        threads, use_fast_kernel);
 }
 
 /* ---------------------------------------------------------------- XXH64 */

@@ -76,6 +76,15 @@ int vdxo_hnsw_search_batch_from(const vdxo_graph* g, const float* queries, const
                                 uint64_t layer, uint64_t nq, uint64_t k, uint64_t ef, vdxo_neighbor* out,
                                 uint64_t out_stride, uint32_t* out_count, int threads, int use_fast_kernel);
 
+/* vdxo_hnsw_search_batch with an expansion budget (no vecdex counterpart): a query stops after
+ * `max_expansions` layer-0 expansions (0: no budget) and returns the closest k found so far;
+ * expansions[q] (if not NULL) gets the expansions it took. With a budget at or above a query's
+ * own count the search is vecdex's. */
+int vdxo_hnsw_search_batch_bounded(const vdxo_graph* g, const float* queries, uint64_t nq, uint64_t k,
+                                   uint64_t ef, uint64_t max_expansions, vdxo_neighbor* out,
+                                   uint64_t out_stride, uint32_t* out_count, uint32_t* expansions,
+                                   int threads, int use_fast_kernel);
+
 /* XXH64 from the xxHash specification (doc/xxhash_spec.md), one shot. */
 uint64_t vdxo_xxh64(const void* data, size_t size, uint64_t seed);
 
