@@ -53,6 +53,8 @@ BF16_MFMA_TFLOPS = 2500.0
 FP8_MFMA_TFLOPS = 5000.0
 EF_SWEEP = (16, 32, 64, 128, 512)
 EF_FINE = (34, 36, 38, 40, 44, 48, 56)
+BUDGET_EFS = (36, 38, 40, 42, 44, 48, 56)    # the expansion-budget leg's grid
+BUDGETS = (44, 48, 52, 56, 60, 64, 72, 96)
 TARGET_RECALL = 0.95
 CALIB_MARGIN = 0.002  # about the sampling error of recall@10 over 10,000 held-out queries
 THRESHOLDS = (0.5, 0.6, 0.65, 0.7, 0.72, 0.75, 0.78, 0.8, 0.83, 0.85, 0.88, 0.9, 0.93, 0.95, 0.97, 0.98, 0.99,
@@ -198,6 +200,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bruteforce", action="store_true", help="skip the brute-force leg (profiling runs)")
     ap.add_argument("--efs", type=str, default=",".join(map(str, EF_SWEEP + EF_FINE)))
+    ap.add_argument("--budget-efs", type=str, default=",".join(map(str, BUDGET_EFS)),
+                    help="ef grid of the expansion-budget leg")
+    ap.add_argument("--budgets", type=str, default=",".join(map(str, BUDGETS)),
+                    help="expansion budgets of the budget leg (one ef and one budget pin the policy)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run the library's copy kernel once over this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -746,10 +752,6 @@ def run_gpu_build(vdx, nat, base, queries, truth, threads, host_build_s, sweep):
             "search": rows}
 
 
-BUDGET_EFS = (36, 38, 40, 42, 44, 48, 56)
-BUDGETS = (44, 48, 52, 56, 60, 64, 72, 96)
-
-
 def run_budget(args, nat, index, queries, truth, calib, calib_truth, fixed_best, headline):
     """Fixed ef with an expansion budget (vdx_search_opts.max_expansions): every query stops after
     at most B layer-0 expansions. A batch takes as long as its slowest queries, and a budget cuts
@@ -758,8 +760,8 @@ def run_budget(args, nat, index, queries, truth, calib, calib_truth, fixed_best,
     evaluation queries."""
     rc, re_ = Runner(nat, index, calib), Runner(nat, index, queries)
     table, best = [], None
-    for budget in BUDGETS:
-        for ef in BUDGET_EFS:
+    for budget in map(int, args.budgets.split(",")):
+        for ef in map(int, args.budget_efs.split(",")):
             o = nat.SearchOpts()
             o.max_expansions = budget
             rc.launch(ef, o)
