@@ -94,10 +94,12 @@ inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited
 
 // The register-resident beam (hnsw_beam_reg.hpp) holds R in registers in chunks of 64 keys;
 // it serves ef' <= kRegMaxEf. Its LDS per wave: the merge buffer, fresh ids and distances,
-// features and the visited cache.
+// features (trace and learned modes only) and the visited cache. Without the features a
+// 4-wave workgroup with capacity 64 and 2^10 visited slots takes 20,480 bytes, so eight of
+// them share a CU's 160 KiB: 32 waves per CU, all the integer kernel's registers allow.
 constexpr std::uint32_t kRegMaxEf = 512;
-inline std::size_t beam_reg_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {
-    return 8u * static_cast<std::size_t>(ef_cap) + 4u * 64 + 4u * 64 + 4u * 16 +
+inline std::size_t beam_reg_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2, bool features) {
+    return 8u * static_cast<std::size_t>(ef_cap) + 4u * 64 + 4u * 64 + (features ? 4u * 16 : 0u) +
            (std::size_t{4} << visited_log2);
 }
 

@@ -82,7 +82,7 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     vdx::require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
     p.waves = cfg.waves ? cfg.waves : 4;
     vdx::require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
-    p.lds_per_wave = p.chunks > 0 ? vdx::dev::beam_reg_lds_per_wave(p.ef_cap, p.visited_log2)
+    p.lds_per_wave = p.chunks > 0 ? vdx::dev::beam_reg_lds_per_wave(p.ef_cap, p.visited_log2, mode != vdx::dev::kBeamPlain)
                                   : vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
     const std::size_t limit = h.ctx->lds_per_block;
     const std::size_t shared = vdx::dev::beam_lds_shared(mode);

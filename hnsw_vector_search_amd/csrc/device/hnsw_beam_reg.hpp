@@ -115,7 +115,8 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
     std::uint32_t* fid = reinterpret_cast<std::uint32_t*>(B + 64 * CH);
     float* fd = reinterpret_cast<float*>(fid + 64);
     float* feat = fd + 64;
-    std::uint32_t* vis = reinterpret_cast<std::uint32_t*>(feat + 16);
+    // the plain search keeps no features: the visited cache follows the fresh distances
+    std::uint32_t* vis = reinterpret_cast<std::uint32_t*>(MODE == kBeamPlain ? feat : feat + 16);
     MlpWeights* model = nullptr;
     if constexpr (MODE == kBeamLearned) {
         model = reinterpret_cast<MlpWeights*>(lds + static_cast<std::size_t>(blockDim.x >> 6) * a.lds_per_wave);
