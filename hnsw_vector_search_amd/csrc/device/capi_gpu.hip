@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../host/batched_build.hpp"
@@ -445,14 +447,68 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
 }
 
 // Validates and prepares nq host queries into pinned staging (rows of dim floats).
-void prepare_queries(const char* where, vdx::Metric metric, std::uint64_t dim, std::uint64_t size,
-                     const float* queries, std::uint64_t nq, std::uint64_t query_dim,
-                     const std::uint32_t* ks, std::uint32_t k, float* staging) {
+// Queries per item of the host pool's validation and unpacking, and per group of the host
+// entry point's staged upload.
+constexpr std::uint64_t kHostChunk = 128;
+constexpr std::uint64_t kHostGroup = 2048;
+
+// Checks every query (vecdex's check_query, its messages; the first bad row's error, as a
+// sequential loop reports) and writes the prepared rows (normalized under cosine) to `staging`,
+// in chunks on the context's host threads. `on_group(b, e)`, when given, is called once rows
+// [b, e) of a group of kHostGroup rows are all staged (from whichever thread staged the last of
+// them; not for a group holding a bad row).
+void prepare_queries(vdx_gpu_ctx& ctx, const char* where, vdx::Metric metric, std::uint64_t dim,
+                     std::uint64_t size, const float* queries, std::uint64_t nq, std::uint64_t query_dim,
+                     const std::uint32_t* ks, std::uint32_t k, float* staging,
+                     const std::function<void(std::uint64_t, std::uint64_t)>& on_group = {}) {
     const vdx::IndexShape shape{metric, static_cast<std::size_t>(dim), static_cast<std::size_t>(size)};
-    for (std::uint64_t q = 0; q < nq; ++q) {
-        const float* row = queries + q * query_dim;
-        vdx::check_query(where, shape, row, query_dim, ks ? ks[q] : k);
-        vdx::prepare_query(metric, row, dim, staging + q * dim);
+    const std::string w(where);
+    constexpr std::uint64_t kPerGroup = kHostGroup / kHostChunk;
+    const std::size_t chunks = static_cast<std::size_t>((nq + kHostChunk - 1) / kHostChunk);
+    const std::size_t groups = (chunks + kPerGroup - 1) / kPerGroup;
+    std::unique_ptr<std::atomic<std::uint32_t>[]> left(new std::atomic<std::uint32_t>[groups]);
+    for (std::size_t g = 0; g < groups; ++g) {
+        left[g].store(static_cast<std::uint32_t>(std::min<std::uint64_t>(kPerGroup, chunks - g * kPerGroup)));
+    }
+    const auto stage = [&](std::uint64_t begin, std::uint64_t end) {
+        if (metric != vdx::Metric::Cosine && query_dim == dim) {
+            // Squared L2 and inner product stage the rows as they are: one pass copies them and
+            // looks for a non-finite value (exponent all ones) and a zero k; a chunk with either
+            // is redone row by row below, which raises check_query's error for its first bad row.
+            bool clean = true;
+            for (std::uint64_t q = begin; q < end; ++q) {
+                const std::uint32_t* src = reinterpret_cast<const std::uint32_t*>(queries + q * dim);
+                std::uint32_t* dst = reinterpret_cast<std::uint32_t*>(staging + q * dim);
+                std::uint32_t bad = (ks ? ks[q] : k) == 0 ? 1u : 0u;
+                for (std::uint64_t i = 0; i < dim; ++i) {
+                    const std::uint32_t b = src[i];
+                    dst[i] = b;
+                    bad |= (b & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+                }
+                clean = clean && bad == 0;
+            }
+            if (clean) {
+                return;
+            }
+        }
+        for (std::uint64_t q = begin; q < end; ++q) {
+            const float* row = queries + q * query_dim;
+            vdx::check_query(w, shape, row, query_dim, ks ? ks[q] : k);
+            vdx::prepare_query(metric, row, dim, staging + q * dim);
+        }
+    };
+    const auto chunk = [&](std::size_t c) {
+        const std::uint64_t begin = c * kHostChunk;
+        stage(begin, std::min<std::uint64_t>(nq, begin + kHostChunk));
+        const std::size_t g = c / kPerGroup;
+        if (on_group && left[g].fetch_sub(1) == 1) {
+            on_group(g * kHostGroup, std::min<std::uint64_t>(nq, (g + 1) * kHostGroup));
+        }
+    };
+    if (ctx.host_pool) {
+        ctx.host_pool->run(chunks, chunk);
+    } else {
+        for (std::size_t c = 0; c < chunks; ++c) chunk(c);
     }
 }
 
@@ -477,6 +533,8 @@ int vdx_gpu_open(int device, vdx_gpu_ctx** out) {
         ctx->cu_count = prop.multiProcessorCount;
         ctx->lds_per_block = prop.sharedMemPerBlock;
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        ctx->host_pool = std::make_unique<vdx::HostPool>(
+            std::min<std::size_t>(16, std::max(1u, std::thread::hardware_concurrency())));
         vdx::gpu::noalloc_scratch_attach(device);  // with the device current
         *out = ctx.release();
     });
@@ -817,8 +875,34 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             vdx::check_query(w, {static_cast<vdx::Metric>(h->metric), h->dim, h->n}, queries,
                              query_dim, ks ? ks[0] : k);
         }
-        prepare_queries(w, static_cast<vdx::Metric>(h->metric), h->dim, h->n, queries, nq, query_dim,
-                        ks, k, ws->h_queries.as<float>());
+        const vdx::Metric metric = static_cast<vdx::Metric>(h->metric);
+        // The host threads check and stage the queries; each group of rows crosses PCIe as soon
+        // as it is staged, while the rest are checked, and one search serves the whole batch (a
+        // search takes as long as its slowest queries, so splitting the batch into launches
+        // would pay that more than once). Every query is checked before anything reaches the
+        // caller's buffers, and the first bad row's error is the one reported, as with one pass.
+        const bool early = h->n > 0 && efmax <= kMaxBeamEf;
+        hipStream_t s = ws->stream;
+        if (early) {
+            ws->queries.reserve(nq * h->dim * sizeof(float));
+        }
+        const int device = h->ctx->device;
+        const auto upload = [&](std::uint64_t b, std::uint64_t e) {
+            DeviceGuard g(device);
+            check(hipMemcpyAsync(ws->queries.as<float>() + b * h->dim, ws->h_queries.as<float>() + b * h->dim,
+                                 (e - b) * h->dim * sizeof(float), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(queries)");
+        };
+        try {
+            prepare_queries(*h->ctx, w, metric, h->dim, h->n, queries, nq, query_dim, ks, k,
+                            ws->h_queries.as<float>(),
+                            early ? std::function<void(std::uint64_t, std::uint64_t)>(upload)
+                                  : std::function<void(std::uint64_t, std::uint64_t)>());
+        } catch (...) {
+            // nothing in flight may outlive the call (the workspace goes back to the pool)
+            (void)hipStreamSynchronize(s);
+            throw;
+        }
         if (h->n == 0) {
             std::fill(out_count, out_count + nq, 0u);
             return;
@@ -827,14 +911,11 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             throw std::invalid_argument(vdx::fmt("%s: ef %u is above the GPU path's limit of %u", w,
                                                  efmax, kMaxBeamEf));
         }
-        hipStream_t s = ws->stream;
-        ws->queries.reserve(nq * h->dim * sizeof(float));
         ws->out.reserve(nq * kmax * sizeof(vdx_neighbor));
         ws->count.reserve(nq * sizeof(std::uint32_t));
         ws->counters.reserve(128);  // the queue counter, then the stats at byte 16
-        check(hipMemcpyAsync(ws->queries.as<void>(), ws->h_queries.as<void>(), nq * h->dim * sizeof(float),
-                             hipMemcpyHostToDevice, s),
-              "hipMemcpyAsync(queries)");
+        ws->h_out.reserve(nq * kmax * sizeof(vdx_neighbor));
+        ws->h_count.reserve((nq * 4 + 7) / 8 * 8 + sizeof(unsigned long long) * vdx::dev::kBeamStats);
         const std::uint32_t* d_ks = nullptr;
         const std::uint32_t* d_efs = nullptr;
         if (ks) {
@@ -851,27 +932,36 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         unsigned long long* d_stats = nullptr;
         if (opts && opts->stats) {
             d_stats = reinterpret_cast<unsigned long long*>(ws->counters.as<char>() + 16);
-            check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
+            check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s),
+                  "hipMemsetAsync(stats)");
         }
         enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
                      ws->counters.as<std::uint32_t>(), d_stats, opts, s);
-        ws->h_out.reserve(nq * kmax * sizeof(vdx_neighbor));
         check(hipMemcpyAsync(ws->h_out.as<void>(), ws->out.as<void>(), nq * kmax * sizeof(vdx_neighbor),
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(out)");
-        check(hipMemcpyAsync(out_count, ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
+        check(hipMemcpyAsync(ws->h_count.as<void>(), ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(count)");
-        unsigned long long host_stats[vdx::dev::kBeamStats] = {};
+        // stats after the counts, 8-byte aligned
+        unsigned long long* host_stats =
+            reinterpret_cast<unsigned long long*>(ws->h_count.as<char>() + (nq * 4 + 7) / 8 * 8);
         if (d_stats) {
-            check(hipMemcpyAsync(host_stats, d_stats, sizeof(host_stats), hipMemcpyDeviceToHost, s),
+            check(hipMemcpyAsync(host_stats, d_stats, sizeof(unsigned long long) * vdx::dev::kBeamStats,
+                                 hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(stats)");
         }
         check(hipStreamSynchronize(s), "hnsw search");
         const vdx_neighbor* staged = ws->h_out.as<vdx_neighbor>();
-        for (std::uint64_t q = 0; q < nq; ++q) {
-            std::memcpy(out + q * out_stride, staged + q * kmax, out_count[q] * sizeof(vdx_neighbor));
-        }
+        const std::uint32_t* staged_count = ws->h_count.as<std::uint32_t>();
+        const auto unpack = [&](std::size_t c) {
+            const std::uint64_t end = std::min<std::uint64_t>(nq, (c + 1) * kHostChunk);
+            for (std::uint64_t q = c * kHostChunk; q < end; ++q) {
+                out_count[q] = staged_count[q];
+                std::memcpy(out + q * out_stride, staged + q * kmax, staged_count[q] * sizeof(vdx_neighbor));
+            }
+        };
+        h->ctx->host_pool->run(static_cast<std::size_t>((nq + kHostChunk - 1) / kHostChunk), unpack);
         if (d_stats) {
             for (int i = 0; i < vdx::dev::kBeamStats; ++i) {
                 opts->stats[i] += host_stats[i];
@@ -1206,7 +1296,7 @@ int vdx_gpu_bf_search(vdx_gpu_bf* b, const float* queries, uint64_t nq, uint64_t
         if (query_dim != b->dim) {
             vdx::check_query(w, {static_cast<vdx::Metric>(b->metric), b->dim, b->n}, queries, query_dim, k);
         }
-        prepare_queries(w, static_cast<vdx::Metric>(b->metric), b->dim, b->n, queries, nq, query_dim,
+        prepare_queries(*b->ctx, w, static_cast<vdx::Metric>(b->metric), b->dim, b->n, queries, nq, query_dim,
                         nullptr, k, prepared.data());
         const std::uint64_t keep = std::min<std::uint64_t>(k, b->n);
         require(out_stride >= keep, "vdx_gpu_bf_search: out_stride is below k");

@@ -100,6 +100,7 @@ DimSplit split_dim(std::uint64_t dim) {
 }  // namespace vdx::gpu
 
 vdx_gpu_ctx::~vdx_gpu_ctx() {
+    host_pool.reset();
     free_workspaces.clear();
     for (hipEvent_t& e : events) {
         if (e != nullptr) {

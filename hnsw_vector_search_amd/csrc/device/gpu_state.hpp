@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../host/common.hpp"
+#include "../host/host_pool.hpp"
 
 namespace vdx::gpu {
 
@@ -101,6 +102,8 @@ struct vdx_gpu_ctx {
     hipEvent_t events[16] = {};
     std::mutex pool_mutex;
     std::vector<std::unique_ptr<vdx::gpu::Workspace>> free_workspaces;
+    // host threads for the search entry points' validation, staging and unpacking
+    std::unique_ptr<vdx::HostPool> host_pool;
     ~vdx_gpu_ctx();
 };
 

@@ -4,6 +4,8 @@ The GPU computes every distance in the reference's summation order and reproduce
 order exactly, so results must be identical per query (ids and distance bits) — stricter than
 the >= 99.5% identical / recall-within-0.002 bar."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -300,3 +302,63 @@ def test_upload_applies_check_links_rules():
         g["upper"][off * bu + 1] = low
     status, msg = mutated(below)
     assert status == nat.VDX_INVALID_ARGUMENT and "below the layer" in msg
+
+
+def test_large_host_batch_staged_in_groups(gmm10k):
+    """A host batch of >= 4096 queries is checked and uploaded in groups: per query the same
+    results as the oracle, with per-query k/ef arrays, and the expansions counted once."""
+    print("case: host batch of 5,000 queries (staged groups)", flush=True)
+    index, base, _ = gmm10k
+    queries = datasets.gmm_base_and_queries(datasets.gmm64(), 10, 5000, query_seed=7)[1]
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 100], size=len(queries)).astype(np.uint32)
+    efs = rng.choice([16, 40, 64, 128], size=len(queries)).astype(np.uint32)
+    stats = (ctypes.c_uint64 * 8)()
+    opts = nat.SearchOpts()
+    opts.stats = ctypes.cast(stats, ctypes.c_void_p)
+    got, cnt = index.search_arrays(queries, ks, efs, opts=opts)
+    g = index.graph_arrays()
+    expansions = 0
+    for kk in (1, 10, 100):
+        for e in (16, 40, 64, 128):
+            sel = np.nonzero((ks == kk) & (efs == e))[0]
+            if len(sel) == 0:
+                continue
+            ref, ref_cnt, taken = o.hnsw_search(g, queries[sel], kk, e, max_expansions=0)
+            np.testing.assert_array_equal(cnt[sel], ref_cnt)
+            assert got[sel][:, :kk].tobytes() == ref.tobytes()
+            expansions += int(taken.sum())
+    assert stats[0] == expansions
+
+
+def test_large_host_batch_rejects_a_bad_row_in_any_group(gmm10k):
+    index, _, _ = gmm10k
+    queries = datasets.gmm_base_and_queries(datasets.gmm64(), 10, 6000, query_seed=8)[1]
+    out = np.full((len(queries), 10), 7, dtype=nat.NEIGHBOR_DTYPE)
+    cnt = np.full(len(queries), 5, dtype=np.uint32)
+    for row in (10, 2047, 2048, 4500, 5999):
+        bad = queries.copy()
+        bad[row, 5] = np.inf
+        with pytest.raises(ValueError, match="HnswIndex::search_batch: query must not contain NaN or inf"):
+            nat.check(nat.lib.vdx_gpu_hnsw_search(
+                index.gpu_handle(), nat.ptr(bad), len(bad), 128, None, 10, None, 40, nat.ptr(out), 10,
+                nat.ptr(cnt), b"HnswIndex::search_batch", None))
+        # a rejected call leaves the caller's buffers as they were
+        assert (cnt == 5).all() and (out["id"] == 7).all()
+    # two bad rows in different groups: the first one's error, as one sequential pass reports
+    ks = np.full(len(queries), 10, dtype=np.uint32)
+    for zero_k, inf_row, message in ((4100, 300, "NaN or inf"), (300, 4100, "k must be positive")):
+        bad = queries.copy()
+        bad[inf_row, 0] = np.nan
+        kz = ks.copy()
+        kz[zero_k] = 0
+        with pytest.raises(ValueError, match="HnswIndex::search_batch: .*" + message):
+            nat.check(nat.lib.vdx_gpu_hnsw_search(
+                index.gpu_handle(), nat.ptr(bad), len(bad), 128, nat.ptr(kz), 0, None, 40, nat.ptr(out), 10,
+                nat.ptr(cnt), b"HnswIndex::search_batch", None))
+        assert (cnt == 5).all() and (out["id"] == 7).all()
+    # the search still serves the good batch afterwards
+    got, c = index.search_arrays(queries, 10, 40)
+    ref, ref_cnt = o.hnsw_search(index.graph_arrays(), queries, 10, 40)
+    np.testing.assert_array_equal(c, ref_cnt)
+    assert got.tobytes() == ref.tobytes()
