@@ -79,7 +79,7 @@ class SearchOpts(ctypes.Structure):
     _fields_ = [("visited_log2", c_u32), ("waves_per_group", c_u32), ("termination", c_u32),
                 ("min_steps", c_u32), ("mlp", c_ptr), ("mlp_threshold", c_float),
                 ("reserved_f", c_float), ("stats", c_ptr), ("row_format", c_u32),
-                ("mlp_every", c_u32), ("max_expansions", c_u32)]
+                ("mlp_every", c_u32), ("max_expansions", c_u32), ("max_expansions_per_query", c_ptr)]
 
 
 class BatchClass(ctypes.Structure):

@@ -73,6 +73,7 @@ struct BeamArgs {
     std::uint32_t min_steps;
     std::uint32_t mlp_every;  // consult the model every this many expansions (>= 1)
     std::uint32_t max_exp;  // expansion budget: layer-0 link blocks per query at most (0xFFFFFFFF: none)
+    const std::uint32_t* max_exps;  // per-query budgets replacing max_exp (0: none), or null
 };
 
 constexpr int kBeamStats = 7;

@@ -165,6 +165,7 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
     a.stats = d_stats;
     a.M2 = static_cast<std::uint32_t>(2 * h.M);
     a.max_exp = opts != nullptr && opts->max_expansions != 0 ? opts->max_expansions : 0xFFFFFFFFu;
+    a.max_exps = opts != nullptr ? opts->max_expansions_per_query : nullptr;
     if (mode == vdx::dev::kBeamLearned) {
         const auto* model = static_cast<const vdx_gpu_mlp*>(opts->mlp);
         a.mlp = model->weights.as<void>();

@@ -935,9 +935,21 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s),
                   "hipMemsetAsync(stats)");
         }
+        // per-query budgets arrive in host memory here: the kernels read a device copy
+        vdx_search_opts dopts{};
+        const vdx_search_opts* kopts = opts;
+        if (opts && opts->max_expansions_per_query) {
+            ws->aux3.reserve(nq * 4);
+            check(hipMemcpyAsync(ws->aux3.as<void>(), opts->max_expansions_per_query, nq * 4,
+                                 hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(budgets)");
+            dopts = *opts;
+            dopts.max_expansions_per_query = ws->aux3.as<std::uint32_t>();
+            kopts = &dopts;
+        }
         enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
-                     ws->counters.as<std::uint32_t>(), d_stats, opts, s);
+                     ws->counters.as<std::uint32_t>(), d_stats, kopts, s);
         check(hipMemcpyAsync(ws->h_out.as<void>(), ws->out.as<void>(), nq * kmax * sizeof(vdx_neighbor),
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(out)");

@@ -311,6 +311,12 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = 0xFFFFFFFFu;
         std::uint32_t qexp = 0;
+        // the query's expansion budget (per query when given, 0 there meaning none)
+        std::uint32_t qmax = a.max_exp;
+        if (a.max_exps != nullptr) {
+            qmax = uniform(a.max_exps[q]);
+            qmax = qmax == 0 ? 0xFFFFFFFFu : qmax;
+        }
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
                 truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
             if (p < 0) {
                 break;
             }
-            if (qexp >= a.max_exp) {
+            if (qexp >= qmax) {
                 // the expansion budget ends the search with an open member left
                 if (lane == 0 && a.stats != nullptr) {
                     atomicAdd(a.stats + 4, 1ull);

@@ -310,6 +310,12 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
         std::uint32_t steps = 0, improved_hist = 0, since_improve = 0, countdown = 1;
         std::uint32_t truth_id = kNone;
         std::uint32_t qexp = 0;
+        // the query's expansion budget (per query when given, 0 there meaning none)
+        std::uint32_t qmax = a.max_exp;
+        if (a.max_exps != nullptr) {
+            qmax = uniform(a.max_exps[q]);
+            qmax = qmax == 0 ? 0xFFFFFFFFu : qmax;
+        }
         if constexpr (MODE == kBeamTrace) {
             if (lane < 10) {
                 truth_id = a.truth[static_cast<std::size_t>(q) * 10 + lane];
@@ -323,7 +329,7 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             if (pc < 0) {
                 break;
             }
-            if (qexp >= a.max_exp) {
+            if (qexp >= qmax) {
                 // the expansion budget ends the search with an open member left (counted at once:
                 // no counter kept live through the search)
                 if (lane == 0 && a.stats != nullptr) {

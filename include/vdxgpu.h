@@ -270,6 +270,10 @@ typedef struct vdx_search_opts {
                                   blocks (0: no budget, vecdex's search). A stopped query returns
                                   the closest k found so far; under either termination. No vecdex
                                   counterpart: a serving bound on the slowest queries of a batch */
+    const uint32_t* max_expansions_per_query; /* optional per-query budgets (0: none for that
+                                  query) replacing max_expansions, indexed like the queries: a host
+                                  array for vdx_gpu_hnsw_search, a device array for the device and
+                                  batch entry points. No vecdex counterpart */
 } vdx_search_opts;
 
 enum vdx_stat {

@@ -155,3 +155,41 @@ def test_budget_with_the_learned_stop(gmm):
     opts.max_expansions = 5
     index.search_arrays(queries, 10, 64, opts=opts)
     assert stats[0] == 5 * len(queries) and stats[4] == len(queries)
+
+
+@pytest.mark.parametrize("entry", ["host", "device"])
+def test_per_query_budgets(gmm, entry):
+    """vdx_search_opts.max_expansions_per_query: each query's own budget (0: none), a host array
+    for the host entry point and a device array for the device one; every query equals the
+    oracle's bounded search at its budget."""
+    print(f"case: per-query budgets through the {entry} entry point", flush=True)
+    index, queries = gmm
+    rng = np.random.default_rng(5)
+    budgets = rng.choice([0, 3, 12, 25, 40, 90], size=len(queries)).astype(np.uint32)
+    k, ef = 10, 64
+    stats = (ctypes.c_uint64 * 8)()
+    opts = nat.SearchOpts()
+    opts.stats = ctypes.cast(stats, ctypes.c_void_p)
+    opts.max_expansions = 7  # replaced by the per-query array
+    if entry == "host":
+        opts.max_expansions_per_query = budgets.ctypes.data
+        got, cnt = index.search_arrays(queries, k, ef, opts=opts)
+    else:
+        ctx = nat.Context.get(0)
+        dq = nat.DeviceArray.from_numpy(queries, ctx)
+        db = nat.DeviceArray.from_numpy(budgets, ctx)
+        dout = nat.DeviceArray((len(queries), k), nat.NEIGHBOR_DTYPE, ctx)
+        dcnt = nat.DeviceArray(len(queries), np.uint32, ctx)
+        opts.max_expansions_per_query = db.ptr
+        nat.check(nat.lib.vdx_gpu_hnsw_search_device(index.gpu_handle(), dq.ptr, len(queries), None, k, None, ef,
+                                                     k, dout.ptr, k, dcnt.ptr, ctypes.byref(opts), None))
+        got, cnt = dout.numpy(), dcnt.numpy()
+    g = index.graph_arrays()
+    expansions = 0
+    for b in np.unique(budgets):
+        sel = np.nonzero(budgets == b)[0]
+        ref, ref_cnt, taken = o.hnsw_search(g, queries[sel], k, ef, threads=16, max_expansions=int(b))
+        np.testing.assert_array_equal(cnt[sel], ref_cnt)
+        assert got[sel].tobytes() == ref.tobytes(), f"budget {b}"
+        expansions += int(taken.sum())
+    assert stats[nat.STAT_NAMES.index("expansions")] == expansions
