@@ -895,15 +895,35 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
         re_ = Runner(nat, s["index"], s["eval"])
         # the ceiling, from the evaluation queries' traced searches
         ceiling = []
+        fixed_qps = fixed_eval[name]["qps"] if fixed_eval[name] else None
         for cap in ((36, 48, 64, 128) if name == "l2" else (cap_cos,)):
             hits, steps, final = learned.trace_hits(s["index"], s["eval"], s["truth_eval"]["id"], K, cap,
                                                     max_steps=2 * cap)
-            c = learned.stop_ceiling(hits, steps, final, target, K)
+            c = learned.stop_ceiling(hits, steps, final, target, K, per_query=True)
             c["cap"] = cap
-            ceiling.append(c)
             log(f"learned/{name} ceiling at cap {cap}: fixed {c['fixed_expansions']} expansions (recall "
                 f"{c['fixed_recall']}), oracle stop {c['oracle_stop_expansions']}, budgeted to {target}: "
                 f"{c.get('budgeted_expansions')}")
+            # The ceiling in time: the same searches run with the oracle's per-query stops as
+            # per-query expansion budgets (a stop at 0 expansions runs 1), timed like the others.
+            for key in ("oracle_stops", "budgeted_stops"):
+                stops = c.pop(key, None)
+                if stops is None:
+                    continue
+                db = nat.DeviceArray.from_numpy(np.maximum(stops, 1).astype(np.uint32), re_.ctx)
+                ob = nat.SearchOpts()
+                ob.max_expansions_per_query = db.ptr
+                re_.launch(cap, ob)
+                rec_b = recall(re_.results(), s["truth_eval"])
+                ms_b, _ = re_.timed(cap, max(3, args.steps // 2), 2, ob)
+                t = {"recall": round(rec_b, 4), "ms": round(ms_b, 4), "qps": round(len(s["eval"]) / ms_b * 1000, 1)}
+                if fixed_qps:
+                    t["speedup_vs_fixed_ef"] = round(t["qps"] / fixed_qps, 3)
+                c[key.replace("_stops", "_stop_time")] = t
+                log(f"learned/{name} ceiling at cap {cap}, {key} as per-query budgets: recall {rec_b:.4f}, "
+                    f"{ms_b:.3f} ms, {t['qps']:,.0f} QPS ({t.get('speedup_vs_fixed_ef')}x fixed ef)")
+                del db
+            ceiling.append(c)
         best = None
         # expansion budgets tried with the model: none, and (L2) the budget leg's and a looser one
         budgets = [0]

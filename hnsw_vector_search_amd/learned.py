@@ -135,13 +135,15 @@ def trace_hits(index, queries: np.ndarray, truth_ids: np.ndarray, k: int = 10, e
     return np.concatenate(hits), np.concatenate(all_steps), np.concatenate(all_final)
 
 
-def stop_ceiling(hits: np.ndarray, steps: np.ndarray, final: np.ndarray, target: float, k: int = 10):
+def stop_ceiling(hits: np.ndarray, steps: np.ndarray, final: np.ndarray, target: float, k: int = 10,
+                 per_query: bool = False):
     """What an oracle stop could save on these traced searches: each query stopped at the first
     expansion whose top-k hits equal its final hits (the training label's event), and the fewest
     total expansions over per-query stops whose mean recall@k still reaches `target` — stopping
     a query at the first expansion reaching h hits, h chosen per query, solved exactly as a
     multiple-choice knapsack over the hits given up (dynamic programming). Returns mean
-    expansions per query."""
+    expansions per query; with per_query, also the stops themselves ("oracle_stops",
+    "budgeted_stops": expansions per query, usable as per-query expansion budgets)."""
     nq, cap = hits.shape
     recorded = np.minimum(steps, cap)
     # first[q][h]: the first expansion (1-based) with >= h hits; 0 for h = 0
@@ -154,22 +156,40 @@ def stop_ceiling(hits: np.ndarray, steps: np.ndarray, final: np.ndarray, target:
     out = {"fixed_expansions": round(float(steps.mean()), 2),
            "fixed_recall": round(float(final.mean() / k), 4),
            "oracle_stop_expansions": round(float(at_final.mean()), 2)}
+    if per_query:
+        out["oracle_stops"] = at_final.astype(np.uint32)
     need = int(np.ceil(target * k * nq - 1e-9))
     budget = int(level.sum()) - need  # hits that may be given up
     if budget < 0:
         out["budgeted_expansions"] = None
         return out
-    # best[b]: the most expansions saved giving up at most b hits over the queries so far
+    # best[b]: the most expansions saved giving up at most b hits over the queries so far;
+    # choice[i][b]: the hits the i-th query gives up in that optimum (for the per-query stops)
     best = np.zeros(budget + 1, dtype=np.int64)
-    for q in np.nonzero(level > 0)[0]:
+    active = np.nonzero(level > 0)[0]
+    choice = np.zeros((len(active), budget + 1), dtype=np.int8) if per_query else None
+    for i, q in enumerate(active):
         f = int(level[q])
         nxt = best.copy()
         for d in range(1, min(f, budget) + 1):
             save = int(at_final[q] - first[q, f - d])
-            np.maximum(nxt[d:], best[:budget + 1 - d] + save, out=nxt[d:])
+            cand = best[:budget + 1 - d] + save
+            if per_query:
+                better = cand > nxt[d:]
+                choice[i, d:][better] = d
+            np.maximum(nxt[d:], cand, out=nxt[d:])
         best = nxt
     out["budgeted_expansions"] = round(float((at_final.sum() - best[budget]) / nq), 2)
     out["budgeted_recall"] = round(need / (k * nq), 4)
+    if per_query:
+        stops = at_final.copy()
+        b = budget
+        for i in range(len(active) - 1, -1, -1):
+            d = int(choice[i, b])
+            q = active[i]
+            stops[q] = first[q, int(level[q]) - d]
+            b -= d
+        out["budgeted_stops"] = stops.astype(np.uint32)
     return out
 
 

@@ -44,3 +44,22 @@ def test_budgeted_stop_is_the_exhaustive_optimum():
         got = learned.stop_ceiling(hits, steps, final, target)
         assert got["budgeted_expansions"] == round(exhaustive(hits, steps, final, target), 2), trial
         assert got["budgeted_expansions"] <= got["oracle_stop_expansions"] + 1e-9 or target > final.mean() / 10
+
+
+def test_per_query_stops_realise_the_ceiling():
+    """The per-query stops the DP returns: their mean is the reported ceiling, and the hits each
+    query holds at its stop reach the target in total."""
+    rng = np.random.default_rng(2)
+    for trial in range(60):
+        nq, cap = int(rng.integers(2, 6)), 10
+        hits = np.minimum(np.cumsum(rng.integers(0, 3, (nq, cap)), axis=1), 10).astype(np.uint8)
+        steps = rng.integers(4, cap + 1, nq)
+        final = hits[np.arange(nq), steps - 1].astype(np.int64)
+        target = rng.integers(max(0, final.sum() - 9), final.sum() + 1) / (10 * nq)
+        got = learned.stop_ceiling(hits, steps, final, target, per_query=True)
+        stops = got["budgeted_stops"].astype(np.int64)
+        assert round(stops.mean(), 2) == got["budgeted_expansions"], trial
+        held = [0 if s == 0 else int(hits[q, s - 1]) for q, s in enumerate(stops)]
+        assert sum(held) >= int(np.ceil(target * 10 * nq - 1e-9)), trial
+        assert (got["oracle_stops"].astype(np.int64) <= steps).all()
+        assert round(float(got["oracle_stops"].mean()), 2) == got["oracle_stop_expansions"]
