@@ -616,6 +616,74 @@ def run_10m(args, vdx, nat, datasets, threads):
                         "tuned_graph_qps": round(nq / ms_graph * 1000, 1), "identical_to_one_launch": identical}}
 
 
+def e4m3_per_vector(v):
+    """FP8-e4m3 with a per-vector scale as the GPU quantizes (scale max|v| / 448, round to
+    nearest even, saturating): the dequantized rows."""
+    v = np.asarray(v, dtype=np.float32)
+    scale = np.abs(v).max(axis=1, keepdims=True) / np.float32(448.0)
+    scale[scale == 0] = 1.0
+    y = (v / scale).astype(np.float64)
+    e = np.floor(np.log2(np.maximum(np.abs(y), 2.0 ** -20)))
+    step = 2.0 ** (np.maximum(e, -6.0) - 3.0)  # 3 mantissa bits; subnormal step 2^-9
+    q = np.clip(np.round(y / step) * step, -448.0, 448.0)  # numpy rounds halves to even
+    return (q * scale).astype(np.float32)
+
+
+def fp8_bound_study(base, queries, metric, ks=(1, 10, 100), sample=200):
+    """Why the FP8 shortlist does not certify: over a sample of queries, the certification
+    bound E (the largest |FP8 GEMM distance - exact distance| any row may have, from the
+    quantization errors by Cauchy-Schwarz, the leading terms of the re-rank's test) against the
+    room there is (the (S+1)-th smallest exact distance minus the k-th, S = 8k), and the
+    fractions of queries certified: with the global bound (the smallest FP8 distance outside the
+    S smallest, minus E, above the k-th exact distance) and with a per-row bound (every row
+    outside the S smallest, its FP8 distance minus its own row's bound above the k-th)."""
+    x = np.asarray(base, dtype=np.float32)
+    q = np.ascontiguousarray(queries[:sample], dtype=np.float32)
+    xq, qq = e4m3_per_vector(x), e4m3_per_vector(q)
+    x_err = np.linalg.norm(xq - x, axis=1)
+    x_norm = np.linalg.norm(x, axis=1)
+    q_err = np.linalg.norm(qq - q, axis=1)
+    q_qnorm = np.linalg.norm(qq, axis=1)
+    scale = 2.0 if metric == 0 else 1.0  # L2 distances carry -2 q.x
+    e_global = scale * (q_qnorm * x_err.max() + q_err * x_norm.max())
+    qn2 = np.einsum("ij,ij->i", q, q)
+    margins = {k: [] for k in ks}
+    cert_global = {k: 0 for k in ks}
+    cert_row = {k: 0 for k in ks}
+    for c0 in range(0, len(q), 50):  # distances of 50 queries at a time (matrix products)
+        dots = x @ q[c0:c0 + 50].T
+        adots = xq @ qq[c0:c0 + 50].T
+        for j in range(dots.shape[1]):
+            i = c0 + j
+            if metric == 0:
+                exact = (x_norm ** 2 + qn2[i]) - 2 * dots[:, j]
+                approx = (x_norm ** 2 + qn2[i]) - 2 * adots[:, j]
+            else:
+                exact = 1 - dots[:, j]
+                approx = 1 - adots[:, j]
+            row_bound = scale * (q_qnorm[i] * x_err + q_err[i] * x_norm)
+            for k in ks:
+                S = 8 * k
+                part = np.partition(exact, (k - 1, S))
+                dk, ds1 = float(part[k - 1]), float(part[S])
+                margins[k].append(ds1 - dk)
+                # rows outside the S smallest FP8 distances; L, the smallest of those, must clear
+                # the k-th exact distance by the bound
+                outside = np.argpartition(approx, S)[S:]
+                cert_global[k] += float(approx[outside].min()) - e_global[i] > dk
+                cert_row[k] += bool(((approx[outside] - row_bound[outside]) > dk).all())
+    out = {"sample": len(q), "per_k": {}}
+    for k in ks:
+        m = np.array(margins[k])
+        out["per_k"][f"k{k}"] = {
+            "bound_median": round(float(np.median(e_global)), 4),
+            "margin_median": round(float(np.median(m)), 4),
+            "margin_p90": round(float(np.percentile(m, 90)), 4),
+            "certified_with_global_bound": round(float(cert_global[k]) / len(q), 4),
+            "certified_with_per_row_bounds": round(float(cert_row[k]) / len(q), 4)}
+    return out
+
+
 def bruteforce_leg(args, nat, bf, queries, metric, threads):
     """BruteForceIndex::search on the GPU over all (prepared) queries at k in {1, 10, 100}: the
     exact scan (the reference's result), then the FP32, BF16 and FP8 MFMA shortlists with exact
@@ -685,6 +753,12 @@ def bruteforce_leg(args, nat, bf, queries, metric, threads):
             f"{st.max_err_fp64:.3g}")
         leg[f"k{k}"] = row
         del bout, bcnt
+    t0 = time.perf_counter()
+    leg["fp8_bounds"] = fp8_bound_study(base, queries, metric)
+    log(f"FP8 certification study ({time.perf_counter() - t0:.1f} s): " + "; ".join(
+        f"{kk}: bound {v['bound_median']:.4g} vs margin {v['margin_median']:.4g} (p90 {v['margin_p90']:.4g}), "
+        f"certified {v['certified_with_global_bound']:.0%} global / {v['certified_with_per_row_bounds']:.0%} per row"
+        for kk, v in leg["fp8_bounds"]["per_k"].items()))
     if not args.no_cpu:
         t0 = time.perf_counter()
         host = o.bruteforce_f32(base, np.ascontiguousarray(queries[:sample]), K, metric=metric, threads=threads)

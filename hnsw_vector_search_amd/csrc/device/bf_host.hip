@@ -114,9 +114,9 @@ std::uint32_t next_pow2(std::uint32_t v) {
 
 void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, std::uint32_t k,
              vdx::dev::Neighbor* d_out, std::uint64_t out_stride, std::uint32_t* d_count,
-             Workspace& ws, hipStream_t s, vdx_bf_stats* stats) {
+             Workspace& ws, hipStream_t s, vdx_bf_stats* stats, std::uint32_t widen) {
     const std::uint32_t want = static_cast<std::uint32_t>(std::min<std::uint64_t>(k, b.n));
-    const std::uint32_t mult = prec == dev::kPrecF32 ? 2 : (prec == dev::kPrecBF16 ? 4 : 8);
+    const std::uint32_t mult = (prec == dev::kPrecF32 ? 2 : (prec == dev::kPrecBF16 ? 4 : 8)) * widen;
     std::uint64_t S = std::min<std::uint64_t>(std::max<std::uint64_t>(std::uint64_t{mult} * k, want), b.n);
     if (S > kMaxShortlist) S = std::max<std::uint64_t>(kMaxShortlist, want);
     if (b.dim > kMaxMfmaDim || want > kMaxShortlist) {
@@ -295,26 +295,34 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         }
         if (!redo.empty()) {
             const std::uint64_t nr = redo.size();
-            DeviceBuffer& idx = ws.extra[16];
-            DeviceBuffer& sub_q = ws.extra[17];
-            DeviceBuffer& sub_out = ws.extra[18];
-            DeviceBuffer& sub_cnt = ws.extra[19];
-            idx.reserve(nr * 4);
-            sub_q.reserve(nr * b.dim * 4);
-            sub_out.reserve(nr * want * sizeof(dev::Neighbor));
-            sub_cnt.reserve(nr * 4);
+            // own buffers: the wider pass below reuses the workspace's
+            DeviceBuffer idx(nr * 4), sub_q(nr * b.dim * 4), sub_out(nr * want * sizeof(dev::Neighbor)),
+                sub_cnt(nr * 4);
             check(hipMemcpyAsync(idx.as<void>(), redo.data(), nr * 4, hipMemcpyHostToDevice, s), "copy");
             check(dev::launch_bf_gather_rows(cq_src, idx.as<std::uint32_t>(), static_cast<std::uint32_t>(nr),
                                              static_cast<std::uint32_t>(b.dim), sub_q.as<float>(), s),
                   "gather launch");
-            bf_exact(b, sub_q.as<float>(), nr, k, sub_out.as<dev::Neighbor>(), want,
-                     sub_cnt.as<std::uint32_t>(), ws, s);
+            if (widen == 1 && S < std::min<std::uint64_t>(kMaxShortlist, b.n)) {
+                // a few uncertified queries: an 8x wider shortlist usually certifies them, at a
+                // fraction of an exact scan (which a small batch leaves mostly idle)
+                vdx_bf_stats inner{};
+                bf_mfma(b, prec, sub_q.as<float>(), nr, k, sub_out.as<dev::Neighbor>(), want,
+                        sub_cnt.as<std::uint32_t>(), ws, s, &inner, 8);
+                if (stats) {
+                    stats->widened += nr;
+                    stats->exact_fallback += inner.exact_fallback;
+                    stats->max_abs_error = std::max(stats->max_abs_error, inner.max_abs_error);
+                }
+            } else {
+                bf_exact(b, sub_q.as<float>(), nr, k, sub_out.as<dev::Neighbor>(), want,
+                         sub_cnt.as<std::uint32_t>(), ws, s);
+                fallbacks += nr;
+            }
             check(dev::launch_bf_scatter_results(sub_out.as<dev::Neighbor>(), sub_cnt.as<std::uint32_t>(),
                                                  idx.as<std::uint32_t>(), static_cast<std::uint32_t>(nr),
                                                  want, d_out + c0 * out_stride, out_stride, d_count + c0, s),
                   "scatter launch");
             check(hipStreamSynchronize(s), "bf fallback");
-            fallbacks += nr;
         }
     }
     if (stats) {

@@ -181,3 +181,23 @@ def test_modes_equal_the_oracle_on_the_config_generators(precision, workload):
         got, cnt = index.search_arrays(queries, k, precision)
         assert (cnt == k).all()
         assert got.tobytes() == ref.tobytes(), (precision, workload, k)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_uncertified_queries_are_widened_then_exact(precision):
+    """Queries the first shortlist does not certify are redone with an 8x wider one, and only
+    the ones that still fail go to the exact scan; every result equals the exact scan's."""
+    print(f"case: widened shortlist, {precision}, 60,000 x 100 cosine", flush=True)
+    gen = datasets.AnisotropicGaussian(100, 0.8, seed=11)
+    base, queries = gen.sample(60_000, 1), gen.sample(400, 2)
+    index = vdx.BruteForceIndex(100, 2)
+    index.add(base)
+    for k in (1, 10):
+        exact, _ = index.search_arrays(queries, k, "exact")
+        got, _ = index.search_arrays(queries, k, precision)
+        st = index.last_stats
+        print(f"  k {k}: widened {st.widened}, exact fallbacks {st.exact_fallback}", flush=True)
+        assert got.tobytes() == exact.tobytes(), (precision, k)
+        assert st.exact_fallback <= st.widened <= len(queries)
+        if precision == "fp8":
+            assert st.widened > 0  # FP8's bound rarely certifies the first shortlist here
