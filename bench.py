@@ -514,7 +514,8 @@ def run_serving(args, vdx, nat, datasets, threads):
         f"({nq / ms_graph * 1000:,.0f} QPS); identical results: {identical}; tuning {tune_s:.1f} s")
     for c in classes:
         log(f"  class cap {c['ef_cap']}: {c['queries']} queries, visited 2^{c['visited_log2']}, "
-            f"{c['waves_per_group']} waves/group, blocks/CU {c['blocks_per_cu'] or 'max'}, {c['ms']:.2f} ms")
+            f"{c['waves_per_group']} waves/group, blocks/CU {c['blocks_per_cu'] or 'max'}, "
+            f"{'HBM bitmap' if c.get('visited_form') == 2 else 'LDS cache'}, {c['ms']:.2f} ms")
     sizes = []
     for b in sorted({*(2 ** i for i in range(17) if 2 ** i < nq), nq}):
         r = 20 if b <= 1024 else 5

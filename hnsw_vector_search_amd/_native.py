@@ -79,12 +79,13 @@ class SearchOpts(ctypes.Structure):
     _fields_ = [("visited_log2", c_u32), ("waves_per_group", c_u32), ("termination", c_u32),
                 ("min_steps", c_u32), ("mlp", c_ptr), ("mlp_threshold", c_float),
                 ("reserved_f", c_float), ("stats", c_ptr), ("row_format", c_u32),
-                ("mlp_every", c_u32), ("max_expansions", c_u32), ("max_expansions_per_query", c_ptr)]
+                ("mlp_every", c_u32), ("max_expansions", c_u32), ("max_expansions_per_query", c_ptr),
+                ("visited_form", c_u32)]
 
 
 class BatchClass(ctypes.Structure):
     _fields_ = [("ef_cap", c_u32), ("queries", c_u32), ("visited_log2", c_u32),
-                ("waves_per_group", c_u32), ("blocks_per_cu", c_u32), ("ms", c_float)]
+                ("waves_per_group", c_u32), ("blocks_per_cu", c_u32), ("ms", c_float), ("visited_form", c_u32)]
 
 
 class BfStats(ctypes.Structure):

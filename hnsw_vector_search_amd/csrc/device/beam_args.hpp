@@ -74,6 +74,10 @@ struct BeamArgs {
     std::uint32_t mlp_every;  // consult the model every this many expansions (>= 1)
     std::uint32_t max_exp;  // expansion budget: layer-0 link blocks per query at most (0xFFFFFFFF: none)
     const std::uint32_t* max_exps;  // per-query budgets replacing max_exp (0: none), or null
+    // the visited set as an exact bitmap in HBM instead of the LDS cache (the LDS kernel): one
+    // row of vbit_words words per resident wave, cleared by the wave at the start of each query
+    std::uint32_t* vbits;
+    std::uint32_t vbit_words;
 };
 
 constexpr int kBeamStats = 7;
@@ -88,9 +92,10 @@ enum BeamMode : int { kBeamPlain = 0, kBeamTrace = 1, kBeamLearned = 2,
 enum class BeamShape { D128, D100, Generic, D128U8 };
 
 // Bytes of LDS one wave uses for a given R capacity and visited-cache size.
-inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2) {
+// (bitmap: the visited set lives in HBM, BeamArgs::vbits)
+inline std::size_t beam_lds_per_wave(std::uint32_t ef_cap, std::uint32_t visited_log2, bool bitmap = false) {
     return 8u * static_cast<std::size_t>(ef_cap) + 8u * 64 + 4u * 64 + 4u * 64 + 4u * 16 +
-           (std::size_t{4} << visited_log2);
+           (bitmap ? 0 : (std::size_t{4} << visited_log2));
 }
 
 // The register-resident beam (hnsw_beam_reg.hpp) holds R in registers in chunks of 64 keys;

@@ -188,6 +188,7 @@ struct vdx_gpu_batch {
     vdx::gpu::BeamConfig cfg[vdx::dev::kMaxClasses];
     vdx_batch_class info[vdx::dev::kMaxClasses] = {};
     DeviceBuffer order, ranges, queues;
+    DeviceBuffer bits[vdx::dev::kMaxClasses];  // visited bitmaps of classes tuned to visited_form 2
     hipStream_t side[vdx::dev::kMaxClasses] = {};
     hipEvent_t fork = nullptr, join[vdx::dev::kMaxClasses] = {};
     // the captured launch sequence and the call it was captured for
@@ -242,6 +243,7 @@ struct ClassBuffers {
     std::uint32_t* ranges;
     std::uint32_t* queues;
     std::uint32_t* qflag;  // the byte-query flag word the integer kernel reads
+    DeviceBuffer* bits = nullptr;  // per class, the visited bitmaps (reserved before enqueueing)
 };
 
 // The R-capacity classes of batches with k <= max_k and ef <= max_ef: 64, 128, 256, 512 (the
@@ -303,12 +305,28 @@ void enqueue_class_into(vdx_gpu_hnsw& h, int mode, const vdx_search_opts* opts, 
         a.qflag = buf.qflag;
         a.qflag_epoch = 1u;
     }
+    if (plan.bitmap_bytes > 0) {
+        require(buf.bits != nullptr, "visited_form 2 (HBM bitmap) is not available on this path");
+        require(buf.bits[cls].bytes() >= plan.bitmap_bytes, "batch visited bitmap not reserved");
+        a.vbits = buf.bits[cls].as<std::uint32_t>();
+        a.vbit_words = plan.bitmap_words;
+    }
     vdx::gpu::launch_beam(h, plan, mode, a, s);
+}
+
+// Grows the visited bitmaps the classes' plans need (before anything is enqueued or captured).
+void reserve_bitmaps(vdx_gpu_batch& b, std::uint64_t nq, const vdx::gpu::BeamConfig* cfgs) {
+    for (int cls = 0; cls < b.ncls; ++cls) {
+        const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(*b.index, b.caps[cls], nq, cfgs[cls], b.mode);
+        if (plan.bitmap_bytes > 0) {
+            b.bits[cls].reserve(plan.bitmap_bytes);
+        }
+    }
 }
 
 ClassBuffers buffers_of(vdx_gpu_batch& b) {
     return {b.order.as<std::uint32_t>(), b.ranges.as<std::uint32_t>(), b.queues.as<std::uint32_t>(),
-            b.queues.as<std::uint32_t>() + vdx::dev::kMaxClasses};
+            b.queues.as<std::uint32_t>() + vdx::dev::kMaxClasses, b.bits};
 }
 
 void enqueue_classify(vdx_gpu_batch& b, const BatchCall& c, hipStream_t s) {
@@ -389,6 +407,19 @@ int vdx_gpu_hnsw_batch_run(vdx_gpu_batch* b, const float* d_queries, uint64_t nq
         check_call(*b, c);
         DeviceGuard guard(b->index->ctx->device);
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : b->index->ctx->stream;
+        {
+            // a bitmap that grows changes its address: a captured sequence holding the old one goes
+            void* before[vdx::dev::kMaxClasses];
+            for (int cls = 0; cls < b->ncls; ++cls) before[cls] = b->bits[cls].as<void>();
+            check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            reserve_bitmaps(*b, nq, b->cfg);
+            for (int cls = 0; cls < b->ncls; ++cls) {
+                if (before[cls] != b->bits[cls].as<void>() && b->exec != nullptr) {
+                    check(hipGraphExecDestroy(b->exec), "hipGraphExecDestroy");
+                    b->exec = nullptr;
+                }
+            }
+        }
         if (!use_graph) {
             enqueue_all(*b, c, s);
             return;
@@ -445,6 +476,10 @@ int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* b, const float* d_queries, uint64_t n
         check(hipEventCreate(&t0), "hipEventCreate");
         check(hipEventCreate(&t1), "hipEventCreate");
         const auto time_class = [&](int cls, const vdx::gpu::BeamConfig& cfg) {
+            vdx::gpu::BeamConfig cfgs[vdx::dev::kMaxClasses];
+            for (int i = 0; i < b->ncls; ++i) cfgs[i] = b->cfg[i];
+            cfgs[cls] = cfg;
+            reserve_bitmaps(*b, c.nq, cfgs);
             float best = 0.f;
             for (std::uint32_t r = 0; r < reps; ++r) {
                 check(vdx::dev::launch_fill_u32(b->queues.as<std::uint32_t>() + cls, 0u, 1, s), "fill");
@@ -473,12 +508,22 @@ int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* b, const float* d_queries, uint64_t n
                     vl2s = {9, 10, 11};
                 }
                 wavess = fixed.waves ? std::vector<std::uint32_t>{fixed.waves} : std::vector<std::uint32_t>{2, 4};
+                // the visited form: the LDS cache at each size, and (the LDS kernel's classes,
+                // R capacity above 512) the exact bitmap in HBM, which frees that LDS for waves
+                std::vector<std::pair<std::uint32_t, std::uint32_t>> forms;  // (form, visited_log2)
+                if (fixed.visited_form != 2) {
+                    for (const std::uint32_t v : vl2s) forms.push_back({1u, v});
+                }
+                if (b->caps[cls] > vdx::dev::kRegMaxEf && fixed.visited_form != 1) {
+                    forms.push_back({2u, 10u});
+                }
                 float best_ms = -1.f;
-                for (const std::uint32_t v : vl2s) {
+                for (const auto& [form, v] : forms) {
                     for (const std::uint32_t w : wavess) {
                         for (const std::uint32_t bpc : bpcs) {
                             vdx::gpu::BeamConfig cfg;
                             cfg.visited_log2 = v;
+                            cfg.visited_form = form;
                             cfg.waves = w;
                             cfg.blocks_per_cu = bpc;
                             const float ms = time_class(cls, cfg);
@@ -503,6 +548,7 @@ int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* b, const float* d_queries, uint64_t n
             b->info[cls].visited_log2 = plan.visited_log2;
             b->info[cls].waves_per_group = plan.waves;
             b->info[cls].blocks_per_cu = b->cfg[cls].blocks_per_cu;
+            b->info[cls].visited_form = plan.bitmap_bytes > 0 ? 2u : 1u;
         }
         // a captured sequence holds the old shapes
         if (b->exec != nullptr) {
@@ -524,6 +570,7 @@ int vdx_gpu_hnsw_batch_classes(const vdx_gpu_batch* b, vdx_batch_class* out, uin
                 info.visited_log2 = plan.visited_log2;
                 info.waves_per_group = plan.waves;
                 info.blocks_per_cu = b->cfg[c].blocks_per_cu;
+                info.visited_form = plan.bitmap_bytes > 0 ? 2u : 1u;
             }
             out[c] = info;
         }
@@ -554,6 +601,9 @@ int vdx_gpu_hnsw_search_uneven_noalloc(vdx_gpu_hnsw* index, const float* d_queri
         require(out_stride >= max_k, "vdx_gpu_hnsw_search_uneven_noalloc: out_stride is below max_k");
         const int mode = vdx::gpu::beam_mode(opts, false);
         const vdx::gpu::BeamConfig cfg = vdx::gpu::config_from(opts);
+        require(cfg.visited_form != 2,
+                "vdx_gpu_hnsw_search_uneven_noalloc: visited_form 2 (HBM bitmap) needs device memory this entry "
+                "point does not allocate");
         std::uint32_t caps[vdx::dev::kMaxClasses] = {};
         const int ncls = class_caps(max_k, max_ef, caps);
         DeviceGuard guard(index->ctx->device);

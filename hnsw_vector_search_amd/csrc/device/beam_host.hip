@@ -54,6 +54,8 @@ BeamConfig config_from(const vdx_search_opts* opts) {
         c.waves = opts->waves_per_group;
         vdx::require(opts->row_format <= 1, "row_format must be 0 (8-bit rows where kept) or 1 (FP32 rows)");
         c.f32_rows = opts->row_format == 1;
+        vdx::require(opts->visited_form <= 2, "visited_form must be 0 (auto), 1 (LDS cache) or 2 (HBM bitmap)");
+        c.visited_form = opts->visited_form;
     }
     return c;
 }
@@ -82,8 +84,9 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     vdx::require(p.visited_log2 >= 6 && p.visited_log2 <= 15, "visited_log2 must be within [6, 15]");
     p.waves = cfg.waves ? cfg.waves : 4;
     vdx::require(p.waves >= 1 && p.waves <= 4, "waves_per_group must be within [1, 4]");
+    const bool bitmap = p.chunks == 0 && cfg.visited_form == 2;
     p.lds_per_wave = p.chunks > 0 ? vdx::dev::beam_reg_lds_per_wave(p.ef_cap, p.visited_log2, mode != vdx::dev::kBeamPlain)
-                                  : vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2);
+                                  : vdx::dev::beam_lds_per_wave(p.ef_cap, p.visited_log2, bitmap);
     const std::size_t limit = h.ctx->lds_per_block;
     const std::size_t shared = vdx::dev::beam_lds_shared(mode);
     while (p.waves > 1 && p.lds_per_wave * p.waves + shared > limit) {
@@ -103,6 +106,13 @@ BeamPlan plan_beam(const vdx_gpu_hnsw& h, std::uint32_t max_efp, std::uint64_t n
     const std::uint64_t resident = static_cast<std::uint64_t>(per_cu) * h.ctx->cu_count;
     const std::uint64_t needed = (nq + p.waves - 1) / p.waves;
     p.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, std::min(resident, needed)));
+    p.bitmap_words = 0;
+    p.bitmap_bytes = 0;
+    if (bitmap) {
+        // rows of whole 16-byte stores (the clear), one per resident wave
+        p.bitmap_words = static_cast<std::uint32_t>((h.n + 127) / 128 * 4);
+        p.bitmap_bytes = static_cast<std::uint64_t>(p.grid) * p.waves * p.bitmap_words * 4;
+    }
     p.grid_int = 0;
     if (p.shape == vdx::dev::BeamShape::D128U8) {
         int per_cu_int = 0;

@@ -19,6 +19,7 @@ struct BeamConfig {
     std::uint32_t waves = 0;          // wavefronts per workgroup (1..4)
     std::uint32_t blocks_per_cu = 0;  // resident workgroups per CU to fill (queries in flight)
     bool f32_rows = false;            // never the 8-bit row copy
+    std::uint32_t visited_form = 0;   // 0: the LDS cache; 2: the HBM bitmap (the LDS kernel only)
 };
 
 struct BeamPlan {
@@ -27,6 +28,10 @@ struct BeamPlan {
     int chunks;  // > 0: the register-resident kernel with R in `chunks` x 64 keys; 0: LDS kernel
     std::size_t lds_per_wave, lds_bytes;
     vdx::dev::BeamShape shape;  // D128U8: the plain register kernel over the 8-bit row copy
+    // the visited set as an HBM bitmap: words per wave and the bytes the launch needs (grid x
+    // waves rows), the caller's buffer; 0 with the LDS cache
+    std::uint32_t bitmap_words;
+    std::uint64_t bitmap_bytes;
 };
 
 vdx::dev::BeamShape beam_shape(std::uint64_t dim);

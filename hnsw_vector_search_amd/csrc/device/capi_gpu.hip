@@ -123,11 +123,17 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
                   const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr,
-                  std::uint32_t layer = 0) {
+                  std::uint32_t layer = 0, DeviceBuffer* bitmap = nullptr) {
     const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
     const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
     vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
                                                out_stride, d_count, d_queue, d_stats);
+    if (plan.bitmap_bytes > 0) {
+        require(bitmap != nullptr, "visited_form 2 (HBM bitmap) is not available on this path");
+        bitmap->reserve(plan.bitmap_bytes);
+        a.vbits = bitmap->as<std::uint32_t>();
+        a.vbit_words = plan.bitmap_words;
+    }
     if (trace != nullptr) {
         a.truth = trace->truth;
         a.trace_feat = trace->trace_feat;
@@ -949,7 +955,7 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         }
         enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
-                     ws->counters.as<std::uint32_t>(), d_stats, kopts, s);
+                     ws->counters.as<std::uint32_t>(), d_stats, kopts, s, nullptr, nullptr, 0, &ws->bitmap);
         check(hipMemcpyAsync(ws->h_out.as<void>(), ws->out.as<void>(), nq * kmax * sizeof(vdx_neighbor),
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync(out)");
@@ -1023,9 +1029,22 @@ int vdx_gpu_hnsw_search_device(vdx_gpu_hnsw* h, const float* d_queries, uint64_t
         if (want_stats) {
             check(hipMemsetAsync(d_stats, 0, vdx::dev::kBeamStats * sizeof(unsigned long long), s), "hipMemsetAsync(stats)");
         }
-        enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
-                     reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
-                     want_stats ? d_stats : nullptr, opts, s);
+        if (opts != nullptr && opts->visited_form == 2) {
+            const std::lock_guard<std::mutex> bitmap_lock(h->bitmap_mutex);
+            if (h->bitmap_done == nullptr) {
+                check(hipEventCreateWithFlags(&h->bitmap_done, hipEventDisableTiming), "hipEventCreate(bitmap)");
+            } else {
+                check(hipEventSynchronize(h->bitmap_done), "hipEventSynchronize(bitmap)");
+            }
+            enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
+                         reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
+                         want_stats ? d_stats : nullptr, opts, s, nullptr, nullptr, 0, &h->bitmap);
+            check(hipEventRecord(h->bitmap_done, s), "hipEventRecord(bitmap)");
+        } else {
+            enqueue_beam(*h, d_queries, nq, d_ks, k, d_efs, ef, efmax,
+                         reinterpret_cast<vdx::dev::Neighbor*>(d_out), out_stride, d_out_count, counter,
+                         want_stats ? d_stats : nullptr, opts, s);
+        }
         check(hipEventRecord(slot_done, s), "hipEventRecord(ring slot)");
         if (want_stats) {
             unsigned long long host_stats[vdx::dev::kBeamStats];

@@ -42,11 +42,20 @@ struct vdx_gpu_hnsw {
     std::atomic<std::uint32_t> next_slot{0};
     std::mutex ring_mutex[kRingSlots];
     hipEvent_t ring_event[kRingSlots] = {};
+    // the device entry point's visited bitmaps (visited_form 2): one buffer, so launches that
+    // use it follow one another (each waits for the previous one's event)
+    std::mutex bitmap_mutex;
+    vdx::gpu::DeviceBuffer bitmap;
+    hipEvent_t bitmap_done = nullptr;
 
     vdx_gpu_hnsw() = default;
     vdx_gpu_hnsw(const vdx_gpu_hnsw&) = delete;
     vdx_gpu_hnsw& operator=(const vdx_gpu_hnsw&) = delete;
     ~vdx_gpu_hnsw() {
+        if (bitmap_done != nullptr) {
+            (void)hipEventSynchronize(bitmap_done);
+            (void)hipEventDestroy(bitmap_done);
+        }
         for (hipEvent_t& e : ring_event) {
             if (e != nullptr) {
                 (void)hipEventSynchronize(e);

@@ -87,6 +87,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     DeviceBuffer queries, ks, efs, out, count, counters, aux0, aux1, aux2, aux3;
     DeviceBuffer extra[20];  // the MFMA brute force's per-call buffers
+    DeviceBuffer bitmap;     // the HNSW search's visited bitmaps (visited_form 2)
     DeviceBuffer fp64[4];    // the brute force's float64 pass: window, counts, distances, counters
     PinnedBuffer h_queries, h_out, h_count;
     ~Workspace();

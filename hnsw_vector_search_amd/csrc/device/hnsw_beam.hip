@@ -33,6 +33,8 @@ namespace vdx::dev {
 
 namespace {
 
+constexpr std::uint32_t kNoNode = 0xFFFFFFFFu;
+
 constexpr std::uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -176,13 +178,16 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
     }
     const std::uint32_t vslots = 1u << a.visited_log2;
     const std::uint32_t vshift = 32u - a.visited_log2;
+    std::uint32_t* bits = a.vbits == nullptr
+                              ? nullptr
+                              : a.vbits + static_cast<std::size_t>(blockIdx.x * (blockDim.x >> 6) + wave) * a.vbit_words;
     const int nb = static_cast<int>(a.nb);
     const int n8 = static_cast<int>(a.n8);
     const int tail = static_cast<int>(a.tail);
     const int first_tail = static_cast<int>(a.first_tail);
     const RowSource src{a.vectors, a.ld, a.l2};
 
-    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0;
+    unsigned long long n_exp = 0, n_dist = 0, n_up = 0, n_dup = 0, n_stop = 0, n_pref = 0;
 
     std::uint32_t begin = 0, end = a.nq;
     if (a.range != nullptr) {
@@ -293,17 +298,38 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         }
 
         // Layer 0 (src/hnsw.cpp:438-482) with ef' = max(ef, k) (src/hnsw.cpp:517).
-        for (std::uint32_t i = static_cast<std::uint32_t>(lane) * 4; i < vslots; i += kWave * 4) {
-            *reinterpret_cast<uint4*>(vis + i) = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        if (bits != nullptr) {
+            // the exact visited set (vecdex's VisitedList): this wave's bitmap row, cleared, the
+            // stores complete before the test-and-set atomics of the search touch it
+            for (std::uint32_t i = static_cast<std::uint32_t>(lane) * 4; i < a.vbit_words; i += kWave * 4) {
+                *reinterpret_cast<uint4*>(bits + i) = make_uint4(0u, 0u, 0u, 0u);
+            }
+            __threadfence();
+            if (lane == 0) {
+                atomicOr(bits + (cur >> 5), 1u << (cur & 31u));
+            }
+        } else {
+            for (std::uint32_t i = static_cast<std::uint32_t>(lane) * 4; i < vslots; i += kWave * 4) {
+                *reinterpret_cast<uint4*>(vis + i) = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+            }
         }
         wave_lds_fence();
         if (lane == 0) {
-            vis[(cur * 2654435761u) >> vshift] = cur;
+            if (bits == nullptr) {
+                vis[(cur * 2654435761u) >> vshift] = cur;
+            }
             Rl[0] = make_key(cur_d, cur);
         }
         wave_lds_fence();
         std::uint32_t size = 1;
         std::uint32_t hint = 0;
+        std::uint32_t pref_node = kNoNode, pref_cnt = 0, pref_id = 0;
+        const std::uint32_t lcap = a.layer == 0 ? a.M2 : (a.M2 >> 1);
+        const auto block_of = [&](std::uint32_t nd) -> const std::uint32_t* {
+            return a.layer == 0
+                       ? a.layer0 + static_cast<std::size_t>(nd) * a.b0
+                       : a.upper + (static_cast<std::size_t>(a.upper_offset[nd]) + (a.layer - 1)) * a.bu;
+        };
         // search-state features (trace and learned modes)
         const float inv_scale = 1.0f / fmaxf(fabsf(cur_d), 1e-6f);
         const float inv_efp = 1.0f / static_cast<float>(efp);
@@ -354,19 +380,31 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
             hint = static_cast<std::uint32_t>(p) + 1;
             ++n_exp;
 
-            // the searched layer's block (layer 0 for queries; an insertion may search an upper one)
-            const std::uint32_t* blk =
-                a.layer == 0 ? a.layer0 + static_cast<std::size_t>(node) * a.b0
-                             : a.upper + (static_cast<std::size_t>(a.upper_offset[node]) + (a.layer - 1)) * a.bu;
-            const std::uint32_t cnt = uniform(blk[0]);
+            // the searched layer's block (layer 0 for queries; an insertion may search an upper
+            // one): prefetched during the previous expansion when it named this node
+            std::uint32_t cnt, id;
+            if (node == pref_node) {
+                cnt = uniform(pref_cnt);
+                id = pref_id;
+                ++n_pref;
+            } else {
+                const std::uint32_t* blk = block_of(node);
+                cnt = uniform(blk[0]);
+                id = static_cast<std::uint32_t>(lane) < cnt ? blk[1 + lane] : 0u;
+            }
             const bool valid = static_cast<std::uint32_t>(lane) < cnt;
-            const std::uint32_t id = valid ? blk[1 + lane] : 0;
             bool fresh = false;
             if (valid) {
-                const std::uint32_t slot = (id * 2654435761u) >> vshift;
-                fresh = vis[slot] != id;
-                if (fresh) {
-                    vis[slot] = id;
+                if (bits != nullptr) {
+                    // test-and-set: one atomic per neighbour, the ids of a block being distinct
+                    const std::uint32_t bit = 1u << (id & 31u);
+                    fresh = (atomicOr(bits + (id >> 5), bit) & bit) == 0;
+                } else {
+                    const std::uint32_t slot = (id * 2654435761u) >> vshift;
+                    fresh = vis[slot] != id;
+                    if (fresh) {
+                        vis[slot] = id;
+                    }
                 }
             }
             const std::uint64_t fb = ballot(fresh);
@@ -375,6 +413,26 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
                 fid[lanes_below(fb)] = id;
             }
             wave_lds_fence();
+            // Prefetch the block of the next open member of R, the next expansion unless a fresh
+            // key lands ahead of it: its latency passes under this expansion's rows and merge.
+            // (Node `node`'s own block again when none is open: the load is unconditional.)
+            {
+                std::int32_t p2 = -1;
+                for (std::uint32_t b = hint; b < size; b += kWave) {
+                    const std::uint32_t i = b + lane;
+                    const bool open = i < size && (static_cast<std::uint32_t>(Rl[i]) & kExpanded) == 0;
+                    const std::uint64_t bal = ballot(open);
+                    if (bal != 0) {
+                        p2 = static_cast<std::int32_t>(b) + __ffsll(static_cast<long long>(bal)) - 1;
+                        break;
+                    }
+                }
+                const std::uint32_t next = p2 >= 0 ? uniform(key_id(Rl[p2])) : node;
+                pref_node = p2 >= 0 ? next : kNoNode;
+                const std::uint32_t* nb2 = block_of(next);
+                pref_cnt = nb2[0];
+                pref_id = static_cast<std::uint32_t>(lane) < lcap ? nb2[1 + lane] : 0u;
+            }
             n_dist += nf;
             for (std::uint32_t f0 = 0; f0 < nf; f0 += 8 * R) {
                 const std::uint32_t fc = min(8u * R, nf - f0);
@@ -463,6 +521,7 @@ __global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
         atomicAdd(a.stats + 2, n_up);
         atomicAdd(a.stats + 3, n_dup);
         atomicAdd(a.stats + 4, n_stop);
+        atomicAdd(a.stats + 5, n_pref);
     }
 }
 

@@ -274,6 +274,11 @@ typedef struct vdx_search_opts {
                                   query) replacing max_expansions, indexed like the queries: a host
                                   array for vdx_gpu_hnsw_search, a device array for the device and
                                   batch entry points. No vecdex counterpart */
+    uint32_t visited_form;     /* the visited set of searches with max(ef, k) above 512: 0 or 1 the
+                                  LDS cache (lossy, exact duplicate removal in the merge), 2 an
+                                  exact bitmap in HBM per resident wave (n bits, cleared per query;
+                                  frees the LDS for more waves). Results are the same either way;
+                                  vdx_gpu_hnsw_search_uneven_noalloc takes 0 or 1 only */
 } vdx_search_opts;
 
 enum vdx_stat {
@@ -360,6 +365,7 @@ typedef struct vdx_batch_class {
     uint32_t waves_per_group;
     uint32_t blocks_per_cu;   /* resident workgroups per CU the grid fills (0: as many as fit) */
     float ms;                 /* the tuned launch time on the sample (0 before tuning) */
+    uint32_t visited_form;    /* 1: the LDS visited cache; 2: the HBM bitmap (R capacity above 512) */
 } vdx_batch_class;
 /* Device buffers for batches of up to max_nq queries with k <= max_k and ef <= max_ef; opts
  * selects termination (a learned model must outlive the batch) and, when its visited_log2 or
@@ -367,8 +373,9 @@ typedef struct vdx_batch_class {
 VDX_API int vdx_gpu_hnsw_batch_create(vdx_gpu_hnsw* index, uint64_t max_nq, uint32_t max_k, uint32_t max_ef,
                                       const vdx_search_opts* opts, vdx_gpu_batch** out);
 VDX_API void vdx_gpu_hnsw_batch_free(vdx_gpu_batch* batch);
-/* Autotuner: times each class's launch on a sample batch over visited-cache sizes, waves per
- * workgroup and resident workgroups per CU, keeping the fastest shape per class. Synchronous. */
+/* Autotuner: times each class's launch on a sample batch over visited-set forms (LDS cache
+ * sizes; for R capacities above 512 also the HBM bitmap), waves per workgroup and resident
+ * workgroups per CU, keeping the fastest shape per class. Synchronous. */
 VDX_API int vdx_gpu_hnsw_batch_tune(vdx_gpu_batch* batch, const float* d_queries, uint64_t nq,
                                     const uint32_t* d_ks, uint32_t k, const uint32_t* d_efs, uint32_t ef,
                                     vdx_neighbor* d_out, uint64_t out_stride, uint32_t* d_out_count,
