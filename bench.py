@@ -135,6 +135,23 @@ class Runner:
         return [int(stats[i]) for i in range(5)]
 
 
+def timed_best_form(run, ef, steps, warmup, opts=None):
+    """run.timed with the faster visited-set form where there is a choice (R capacity above 512,
+    the LDS beam: the LDS cache or the HBM bitmap; the same results either way). Returns
+    (ms, form), form 0 where the default is the only one."""
+    if ef <= 512:
+        ms, _ = run.timed(ef, steps, warmup, opts)
+        return ms, 0
+    best = None
+    for form in (1, 2):
+        o = run.nat.SearchOpts.from_buffer_copy(opts) if opts is not None else run.nat.SearchOpts()
+        o.visited_form = form
+        ms, _ = run.timed(ef, steps, warmup, o)
+        if best is None or ms < best[0]:
+            best = (ms, form)
+    return best
+
+
 def recall(found, truth):
     """Mean recall@10 by id (vecdex src/recall.cpp:11-34, through the library)."""
     from hnsw_vector_search_amd import recall as r
@@ -920,9 +937,9 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
         r_e = recall(re_.results(), s["truth_eval"])
         rc.launch(ef)
         r_c = recall(rc.results(), s["truth_calib"])
-        ms, _ = re_.timed(ef, max(3, args.steps // 2), 2)
+        ms, form = timed_best_form(re_, ef, max(3, args.steps // 2), 2)
         sweep_cos.append({"ef": ef, "recall": round(r_e, 4), "calib_recall": round(r_c, 4), "ms": round(ms, 4),
-                          "qps": round(len(s["eval"]) / ms * 1000, 1)})
+                          "qps": round(len(s["eval"]) / ms * 1000, 1), "visited_form": form})
         log(f"cos fixed ef {ef}: recall {r_e:.4f} (held-out {r_c:.4f}), {sweep_cos[-1]['qps']:,.0f} QPS")
         if fixed_cos is None and r_c >= TARGET_RECALL:
             fixed_cos = sweep_cos[-1]
@@ -990,7 +1007,7 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
                 ob.max_expansions_per_query = db.ptr
                 re_.launch(cap, ob)
                 rec_b = recall(re_.results(), s["truth_eval"])
-                ms_b, _ = re_.timed(cap, max(3, args.steps // 2), 2, ob)
+                ms_b, _ = timed_best_form(re_, cap, max(3, args.steps // 2), 2, ob)
                 t = {"recall": round(rec_b, 4), "ms": round(ms_b, 4), "qps": round(len(s["eval"]) / ms_b * 1000, 1)}
                 if fixed_qps:
                     t["speedup_vs_fixed_ef"] = round(t["qps"] / fixed_qps, 3)
@@ -1016,7 +1033,8 @@ def run_learned(args, vdx, nat, datasets, learned, index, base, queries, truth, 
                             f"reaches {target + CALIB_MARGIN:.4f} (best {rec_c:.4f})")
                         continue
                     opts = learned.learned_opts(model, thr, 1, every=every, max_expansions=budget)
-                    ms_c, _ = rc.timed(ef_cap, max(3, args.steps // 2), 2, opts)
+                    ms_c, form = timed_best_form(rc, ef_cap, max(3, args.steps // 2), 2, opts)
+                    opts.visited_form = form
                     log(f"learned/{name} ef cap {ef_cap}, every {every}, budget {budget}: threshold {thr}, "
                         f"calib recall {rec_c:.4f}, {work:.1f} expansions, "
                         f"{len(s['calib']) / ms_c * 1000:,.0f} QPS on calibration")

@@ -153,7 +153,7 @@ __device__ __forceinline__ void merge(std::uint64_t* R, std::uint32_t& size, std
 }  // namespace
 
 template <int NB, int N8, int TAIL, int R, int MODE>
-__global__ __launch_bounds__(256) void hnsw_beam_kernel(BeamArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void hnsw_beam_kernel(BeamArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
