@@ -717,6 +717,13 @@ def bruteforce_leg(args, nat, bf, queries, metric, threads):
     leg = {}
     flops = 2.0 * nq * n * d
     sample = 200
+    # the BF16 GEMM shape by measurement (one or two 32-query blocks per wave): the tuner keeps the
+    # faster for the timings below
+    t = bf.tune(queries, 10, reps=3)
+    leg["bf16_gemm_shapes"] = {"ms_one_block_per_wave": round(t["ms"][0], 3),
+                               "ms_two_blocks_per_wave": round(t["ms"][1], 3), "chosen": t["shape"]}
+    log(f"BF16 GEMM shapes (k=10): one query block per wave {t['ms'][0]:.3f} ms, two {t['ms'][1]:.3f} ms; "
+        f"kept shape {t['shape']}")
     for k in (1, 10, 100):
         bout = nat.DeviceArray((nq, k), nat.NEIGHBOR_DTYPE, ctx)
         bcnt = nat.DeviceArray(nq, np.uint32, ctx)

@@ -171,6 +171,8 @@ _SIGS = {
     "vdx_gpu_bf_free": (None, [c_ptr]),
     "vdx_gpu_bf_search": (c_int, [c_ptr, c_ptr, c_u64, c_u64, c_u32, c_u32, c_ptr, c_u64,
                                   c_ptr, c_char_p, ctypes.POINTER(BfStats)]),
+    "vdx_gpu_bf_gemm_shape": (c_int, [c_ptr, c_u32]),
+    "vdx_gpu_bf_tune": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr, c_ptr]),
     "vdx_gpu_bf_search_device": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr, c_u64,
                                          c_ptr, ctypes.POINTER(BfStats)]),
 }

@@ -733,6 +733,225 @@ __global__ __launch_bounds__(512, 4) void bf_gemm_bf16_dma_kernel(GemmArgs a) {
     }
 }
 
+// The same filter with two query blocks per wave: each wave holds 64 queries (two 32-query
+// register operands) and feeds every base fragment it reads from LDS to two MFMAs, so a tile
+// costs half the LDS reads per MFMA of bf_gemm_bf16_dma_kernel (whose LDS reads about match
+// the MFMA rate). 4-wave workgroups of 256 queries, three per CU (168 registers per lane: 64 of
+// query fragments, 64 of accumulators). The tile layout, loads and epilogue are the single-block
+// kernel's; a wave loads 16 of a tile's 64 rows.
+#define VDX_DMA2_EPILOGUE(CHECK_ROWS)                                                             \
+        _Pragma("unroll") for (int b = 0; b < QB; ++b) {                                          \
+        _Pragma("unroll") for (int h = 0; h < NH; ++h) {                                          \
+            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                       \
+                const int r0 = h * 32 + 8 * g + 4 * half;                                         \
+                if constexpr (MODE == 0) {                                                        \
+                    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                               \
+                        const float acc_e = acc[b][h][4 * g + e];                                 \
+                        const float d = L2 ? __builtin_fmaf(-2.0f, acc_e, qn[b]) : 1.0f - acc_e;  \
+                        const std::uint32_t j = tile_t * NT + r0 + e;                             \
+                        if (qok[b] && (!(CHECK_ROWS) || j < a.rows)) {                            \
+                            a.dense[static_cast<std::size_t>(qv[b]) * a.dense_ld + j] = d;        \
+                        }                                                                         \
+                    }                                                                             \
+                } else {                                                                          \
+                    std::uint64_t m4[4];                                                          \
+                    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                               \
+                        m4[e] = __builtin_amdgcn_fcmpf(acc[b][h][4 * g + e], astar[b], 3 /* ordered >= */); \
+                        if (CHECK_ROWS) {                                                         \
+                            m4[e] &= ballot(tile_t * NT + r0 + e < a.rows);                       \
+                        }                                                                         \
+                    }                                                                             \
+                    if ((m4[0] | m4[1] | m4[2] | m4[3]) != 0) {                                   \
+                        _Pragma("unroll") for (int e = 0; e < 4; ++e) {                           \
+                            const std::uint64_t m = m4[e];                                        \
+                            if (m != 0) {                                                         \
+                                const std::uint32_t np = static_cast<std::uint32_t>(__popcll(m)); \
+                                if (staged + np > 64) {                                           \
+                                    flush();                                                      \
+                                }                                                                 \
+                                if ((m >> lane) & 1u) {                                           \
+                                    const float acc_e = acc[b][h][4 * g + e];                     \
+                                    const float d = L2 ? __builtin_fmaf(-2.0f, acc_e, qn[b]) : 1.0f - acc_e; \
+                                    const std::uint32_t at = staged + lanes_below(m);             \
+                                    skey[at] = make_key(d, tile_t * NT + r0 + e);                 \
+                                    squery[at] = qv[b];                                           \
+                                }                                                                 \
+                                staged += np;                                                     \
+                            }                                                                     \
+                        }                                                                         \
+                    }                                                                             \
+                }                                                                                 \
+            }                                                                                     \
+        }                                                                                         \
+        }
+
+template <int MODE, bool L2>
+__global__ __launch_bounds__(256, 2) void bf_gemm_bf16_dma2_kernel(GemmArgs a) {
+    constexpr int NT = 64;
+    constexpr int NH = 2;
+    constexpr int KS = 8;
+    constexpr int QB = 2;
+    constexpr std::uint32_t kRowBytes = 256;
+    constexpr std::uint32_t kTileBytes = NT * kRowBytes;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 31;
+    const int half = lane >> 5;
+    float* const vals = reinterpret_cast<float*>(lds + 3 * kTileBytes);
+
+    v8bf bq[QB][KS];
+    std::uint32_t qv[QB];
+    bool qok[QB];
+    float qn[QB], astar[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+        qv[b] = blockIdx.x * 256 + wave * 64 + b * 32 + col;  // this lane's query of block b
+        const unsigned char* qbase = static_cast<const unsigned char*>(a.qq) + static_cast<std::size_t>(qv[b]) * kRowBytes;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            bq[b][s] = *reinterpret_cast<const v8bf*>(qbase + (16 * s + 8 * half) * 2);
+        }
+        qok[b] = qv[b] < a.nq;
+        qn[b] = qok[b] ? a.qn[qv[b]] : 0.f;
+        const float tau = !qok[b] ? -__int_as_float(0x7f800000) : (a.tau ? a.tau[qv[b]] : __int_as_float(0x7f800000));
+        astar[b] = MODE == 0 ? 0.f : filter_accept_bound(tau, qn[b], L2);
+    }
+
+    const std::uint32_t tiles = (a.rows + NT - 1) / NT;
+    const std::uint32_t per_slab = (tiles + gridDim.y - 1) / gridDim.y;
+    const std::uint32_t t0 = blockIdx.y * per_slab;
+    const std::uint32_t t1 = min(tiles, t0 + per_slab);
+    if (t0 >= t1) {
+        return;
+    }
+    const std::uint32_t last = t1 - 1;
+    // Wave w's four load instructions fill LDS bytes [4w KB, 4w KB + 4 KB) of a buffer: rows
+    // 16w + 4i + lane / 16 (i = 0..3), slot lane % 16, holding chunk slot ^ (row % 16).
+    const std::uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);  // m0 must be uniform
+    const unsigned char* const xq_bytes = static_cast<const unsigned char*>(a.xq);
+    std::uint32_t drow[4], dcol[4];
+    const unsigned char* src[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        drow[i] = 16 * wave + 4 * i + (lane >> 4);
+        dcol[i] = ((lane & 15) ^ (drow[i] & 15)) * 16;
+        src[i] = xq_bytes + static_cast<std::size_t>(drow[i] * a.row_stride) * kRowBytes + dcol[i];
+    }
+    const float* const xh_src = a.xh + static_cast<std::size_t>(lane) * a.row_stride;
+    const std::uint32_t lds_base = lds_offset(lds);
+    const auto issue = [&](std::uint32_t tile, std::uint32_t b) {
+        const std::uint32_t dst = lds_base + b * kTileBytes + wave_u * 4096;
+        const std::size_t first = static_cast<std::size_t>(tile) * NT * a.row_stride;
+        if ((tile + 1) * NT <= a.rows) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                lds_load16(src[i] + first * kRowBytes, dst + 1024 * i);
+            }
+            if (L2 && wave_u == 0) {
+                lds_load4(xh_src + first, lds_base + 3 * kTileBytes + b * NT * 4);
+            }
+        } else {
+            // a tile past the end of the rows repeats the last row (its distances are never used)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const std::uint32_t j = tile * NT + drow[i];
+                const std::uint32_t x = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+                lds_load16(xq_bytes + static_cast<std::size_t>(x) * kRowBytes + dcol[i], dst + 1024 * i);
+            }
+            if (L2 && wave_u == 0) {
+                const std::uint32_t j = tile * NT + lane;
+                const std::uint32_t xr = (j < a.rows ? j : a.rows - 1) * a.row_stride;
+                lds_load4(a.xh + xr, lds_base + 3 * kTileBytes + b * NT * 4);
+            }
+        }
+    };
+    // Loads complete in issue order: leaving the newest tile's loads (5 on wave 0 under L2, 4
+    // otherwise) in flight means every older one has landed.
+    const auto wait_older = [&]() {
+        if (L2 && wave_u == 0) {
+            __builtin_amdgcn_s_waitcnt(0x0F75);  // vmcnt(5)
+        } else {
+            __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
+        }
+    };
+
+    std::uint64_t* skey = reinterpret_cast<std::uint64_t*>(lds + 3 * kTileBytes + 3 * NT * 4 + wave * kStageBytes);
+    std::uint32_t* squery = reinterpret_cast<std::uint32_t*>(skey + 64);
+    std::uint32_t staged = 0;  // wave-uniform
+    const auto flush = [&]() {
+        wave_lds_fence();
+        if (static_cast<std::uint32_t>(lane) < staged) {
+            const std::uint32_t sq = squery[lane];
+            const std::uint32_t slot = atomicAdd(a.cnt + sq, 1u);
+            if (slot < a.cap) {
+                a.buf[static_cast<std::size_t>(sq) * a.cap + slot] = skey[lane];
+            }
+        }
+        wave_lds_fence();
+        staged = 0;
+    };
+
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): query fragments, |q|^2, tau landed
+    issue(t0, 0);
+    issue(min(t0 + 1, last), 1);
+    wait_older();
+    __syncthreads();
+    const std::uint32_t a0 = static_cast<std::uint32_t>(col) * kRowBytes +
+                             16u * (static_cast<std::uint32_t>(half) ^ (static_cast<std::uint32_t>(col) & 15u));
+    std::uint32_t tile_cb = 0;  // buffer of tile t
+    for (std::uint32_t t = t0; t < t1; ++t) {
+        const std::uint32_t tile_t = t;
+        issue(min(t + 2, last), tile_cb == 0 ? 2 : tile_cb - 1);
+        const std::uint32_t ab = a0 + tile_cb * kTileBytes;
+        v16f acc[QB][NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            if constexpr (L2) {
+#pragma unroll
+                for (int b = 0; b < QB; ++b) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const float4 c = *reinterpret_cast<const float4*>(vals + tile_cb * NT + h * 32 + 8 * g + 4 * half);
+                        acc[b][h][4 * g + 0] = c.x;
+                        acc[b][h][4 * g + 1] = c.y;
+                        acc[b][h][4 * g + 2] = c.z;
+                        acc[b][h][4 * g + 3] = c.w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < QB; ++b) {
+                    acc[b][h] = v16f{};
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const unsigned char* p = lds + (ab ^ (32u * s));
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const v8bf x = *reinterpret_cast<const v8bf*>(p + h * 32 * kRowBytes);
+#pragma unroll
+                for (int b = 0; b < QB; ++b) {
+                    acc[b][h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[b][s], acc[b][h], 0, 0, 0);
+                }
+            }
+        }
+        if ((tile_t + 1) * NT <= a.rows) {
+            VDX_DMA2_EPILOGUE(false)
+        } else {
+            VDX_DMA2_EPILOGUE(true)
+        }
+        wait_older();
+        __syncthreads();
+        tile_cb = tile_cb == 2 ? 0 : tile_cb + 1;
+    }
+    if (MODE != 0 && staged > 0) {
+        flush();
+    }
+}
+
 #undef VDX_DMA_EPILOGUE
 #undef VDX_EPILOGUE
 
@@ -1085,6 +1304,7 @@ hipError_t launch_bf_prepare(const PrepArgs& args, hipStream_t stream) {
 }
 
 std::size_t bf_gemm_dma_lds_bytes() { return 3 * 64 * 256 + 3 * 64 * sizeof(float) + 8 * kStageBytes; }
+std::size_t bf_gemm_dma2_lds_bytes() { return 3 * 64 * 256 + 3 * 64 * sizeof(float) + 4 * kStageBytes; }
 
 std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
     const std::size_t eb = prec == kPrecF32 ? 4 : (prec == kPrecBF16 ? 2 : 1);
@@ -1093,7 +1313,7 @@ std::size_t bf_gemm_lds_bytes(int prec, std::uint32_t kp, int waves) {
 }
 
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
-                          hipStream_t stream) {
+                          hipStream_t stream, int shape) {
     if (waves != 8) return hipErrorInvalidValue;
     if (args.mode != 0 && args.row_stride != 1) return hipErrorInvalidValue;  // the filter's ids are rows
     const dim3 grid(qtiles, slabs);
@@ -1119,7 +1339,21 @@ hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs,
     } else {                                                                                      \
         VDX_GEMM_MODES(P, false)                                                                  \
     }
-    if (prec == kPrecBF16 && args.ks == bf_gemm_steps(kPrecBF16) && args.kp == 128) {
+    if (prec == kPrecBF16 && args.ks == bf_gemm_steps(kPrecBF16) && args.kp == 128 && shape == 1) {
+        const std::size_t lds2 = bf_gemm_dma2_lds_bytes();
+        const dim3 block2(256);
+        if (args.mode == 0) {
+            if (args.l2) {
+                hipLaunchKernelGGL((bf_gemm_bf16_dma2_kernel<0, true>), grid, block2, lds2, stream, args);
+            } else {
+                hipLaunchKernelGGL((bf_gemm_bf16_dma2_kernel<0, false>), grid, block2, lds2, stream, args);
+            }
+        } else if (args.l2) {
+            hipLaunchKernelGGL((bf_gemm_bf16_dma2_kernel<1, true>), grid, block2, lds2, stream, args);
+        } else {
+            hipLaunchKernelGGL((bf_gemm_bf16_dma2_kernel<1, false>), grid, block2, lds2, stream, args);
+        }
+    } else if (prec == kPrecBF16 && args.ks == bf_gemm_steps(kPrecBF16) && args.kp == 128) {
         const std::size_t dma_lds = bf_gemm_dma_lds_bytes();
         if (args.mode == 0) {
             if (args.l2) {

@@ -97,9 +97,11 @@ hipError_t launch_bf_scatter_results(const Neighbor* part, const std::uint32_t* 
                                      const std::uint32_t* idx, std::uint32_t count,
                                      std::uint32_t width, Neighbor* out, std::uint64_t out_stride,
                                      std::uint32_t* out_count, hipStream_t stream);
-// One workgroup of `waves` (4 or 8) wavefronts per 32 * waves queries and slab of base rows.
+// One workgroup per 256 queries (waves = 8) and slab of base rows. shape (BF16 with 128-element
+// operand rows): 0, eight waves of one 32-query block each (bf_gemm_bf16_dma_kernel); 1, four
+// waves of two blocks each (bf_gemm_bf16_dma2_kernel). Other precisions and widths ignore it.
 hipError_t launch_bf_gemm(const GemmArgs& args, unsigned qtiles, unsigned slabs, int prec, int waves,
-                          hipStream_t stream);
+                          hipStream_t stream, int shape = 0);
 hipError_t launch_bf_threshold(const ThresholdArgs& args, unsigned nq, hipStream_t stream);
 hipError_t launch_bf_shortlist(const ShortlistArgs& args, unsigned nq, hipStream_t stream);
 hipError_t launch_bf_rerank(const RerankArgs& args, unsigned nq, std::size_t lds_per_wave,

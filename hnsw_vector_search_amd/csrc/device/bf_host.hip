@@ -208,7 +208,8 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
             g.mode = 0;
             g.dense = dense.as<float>();
             g.dense_ld = m;
-            check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (m + 63) / 64), prec, kGemmWaves, s), "bf_gemm(sample) launch");
+            check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (m + 63) / 64), prec, kGemmWaves, s, b.gemm_shape.load()),
+                  "bf_gemm(sample) launch");
             dev::ThresholdArgs t{};
             t.dense = dense.as<float>();
             t.dense_ld = m;
@@ -226,7 +227,8 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         g.buf = buf.as<std::uint64_t>();
         g.cap = cap;
         g.cnt = cnt.as<std::uint32_t>();
-        check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (b.n + 63) / 64), prec, kGemmWaves, s), "bf_gemm(filter) launch");
+        check(dev::launch_bf_gemm(g, qtiles, slabs_for(qtiles, (b.n + 63) / 64), prec, kGemmWaves, s, b.gemm_shape.load()),
+              "bf_gemm(filter) launch");
 
         list.reserve(cq * S * 4);
         approx.reserve(cq * S * 4);

@@ -1211,6 +1211,62 @@ void bf_exact(vdx_gpu_bf& b, const float* d_queries, std::uint64_t nq, std::uint
 
 extern "C" {
 
+int vdx_gpu_bf_gemm_shape(vdx_gpu_bf* b, uint32_t shape) {
+    return guarded([&] {
+        require(b != nullptr, "vdx_gpu_bf_gemm_shape: index is null");
+        require(shape <= 1, "vdx_gpu_bf_gemm_shape: shape must be 0 or 1");
+        b->gemm_shape.store(static_cast<int>(shape));
+    });
+}
+
+int vdx_gpu_bf_tune(vdx_gpu_bf* b, const float* d_queries, uint64_t nq, uint32_t k, uint32_t reps,
+                    float* ms, uint32_t* shape) {
+    return guarded([&] {
+        require(b != nullptr, "vdx_gpu_bf_tune: index is null");
+        require(k >= 1 && nq >= 1, "vdx_gpu_bf_tune: empty sample");
+        require(k <= kMaxBfK || b->n <= kMaxBfK, "vdx_gpu_bf_tune: k above the GPU path's limit");
+        DeviceGuard guard(b->ctx->device);
+        WorkspaceLease ws(*b->ctx);
+        hipStream_t s = ws->stream;
+        const std::uint64_t keep = std::min<std::uint64_t>(k, b->n);
+        DeviceBuffer out(nq * std::max<std::uint64_t>(keep, 1) * sizeof(vdx_neighbor)), count(nq * 4);
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        check(hipEventCreate(&t0), "hipEventCreate");
+        check(hipEventCreate(&t1), "hipEventCreate");
+        float best[2] = {0.f, 0.f};
+        const int before = b->gemm_shape.load();
+        try {
+            for (int sh = 0; sh < 2; ++sh) {
+                b->gemm_shape.store(sh);
+                for (std::uint32_t r = 0; r <= std::max(1u, reps); ++r) {  // the first run prepares operands
+                    check(hipEventRecord(t0, s), "hipEventRecord");
+                    vdx::gpu::bf_mfma(*b, VDX_PREC_BF16, d_queries, nq, k, out.as<vdx::dev::Neighbor>(),
+                                      std::max<std::uint64_t>(keep, 1), count.as<std::uint32_t>(), *ws, s, nullptr);
+                    check(hipEventRecord(t1, s), "hipEventRecord");
+                    check(hipEventSynchronize(t1), "hipEventSynchronize");
+                    float t = 0.f;
+                    check(hipEventElapsedTime(&t, t0, t1), "hipEventElapsedTime");
+                    if (r == 1 || (r > 1 && t < best[sh])) best[sh] = t;
+                }
+            }
+        } catch (...) {
+            b->gemm_shape.store(before);
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            throw;
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        const int pick = best[1] < best[0] ? 1 : 0;
+        b->gemm_shape.store(pick);
+        if (ms) {
+            ms[0] = best[0];
+            ms[1] = best[1];
+        }
+        if (shape) *shape = static_cast<std::uint32_t>(pick);
+    });
+}
+
 int vdx_gpu_bf_search_device(vdx_gpu_bf* b, const float* d_queries, uint64_t nq, uint32_t k,
                              uint32_t precision, vdx_neighbor* d_out, uint64_t out_stride,
                              uint32_t* d_out_count, vdx_bf_stats* stats) {

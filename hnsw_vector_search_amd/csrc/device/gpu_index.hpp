@@ -87,6 +87,8 @@ struct vdx_gpu_bf {
     vdx::gpu::DeviceBuffer vectors;
     std::mutex prep_mutex;
     vdx::gpu::BfOperands prep[3];
+    // the BF16 filter's GEMM shape (launch_bf_gemm's `shape`), set by vdx_gpu_bf_tune
+    std::atomic<int> gemm_shape{0};
 };
 
 struct vdx_gpu_mlp {

@@ -532,6 +532,22 @@ class BruteForceIndex:
             nat.lib.vdx_gpu_bf_free(self._gpu)
             self._gpu = None
 
+    def set_gemm_shape(self, shape: int) -> None:
+        """The BF16 filter GEMM's shape for 97-128-dimensional rows (vdx_gpu_bf_gemm_shape): 0,
+        one 32-query block per wave; 1, two. Results are the same either way."""
+        nat.check(nat.lib.vdx_gpu_bf_gemm_shape(self.gpu_handle(), int(shape)))
+
+    def tune(self, queries, k: int = 10, reps: int = 3) -> dict:
+        """Times the BF16 search in both GEMM shapes on these queries (prepared: normalized under
+        cosine) and keeps the faster (vdx_gpu_bf_tune)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        dq = nat.DeviceArray.from_numpy(q, nat.Context.get(self._device))
+        ms = (ctypes.c_float * 2)()
+        shape = ctypes.c_uint32(0)
+        nat.check(nat.lib.vdx_gpu_bf_tune(self.gpu_handle(), dq.ptr, q.shape[0], int(k), int(reps), ms,
+                                          ctypes.byref(shape)))
+        return {"ms": [float(ms[0]), float(ms[1])], "shape": int(shape.value)}
+
     def search_arrays(self, queries, k: int, precision: str = "bf16",
                       where: str = "BruteForceIndex::search", fp64: str | None = None):
         """fp64: None; "error" also measures the returned distances against float64 on the GPU

@@ -464,6 +464,16 @@ VDX_API int vdx_gpu_bf_search(vdx_gpu_bf* index, const float* queries, uint64_t 
                       uint32_t k, uint32_t precision, vdx_neighbor* out, uint64_t out_stride,
                       uint32_t* out_count, const char* where, vdx_bf_stats* stats);
 
+/* The BF16 filter GEMM's shape for 97-128-dimensional bases (no vecdex counterpart; results are
+ * the same either way): 0, eight-wave workgroups of one 32-query block per wave (the default);
+ * 1, four-wave workgroups of two 32-query blocks per wave (each base fragment read from LDS feeds
+ * two MFMAs). vdx_gpu_bf_tune times both on a sample of prepared device queries (the whole BF16
+ * search at k, best of `reps` after one warm-up each), keeps the faster, and reports the times
+ * (ms[2], may be NULL) and the choice. */
+VDX_API int vdx_gpu_bf_gemm_shape(vdx_gpu_bf* index, uint32_t shape);
+VDX_API int vdx_gpu_bf_tune(vdx_gpu_bf* index, const float* d_queries, uint64_t nq, uint32_t k, uint32_t reps,
+                            float* ms, uint32_t* shape);
+
 /* Device-memory variant: prepared d_queries, results to d_out (rows of out_stride), on the
  * context's stream; returns after the results are complete. */
 VDX_API int vdx_gpu_bf_search_device(vdx_gpu_bf* index, const float* d_queries, uint64_t nq, uint32_t k,

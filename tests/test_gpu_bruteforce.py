@@ -201,3 +201,29 @@ def test_uncertified_queries_are_widened_then_exact(precision):
         assert st.exact_fallback <= st.widened <= len(queries)
         if precision == "fp8":
             assert st.widened > 0  # FP8's bound rarely certifies the first shortlist here
+
+
+@pytest.mark.parametrize("metric,dim", [(0, 128), (2, 100), (1, 128)])
+def test_two_query_blocks_per_wave_gemm_equals_exact(metric, dim):
+    """The BF16 filter's second GEMM shape (two 32-query blocks per wave, 4-wave workgroups)
+    returns the exact scan's bytes; the tuner keeps either shape and still does."""
+    print(f"case: BF16 GEMM shape 1, metric {metric}, dim {dim}", flush=True)
+    if metric == 0:
+        base, queries = datasets.gmm_base_and_queries(datasets.gmm1024(), 60_000, 700)
+    else:
+        gen = datasets.AnisotropicGaussian(dim, 0.8, seed=dim + 1)
+        base, queries = gen.sample(60_000, 1), gen.sample(700, 2)
+    index = vdx.BruteForceIndex(dim, metric)
+    index.add(base)
+    for k in (1, 10, 100):
+        exact, _ = index.search_arrays(queries, k, "exact")
+        for shape in (1, 0):
+            index.set_gemm_shape(shape)
+            got, _ = index.search_arrays(queries, k, "bf16")
+            assert got.tobytes() == exact.tobytes(), (metric, k, shape)
+    prepared = o.normalize(queries) if metric == 2 else queries
+    t = index.tune(prepared, 10, reps=1)
+    assert t["shape"] in (0, 1) and min(t["ms"]) > 0
+    got, _ = index.search_arrays(queries, 10, "bf16")
+    exact, _ = index.search_arrays(queries, 10, "exact")
+    assert got.tobytes() == exact.tobytes()
