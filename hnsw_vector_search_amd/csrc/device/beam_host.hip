@@ -188,7 +188,7 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
 }
 
 void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vdx::dev::BeamArgs& a,
-                 hipStream_t stream) {
+                 hipStream_t stream, bool int_only) {
     if (plan.chunks > 0) {
         if (plan.shape == vdx::dev::BeamShape::D128U8 && a.qflag != nullptr) {
             // the integer kernel and the float kernel, in turn: the flag lets exactly one work
@@ -196,6 +196,9 @@ void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vd
                       a, plan.grid_int, plan.waves, plan.lds_bytes, stream, plan.shape,
                       mode == vdx::dev::kBeamLearned ? vdx::dev::kBeamU8IntLearned : vdx::dev::kBeamU8Int, plan.chunks),
                   "hnsw_beam_reg (integer) launch");
+            if (int_only) {
+                return;
+            }
         }
         check(vdx::dev::launch_hnsw_beam_reg(a, plan.grid, plan.waves, plan.lds_bytes, stream,
                                              plan.shape, mode, plan.chunks),

@@ -51,8 +51,11 @@ vdx::dev::BeamArgs beam_args(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mo
                              std::uint32_t ef, vdx::dev::Neighbor* d_out, std::uint64_t out_stride,
                              std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats);
 
+// Enqueues the plan's kernels. Over the 8-bit rows with a.qflag set, the integer kernel and the
+// float kernel both go (the flag lets one work), or the integer kernel alone with `int_only`
+// (the caller knows every query is byte-valued and the flag says so).
 void launch_beam(const vdx_gpu_hnsw& h, const BeamPlan& plan, int mode, const vdx::dev::BeamArgs& a,
-                 hipStream_t stream);
+                 hipStream_t stream, bool int_only = false);
 
 BeamConfig config_from(const vdx_search_opts* opts);
 

@@ -9,7 +9,6 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
-#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -123,7 +122,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
                   const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr,
-                  std::uint32_t layer = 0, DeviceBuffer* bitmap = nullptr) {
+                  std::uint32_t layer = 0, DeviceBuffer* bitmap = nullptr, int host_bytes = 0) {
     const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
     const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
     vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
@@ -144,7 +143,18 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     }
     a.entries = d_entries;
     a.layer = layer;
-    if (plan.shape == vdx::dev::BeamShape::D128U8) {
+    bool int_only = false;
+    if (plan.shape == vdx::dev::BeamShape::D128U8 && host_bytes != 0) {
+        // the host checked the queries while staging them (host_bytes 1: every component an
+        // integer in [0, 255]): the counter and the flag word after it are cleared, and the
+        // integer kernel alone goes with epoch 1 (the flag 0 differs), or the float kernel alone
+        check(vdx::dev::launch_fill_u32(d_queue, 0u, 2, stream), "fill(queue)");
+        if (host_bytes == 1) {
+            a.qflag = d_queue + 1;
+            a.qflag_epoch = 1;
+            int_only = true;
+        }
+    } else if (plan.shape == vdx::dev::BeamShape::D128U8) {
         // the byte-query check also clears the work counter; its flag is the word after the
         // counter (every caller's counter word has a free word after it), stamped with this
         // launch's epoch, so it needs no clearing
@@ -157,7 +167,7 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
     } else {
         check(hipMemsetAsync(d_queue, 0, sizeof(std::uint32_t), stream), "hipMemsetAsync(queue)");
     }
-    vdx::gpu::launch_beam(h, plan, mode, a, stream);
+    vdx::gpu::launch_beam(h, plan, mode, a, stream, int_only);
 }
 
 // The link rules of HnswIndex::check_links (src/hnsw_io.cpp:472-504) for the blocks of one node:
@@ -452,50 +462,60 @@ int build_search(void* ctx, const std::uint32_t* nodes, const std::uint32_t* ent
     });
 }
 
-// Validates and prepares nq host queries into pinned staging (rows of dim floats).
-// Queries per item of the host pool's validation and unpacking, and per group of the host
-// entry point's staged upload.
+// Queries per item of the host pool's validation and staging, and per item of its unpacking.
 constexpr std::uint64_t kHostChunk = 128;
-constexpr std::uint64_t kHostGroup = 2048;
+constexpr std::uint64_t kUnpackChunk = 1024;
 
 // Checks every query (vecdex's check_query, its messages; the first bad row's error, as a
 // sequential loop reports) and writes the prepared rows (normalized under cosine) to `staging`,
-// in chunks on the context's host threads. `on_group(b, e)`, when given, is called once rows
-// [b, e) of a group of kHostGroup rows are all staged (from whichever thread staged the last of
-// them; not for a group holding a bad row).
+// in chunks on the context's host threads. `nonbyte`, when given, is set to 1 unless every
+// component staged is an integer in [0, 255] (the 8-bit rows' integer kernel).
 void prepare_queries(vdx_gpu_ctx& ctx, const char* where, vdx::Metric metric, std::uint64_t dim,
                      std::uint64_t size, const float* queries, std::uint64_t nq, std::uint64_t query_dim,
                      const std::uint32_t* ks, std::uint32_t k, float* staging,
-                     const std::function<void(std::uint64_t, std::uint64_t)>& on_group = {}) {
+                     std::atomic<std::uint32_t>* nonbyte = nullptr) {
     const vdx::IndexShape shape{metric, static_cast<std::size_t>(dim), static_cast<std::size_t>(size)};
     const std::string w(where);
-    constexpr std::uint64_t kPerGroup = kHostGroup / kHostChunk;
     const std::size_t chunks = static_cast<std::size_t>((nq + kHostChunk - 1) / kHostChunk);
-    const std::size_t groups = (chunks + kPerGroup - 1) / kPerGroup;
-    std::unique_ptr<std::atomic<std::uint32_t>[]> left(new std::atomic<std::uint32_t>[groups]);
-    for (std::size_t g = 0; g < groups; ++g) {
-        left[g].store(static_cast<std::uint32_t>(std::min<std::uint64_t>(kPerGroup, chunks - g * kPerGroup)));
-    }
     const auto stage = [&](std::uint64_t begin, std::uint64_t end) {
         if (metric != vdx::Metric::Cosine && query_dim == dim) {
             // Squared L2 and inner product stage the rows as they are: one pass copies them and
             // looks for a non-finite value (exponent all ones) and a zero k; a chunk with either
             // is redone row by row below, which raises check_query's error for its first bad row.
             bool clean = true;
+            std::uint32_t nb = 0;
             for (std::uint64_t q = begin; q < end; ++q) {
                 const std::uint32_t* src = reinterpret_cast<const std::uint32_t*>(queries + q * dim);
                 std::uint32_t* dst = reinterpret_cast<std::uint32_t*>(staging + q * dim);
                 std::uint32_t bad = (ks ? ks[q] : k) == 0 ? 1u : 0u;
-                for (std::uint64_t i = 0; i < dim; ++i) {
-                    const std::uint32_t b = src[i];
-                    dst[i] = b;
-                    bad |= (b & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+                if (nonbyte != nullptr) {
+                    const float* f = queries + q * dim;
+                    for (std::uint64_t i = 0; i < dim; ++i) {
+                        const std::uint32_t b = src[i];
+                        dst[i] = b;
+                        bad |= (b & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+                        // a value outside [0, 255] (or NaN) becomes 0.5, never an integer
+                        const float c = (f[i] >= 0.0f && f[i] <= 255.0f) ? f[i] : 0.5f;
+                        nb |= static_cast<float>(static_cast<int>(c)) != c ? 1u : 0u;
+                    }
+                } else {
+                    for (std::uint64_t i = 0; i < dim; ++i) {
+                        const std::uint32_t b = src[i];
+                        dst[i] = b;
+                        bad |= (b & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+                    }
                 }
                 clean = clean && bad == 0;
             }
             if (clean) {
+                if (nb != 0) {
+                    nonbyte->store(1u, std::memory_order_relaxed);
+                }
                 return;
             }
+        }
+        if (nonbyte != nullptr) {
+            nonbyte->store(1u, std::memory_order_relaxed);
         }
         for (std::uint64_t q = begin; q < end; ++q) {
             const float* row = queries + q * query_dim;
@@ -506,10 +526,6 @@ void prepare_queries(vdx_gpu_ctx& ctx, const char* where, vdx::Metric metric, st
     const auto chunk = [&](std::size_t c) {
         const std::uint64_t begin = c * kHostChunk;
         stage(begin, std::min<std::uint64_t>(nq, begin + kHostChunk));
-        const std::size_t g = c / kPerGroup;
-        if (on_group && left[g].fetch_sub(1) == 1) {
-            on_group(g * kHostGroup, std::min<std::uint64_t>(nq, (g + 1) * kHostGroup));
-        }
     };
     if (ctx.host_pool) {
         ctx.host_pool->run(chunks, chunk);
@@ -882,28 +898,18 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
                              query_dim, ks ? ks[0] : k);
         }
         const vdx::Metric metric = static_cast<vdx::Metric>(h->metric);
-        // The host threads check and stage the queries; each group of rows crosses PCIe as soon
-        // as it is staged, while the rest are checked, and one search serves the whole batch (a
-        // search takes as long as its slowest queries, so splitting the batch into launches
-        // would pay that more than once). Every query is checked before anything reaches the
-        // caller's buffers, and the first bad row's error is the one reported, as with one pass.
-        const bool early = h->n > 0 && efmax <= kMaxBeamEf;
+        // The host threads check and stage the queries in mapped pinned memory, which the search
+        // kernel reads directly over PCIe (each query once, when a wave takes it up) and into
+        // which it writes the results: no copy phase before or after the launch. Every query is
+        // checked before anything reaches the caller's buffers, and the first bad row's error is
+        // the one reported, as with one pass. Over the 8-bit rows the same pass tells whether
+        // every query is byte-valued, which picks the integer kernel.
         hipStream_t s = ws->stream;
-        if (early) {
-            ws->queries.reserve(nq * h->dim * sizeof(float));
-        }
-        const int device = h->ctx->device;
-        const auto upload = [&](std::uint64_t b, std::uint64_t e) {
-            DeviceGuard g(device);
-            check(hipMemcpyAsync(ws->queries.as<float>() + b * h->dim, ws->h_queries.as<float>() + b * h->dim,
-                                 (e - b) * h->dim * sizeof(float), hipMemcpyHostToDevice, s),
-                  "hipMemcpyAsync(queries)");
-        };
+        const bool want_bytes = h->u8 && h->metric == 0 && h->dim == 128;
+        std::atomic<std::uint32_t> nonbyte{0};
         try {
             prepare_queries(*h->ctx, w, metric, h->dim, h->n, queries, nq, query_dim, ks, k,
-                            ws->h_queries.as<float>(),
-                            early ? std::function<void(std::uint64_t, std::uint64_t)>(upload)
-                                  : std::function<void(std::uint64_t, std::uint64_t)>());
+                            ws->h_queries.as<float>(), want_bytes ? &nonbyte : nullptr);
         } catch (...) {
             // nothing in flight may outlive the call (the workspace goes back to the pool)
             (void)hipStreamSynchronize(s);
@@ -911,14 +917,13 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         }
         if (h->n == 0) {
             std::fill(out_count, out_count + nq, 0u);
+            std::memset(static_cast<void*>(out), 0, nq * out_stride * sizeof(vdx_neighbor));
             return;
         }
         if (efmax > kMaxBeamEf) {
             throw std::invalid_argument(vdx::fmt("%s: ef %u is above the GPU path's limit of %u", w,
                                                  efmax, kMaxBeamEf));
         }
-        ws->out.reserve(nq * kmax * sizeof(vdx_neighbor));
-        ws->count.reserve(nq * sizeof(std::uint32_t));
         ws->counters.reserve(128);  // the queue counter, then the stats at byte 16
         ws->h_out.reserve(nq * kmax * sizeof(vdx_neighbor));
         ws->h_count.reserve((nq * 4 + 7) / 8 * 8 + sizeof(unsigned long long) * vdx::dev::kBeamStats);
@@ -953,14 +958,10 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
             dopts.max_expansions_per_query = ws->aux3.as<std::uint32_t>();
             kopts = &dopts;
         }
-        enqueue_beam(*h, ws->queries.as<float>(), nq, d_ks, k, d_efs, ef, efmax,
-                     ws->out.as<vdx::dev::Neighbor>(), kmax, ws->count.as<std::uint32_t>(),
-                     ws->counters.as<std::uint32_t>(), d_stats, kopts, s, nullptr, nullptr, 0, &ws->bitmap);
-        check(hipMemcpyAsync(ws->h_out.as<void>(), ws->out.as<void>(), nq * kmax * sizeof(vdx_neighbor),
-                             hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync(out)");
-        check(hipMemcpyAsync(ws->h_count.as<void>(), ws->count.as<void>(), nq * 4, hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync(count)");
+        enqueue_beam(*h, ws->h_queries.dev<float>(), nq, d_ks, k, d_efs, ef, efmax,
+                     ws->h_out.dev<vdx::dev::Neighbor>(), kmax, ws->h_count.dev<std::uint32_t>(),
+                     ws->counters.as<std::uint32_t>(), d_stats, kopts, s, nullptr, nullptr, 0, &ws->bitmap,
+                     want_bytes ? (nonbyte.load() != 0 ? 2 : 1) : 0);
         // stats after the counts, 8-byte aligned
         unsigned long long* host_stats =
             reinterpret_cast<unsigned long long*>(ws->h_count.as<char>() + (nq * 4 + 7) / 8 * 8);
@@ -972,14 +973,27 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         check(hipStreamSynchronize(s), "hnsw search");
         const vdx_neighbor* staged = ws->h_out.as<vdx_neighbor>();
         const std::uint32_t* staged_count = ws->h_count.as<std::uint32_t>();
+        // ranges of kUnpackChunk queries: one copy where every row is full and the rows are
+        // adjacent in both layouts, else row by row with the rest of each row zeroed
         const auto unpack = [&](std::size_t c) {
-            const std::uint64_t end = std::min<std::uint64_t>(nq, (c + 1) * kHostChunk);
-            for (std::uint64_t q = c * kHostChunk; q < end; ++q) {
-                out_count[q] = staged_count[q];
-                std::memcpy(out + q * out_stride, staged + q * kmax, staged_count[q] * sizeof(vdx_neighbor));
+            const std::uint64_t begin = c * kUnpackChunk;
+            const std::uint64_t end = std::min<std::uint64_t>(nq, begin + kUnpackChunk);
+            std::memcpy(out_count + begin, staged_count + begin, (end - begin) * sizeof(std::uint32_t));
+            bool full = out_stride == kmax;
+            for (std::uint64_t q = begin; q < end && full; ++q) {
+                full = staged_count[q] == kmax;
+            }
+            if (full) {
+                std::memcpy(out + begin * kmax, staged + begin * kmax, (end - begin) * kmax * sizeof(vdx_neighbor));
+                return;
+            }
+            for (std::uint64_t q = begin; q < end; ++q) {
+                const std::uint32_t c_q = staged_count[q];
+                std::memcpy(out + q * out_stride, staged + q * kmax, c_q * sizeof(vdx_neighbor));
+                std::memset(static_cast<void*>(out + q * out_stride + c_q), 0, (out_stride - c_q) * sizeof(vdx_neighbor));
             }
         };
-        h->ctx->host_pool->run(static_cast<std::size_t>((nq + kHostChunk - 1) / kHostChunk), unpack);
+        h->ctx->host_pool->run(static_cast<std::size_t>((nq + kUnpackChunk - 1) / kUnpackChunk), unpack);
         if (d_stats) {
             for (int i = 0; i < vdx::dev::kBeamStats; ++i) {
                 opts->stats[i] += host_stats[i];

@@ -37,12 +37,14 @@ void PinnedBuffer::reserve(std::size_t bytes) {
     release();
     check(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault), "hipHostMalloc");
     bytes_ = bytes;
+    check(hipHostGetDevicePointer(&dev_, ptr_, 0), "hipHostGetDevicePointer");
 }
 
 void PinnedBuffer::release() noexcept {
     if (ptr_ != nullptr) {
         (void)hipHostFree(ptr_);
         ptr_ = nullptr;
+        dev_ = nullptr;
         bytes_ = 0;
     }
 }

@@ -74,9 +74,15 @@ public:
     T* as() const noexcept {
         return static_cast<T*>(ptr_);
     }
+    // The allocation's address for kernels (mapped: a kernel reads and writes it over PCIe).
+    template <typename T>
+    T* dev() const noexcept {
+        return static_cast<T*>(dev_);
+    }
 
 private:
     void* ptr_ = nullptr;
+    void* dev_ = nullptr;
     std::size_t bytes_ = 0;
 };
 

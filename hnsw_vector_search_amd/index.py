@@ -192,8 +192,9 @@ class _GpuSearch:
         ks = None if np.isscalar(k) else np.ascontiguousarray(k, dtype=np.uint32)
         efs = None if np.isscalar(ef) else np.ascontiguousarray(ef, dtype=np.uint32)
         kmax = int(k) if ks is None else int(ks.max(initial=1))
-        out = np.zeros((nq, max(kmax, 1)), dtype=nat.NEIGHBOR_DTYPE)
-        count = np.zeros(nq, dtype=np.uint32)
+        # every entry is written (rows past their count zeroed)
+        out = np.empty((nq, max(kmax, 1)), dtype=nat.NEIGHBOR_DTYPE)
+        count = np.empty(nq, dtype=np.uint32)
         if nq == 0:
             return out, count
         handle = self.gpu_handle()

@@ -297,9 +297,11 @@ enum vdx_stat {
 /* search_batch over `nq` host queries of `query_dim` floats each (row-major, contiguous).
  * k and ef: per-query arrays `ks`/`efs` (may be NULL), else the uniform `k`/`ef`; ef below k acts
  * as k. Each query is validated like detail::check_query with `where` as the message prefix
- * ("HnswIndex::search_batch" if NULL) and normalized under cosine before upload.
+ * ("HnswIndex::search_batch" if NULL) and normalized under cosine, on the context's host threads,
+ * into pinned staging that the search kernel reads over PCIe (and into which it writes the
+ * results): the call makes no copy to or from the device.
  * Results: out[q * out_stride + i] for i < out_count[q] <= k, closest first, ties by smaller id
- * — element q equals HnswIndex::search(row q, k, ef). */
+ * — element q equals HnswIndex::search(row q, k, ef); the rest of each row is zeroed. */
 VDX_API int vdx_gpu_hnsw_search(vdx_gpu_hnsw* index, const float* queries, uint64_t nq,
                         uint64_t query_dim, const uint32_t* ks, uint32_t k, const uint32_t* efs,
                         uint32_t ef, vdx_neighbor* out, uint64_t out_stride, uint32_t* out_count,
