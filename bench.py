@@ -317,6 +317,7 @@ def main(argv=None) -> int:
     log(f"headline ({headline['mode']}, ef {headline['ef']}, {rows} rows): recall {rec:.4f}, {ms:.3f} ms, "
         f"{qps:,.0f} QPS; {expansions / nq:.1f} expansions, {(distances - dups) / nq:.1f} distances, "
         f"{alg_bytes / nq / 1e3:.1f} KB per query -> {achieved:.0f} GB/s")
+    gather = gather_ceiling(nat, run.ctx, len(base), achieved)
     f32_leg = None
     if rows == "u8":
         # the same launches over the FP32 rows: the kernel the row copy replaces
@@ -399,6 +400,7 @@ def main(argv=None) -> int:
                                                  headline["opts"].max_expansions if headline["opts"] else 0),
                      "algorithmic_bytes_per_launch": alg_bytes, "rows": rows},
         "cpu_baseline": cpu,
+        "gather_ceiling": gather,
         "fixed_ef": {"ef": fixed["ef"], "recall": fixed["recall"], "qps": fixed["qps"],
                      "best_fine_ef": fixed_best["ef"], "best_fine_qps": fixed_best["qps"],
                      "best_fine_recall": fixed_best["recall"]},
@@ -422,6 +424,46 @@ def main(argv=None) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def gather_ceiling(nat, ctx, n, achieved_gbs, reps=3):
+    """The rate random 128-byte row gathers reach on this GPU with no dependent loads between
+    them (vdx_gpu_gather_probe), over tables the size of the base's 8-bit rows (n x 128 B), of
+    those rows with the layer-0 link blocks (n x (128 + 132) B, what the search reads), and of
+    1 GiB (past the 256 MB last-level cache); 8,192 waves, 32 rows in flight each as in the
+    search's row loads, and a wave sweep on the second table. Best of `reps` event-timed runs.
+    The search's algorithmic rate over the second figure is the fraction of the practical
+    gather ceiling it reaches (its loads also wait on each other, which the probe's do not)."""
+    big = 1 << 30
+    table = nat.DeviceArray(big // 4, np.uint32, ctx)
+    sink = nat.DeviceArray(8192, np.uint32, ctx)
+    rows_read = ctypes.c_uint64(0)
+
+    def rate(table_bytes, waves, per_wave=4096):
+        best = None
+        for _ in range(reps):
+            nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 0))
+            nat.check(nat.lib.vdx_gpu_gather_probe(ctx.handle, table.ptr, table_bytes, 128, per_wave, waves,
+                                                   sink.ptr, ctypes.byref(rows_read)))
+            nat.check(nat.lib.vdx_gpu_event_record(ctx.handle, 1))
+            nat.check(nat.lib.vdx_gpu_synchronize(ctx.handle))
+            ms = nat.elapsed_ms(ctx, 0, 1)
+            best = ms if best is None else min(best, ms)
+        return round(rows_read.value * 128 / (best / 1000.0) / 1e9, 1)
+
+    rows_only = min(big, n * 128)
+    with_blocks = min(big, n * (128 + 132))
+    out = {"row_bytes": 128, "waves": 8192,
+           "rows_table_bytes": rows_only, "rows_gbs": rate(rows_only, 8192),
+           "rows_and_blocks_table_bytes": with_blocks, "rows_and_blocks_gbs": rate(with_blocks, 8192),
+           "table_1gib_gbs": rate(big, 8192),
+           "waves_sweep_rows_and_blocks": {w: rate(with_blocks, w) for w in (1024, 2048, 4096, 7168)}}
+    out["search_fraction_of_ceiling"] = round(achieved_gbs / out["rows_and_blocks_gbs"], 3)
+    log(f"gather ceiling (random 128 B rows, 8,192 waves): {out['rows_gbs']:.0f} GB/s over {rows_only >> 20} MB, "
+        f"{out['rows_and_blocks_gbs']:.0f} GB/s over {with_blocks >> 20} MB, {out['table_1gib_gbs']:.0f} GB/s over 1 GiB; "
+        f"waves {out['waves_sweep_rows_and_blocks']}; the search reaches {out['search_fraction_of_ceiling']:.2f} of it")
+    del table, sink
+    return out
 
 
 def run_single_query(args, nat, index, queries, threads, efs=(36, 64)):

@@ -133,6 +133,7 @@ _SIGS = {
     "vdx_gpu_copy_to_device": (c_int, [c_ptr, c_ptr, c_ptr, c_u64]),
     "vdx_gpu_copy_to_host": (c_int, [c_ptr, c_ptr, c_ptr, c_u64]),
     "vdx_gpu_copy_device": (c_int, [c_ptr, c_ptr, c_ptr, c_u64]),
+    "vdx_gpu_gather_probe": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_u32, c_ptr, c_ptr]),
     "vdx_gpu_synchronize": (c_int, [c_ptr]),
     "vdx_gpu_event_record": (c_int, [c_ptr, c_u32]),
     "vdx_gpu_event_elapsed": (c_int, [c_ptr, c_u32, c_u32, ctypes.POINTER(c_float)]),

@@ -609,6 +609,31 @@ int vdx_gpu_copy_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t b
     });
 }
 
+int vdx_gpu_gather_probe(vdx_gpu_ctx* ctx, const void* table, uint64_t table_bytes, uint32_t row_bytes,
+                         uint32_t rows_per_wave, uint32_t waves, void* sink, uint64_t* rows_read) {
+    return guarded([&] {
+        require(ctx != nullptr && table != nullptr && sink != nullptr, "vdx_gpu_gather_probe: null argument");
+        require(row_bytes >= 16 && row_bytes <= 1024 && row_bytes % 16 == 0 && 64 % (row_bytes / 16) == 0,
+                "vdx_gpu_gather_probe: row_bytes must be 16, 32, ..., 1024 (16 bytes a lane, lanes dividing 64)");
+        require(waves >= 4 && waves % 4 == 0, "vdx_gpu_gather_probe: waves must be a positive multiple of 4");
+        require(reinterpret_cast<std::uintptr_t>(table) % 16 == 0, "vdx_gpu_gather_probe: table must be 16-byte aligned");
+        const std::uint64_t rows = table_bytes / row_bytes;
+        require(rows >= 1 && rows <= 0xFFFFFFFFull, "vdx_gpu_gather_probe: the table holds no row or too many");
+        const std::uint64_t round = (64 / (row_bytes / 16)) * 4;
+        const std::uint64_t per_wave = (rows_per_wave + round - 1) / round * round;
+        require(per_wave <= 0xFFFFFFFFull - round, "vdx_gpu_gather_probe: rows_per_wave too large");
+        DeviceGuard guard(ctx->device);
+        check(vdx::dev::launch_gather_probe(table, static_cast<std::uint32_t>(rows), row_bytes,
+                                            static_cast<std::uint32_t>(per_wave), waves,
+                                            static_cast<std::uint32_t*>(sink), ctx->stream),
+              "gather_probe launch");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (rows_read != nullptr) {
+            *rows_read = per_wave * waves;
+        }
+    });
+}
+
 int vdx_gpu_copy_to_host(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     return guarded([&] {
         require(ctx != nullptr, "vdx_gpu_copy_to_host: null context");

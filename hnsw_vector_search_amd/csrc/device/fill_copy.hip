@@ -127,6 +127,64 @@ hipError_t launch_fill_u32(std::uint32_t* dst, std::uint32_t value, std::uint64_
     return hipGetLastError();
 }
 
+// Random row gathers (the access of the beam's row and link-block reads, without the chain of
+// dependent loads between them): each wave reads `per_wave` rows of row_bytes at ids drawn by a
+// hash, row_bytes / 16 lanes per row (16 bytes a lane), four rows per lane group in flight, and
+// leaves the XOR of everything it read in sink[wave] (so no load is dead).
+__device__ __forceinline__ std::uint32_t probe_hash(std::uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void gather_probe_kernel(const unsigned char* table, std::uint32_t rows,
+                                                           std::uint32_t row_bytes, std::uint32_t per_wave,
+                                                           std::uint32_t* sink) {
+    constexpr std::uint32_t kRR = 4;
+    const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t lpr = row_bytes / 16;  // lanes per row, 1..64
+    const std::uint32_t groups = 64 / lpr;      // rows per load instruction
+    const std::uint32_t g = lane / lpr;
+    const std::uint32_t m = lane % lpr;
+    const bool active = g < groups;
+    std::uint32_t acc = 0;
+    for (std::uint32_t it = 0; it < per_wave; it += groups * kRR) {
+        uint4 v[kRR];
+#pragma unroll
+        for (std::uint32_t r = 0; r < kRR; ++r) {
+            const std::uint32_t h = probe_hash(wave * 0x9E3779B9u + (it + r * groups + g) * 0x85EBCA6Bu + 0x632BE5ABu);
+            const std::uint32_t id = static_cast<std::uint32_t>((static_cast<std::uint64_t>(h) * rows) >> 32);
+            v[r] = active ? *reinterpret_cast<const uint4*>(table + static_cast<std::size_t>(id) * row_bytes + 16 * m)
+                          : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (std::uint32_t r = 0; r < kRR; ++r) {
+            acc ^= v[r].x ^ v[r].y ^ v[r].z ^ v[r].w;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc ^= static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(acc), off, 64));
+    }
+    if (lane == 0) {
+        sink[wave] = acc;
+    }
+}
+
+hipError_t launch_gather_probe(const void* table, std::uint32_t rows, std::uint32_t row_bytes,
+                               std::uint32_t per_wave, std::uint32_t waves, std::uint32_t* sink, hipStream_t s) {
+    if (waves == 0 || waves % 4 != 0 || rows == 0 || row_bytes < 16 || row_bytes > 1024 || row_bytes % 16 != 0 ||
+        (64 % (row_bytes / 16)) != 0 || reinterpret_cast<std::uintptr_t>(table) % 16 != 0) {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(gather_probe_kernel, dim3(waves / 4), dim3(256), 0, s,
+                       static_cast<const unsigned char*>(table), rows, row_bytes, per_wave, sink);
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter_rows(std::uint32_t* dst, const std::uint32_t* src, const std::uint32_t* ids,
                                std::uint64_t rows, std::uint32_t row_len, hipStream_t s) {
     if (rows == 0 || row_len == 0) return hipSuccess;

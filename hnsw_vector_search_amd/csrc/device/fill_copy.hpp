@@ -21,6 +21,11 @@ hipError_t launch_gather_rows(float* dst, std::uint32_t dst_ld, const float* src
 // element is not an integer in [0, 255]. ld >= 128, a multiple of 4.
 hipError_t launch_pack_rows_u8(unsigned char* dst, const float* src, std::uint32_t ld, std::uint64_t first,
                                std::uint64_t rows, std::uint32_t* bad, hipStream_t s);
+// Random gathers of `per_wave` rows (rounded up to whole rounds) of row_bytes (16..1024, a
+// multiple of 16 dividing 1024) from rows [0, rows) of `table` (16-byte aligned) by each of
+// `waves` waves (a multiple of 4); sink[waves] receives one word per wave.
+hipError_t launch_gather_probe(const void* table, std::uint32_t rows, std::uint32_t row_bytes,
+                               std::uint32_t per_wave, std::uint32_t waves, std::uint32_t* sink, hipStream_t s);
 // dst and src 16-byte aligned (device allocations are).
 hipError_t launch_copy_bytes(void* dst, const void* src, std::uint64_t bytes, hipStream_t s);
 // Writes `epoch` to *flag if any of the `count` floats at `queries` is not an integer in

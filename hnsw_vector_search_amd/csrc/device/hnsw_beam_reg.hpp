@@ -423,25 +423,33 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
             // Merge: keys that cannot enter drop first (R full and not closer than its worst).
             const std::uint64_t worst = r_at<CH>(Rk, size - 1) & kKeyMask;
             const bool cand = static_cast<std::uint32_t>(lane) < nf && (size < efp || f < worst);
+            // One pass over the candidates: for each, the members of R below it (its rank in R,
+            // written to its own lane) and whether R holds it already; every lane counts the
+            // kept candidates below its own key and below its R entries. Everything else
+            // follows after the pass: an R entry moves up by the kept candidates not below it,
+            // and the closest kept candidate is the one with none below it.
             std::uint64_t rem = ballot(cand);
-            std::uint32_t gt[CH];
+            std::uint64_t Rm[CH];
+            std::uint32_t r_below[CH];
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                gt[c] = 0;
+                Rm[c] = Rk[c] & kKeyMask;
+                r_below[c] = 0;
             }
-            std::uint32_t lb = 0, below = 0, nins = 0, fmin_pos = kNone;
-            std::uint64_t dupmask = 0, fmin = kSentinel;
+            std::uint32_t lb = 0, below = 0, nins = 0;
+            std::uint64_t dupmask = 0;
             while (rem != 0) {
-                const std::uint32_t j = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(rem)) - 1);
-                rem &= rem - 1;
+                const std::uint32_t j = static_cast<std::uint32_t>(__builtin_ctzll(rem));
+                rem &= ~(1ull << j);
                 const std::uint64_t fj = readlane64(f, j);
                 bool eq = false;
                 std::uint32_t lt = 0;
+                bool r_lt[CH];
 #pragma unroll
                 for (int c = 0; c < CH; ++c) {
-                    const std::uint64_t rm = Rk[c] & kKeyMask;
-                    eq |= ballot(rm == fj) != 0;
-                    lt += static_cast<std::uint32_t>(__popcll(ballot(rm < fj)));
+                    eq |= ballot(Rm[c] == fj) != 0;
+                    r_lt[c] = Rm[c] < fj;
+                    lt += static_cast<std::uint32_t>(__popcll(ballot(r_lt[c])));
                 }
                 if (eq) {
                     dupmask |= 1ull << j;  // already in R: a node the visited cache forgot
@@ -449,20 +457,26 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
                 }
 #pragma unroll
                 for (int c = 0; c < CH; ++c) {
-                    gt[c] += fj < (Rk[c] & kKeyMask) ? 1u : 0u;
+                    r_below[c] += r_lt[c] ? 1u : 0u;
                 }
                 below += fj < f ? 1u : 0u;
-                if (static_cast<std::uint32_t>(lane) == j) {
-                    lb = lt;
-                }
+                lb = static_cast<std::uint32_t>(lane) == j ? lt : lb;
                 ++nins;
-                if (fj < fmin) {
-                    fmin = fj;
-                    fmin_pos = lt;
-                }
+            }
+            std::uint32_t gt[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                gt[c] = nins - r_below[c];  // kept candidates below this R entry (none equals it)
             }
             n_dup += static_cast<unsigned long long>(__popcll(dupmask));
             const bool kept = cand && ((dupmask >> lane) & 1u) == 0;
+            std::uint32_t fmin_pos = kNone;
+            std::uint64_t fmin = kSentinel;
+            if (nins > 0) {
+                const int jmin = __builtin_ctzll(ballot(kept && below == 0));
+                fmin = readlane64(f, static_cast<std::uint32_t>(jmin));
+                fmin_pos = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lb), jmin));
+            }
             const std::uint32_t pos = lb + below;
             const bool ins = kept && pos < efp;
             std::uint32_t inserted = 0;

@@ -210,6 +210,15 @@ VDX_API int vdx_gpu_copy_to_host(vdx_gpu_ctx* ctx, void* dst, const void* src, u
 /* Device-to-device copy by a plain kernel (16-byte aligned pointers): also the reference
  * transfer of known size for calibrating HBM traffic counters. No counterpart in vecdex. */
 VDX_API int vdx_gpu_copy_device(vdx_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Random row gathers of known size (no counterpart in vecdex): `waves` waves (a multiple of 4)
+ * each read rows_per_wave rows (rounded up to whole rounds; *rows_read = the total) of row_bytes
+ * (16..1024, 16 bytes a lane, lanes dividing 64) at hashed ids among the table_bytes / row_bytes
+ * rows of `table` (device memory, 16-byte aligned), four rows per lane group in flight; one word
+ * per wave goes to `sink` (4 * waves bytes of device memory). The rate random gathers reach with
+ * no dependent loads between them: the ceiling the search's row and link-block reads face.
+ * Synchronous on the context's stream (time it with vdx_gpu_event_record around it). */
+VDX_API int vdx_gpu_gather_probe(vdx_gpu_ctx* ctx, const void* table, uint64_t table_bytes, uint32_t row_bytes,
+                                 uint32_t rows_per_wave, uint32_t waves, void* sink, uint64_t* rows_read);
 VDX_API int vdx_gpu_synchronize(vdx_gpu_ctx* ctx);
 /* Records event `slot` (0..15) on the context's stream; elapsed milliseconds between two
  * recorded events once both completed. */

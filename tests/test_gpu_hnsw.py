@@ -362,3 +362,47 @@ def test_large_host_batch_rejects_a_bad_row_in_any_group(gmm10k):
     ref, ref_cnt = o.hnsw_search(index.graph_arrays(), queries, 10, 40)
     np.testing.assert_array_equal(c, ref_cnt)
     assert got.tobytes() == ref.tobytes()
+
+
+# ---- the random-gather probe (the practical ceiling of the search's reads) ------------------
+
+def _probe_hash(x):
+    x = np.uint64(x) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    return int(x)
+
+
+@pytest.mark.parametrize("row_bytes", [128, 64])
+def test_gather_probe_reads_the_rows_it_names(row_bytes):
+    """Each wave's sink word is the XOR of every word of the rows its hash names (recomputed
+    here), so the probe reads what it claims and nothing outside the table."""
+    print(f"case: gather probe, row_bytes {row_bytes}", flush=True)
+    rows, waves, per_wave = 1000, 8, 64
+    words = row_bytes // 4
+    data = np.random.default_rng(5).integers(0, 2**32, (rows, words), dtype=np.uint64).astype(np.uint32)
+    ctx = nat.Context.get(0)
+    table = nat.DeviceArray.from_numpy(data, ctx)
+    sink = nat.DeviceArray(waves, np.uint32, ctx)
+    got_rows = ctypes.c_uint64(0)
+    nat.check(nat.lib.vdx_gpu_gather_probe(ctx.handle, table.ptr, rows * row_bytes, row_bytes, per_wave, waves,
+                                           sink.ptr, ctypes.byref(got_rows)))
+    assert got_rows.value == waves * per_wave
+    row_xor = np.bitwise_xor.reduce(data, axis=1)
+    groups = 64 // (row_bytes // 16)
+    want = []
+    for w in range(waves):
+        acc = 0
+        for it in range(0, per_wave, groups * 4):
+            for r in range(4):
+                for g in range(groups):
+                    h = _probe_hash((w * 0x9E3779B9 + (it + r * groups + g) * 0x85EBCA6B + 0x632BE5AB) & 0xFFFFFFFF)
+                    acc ^= int(row_xor[(h * rows) >> 32])
+        want.append(acc)
+    assert sink.numpy().tolist() == want
+    for bad in ((24, 8), (128, 6)):
+        assert nat.lib.vdx_gpu_gather_probe(ctx.handle, table.ptr, rows * row_bytes, bad[0], per_wave, bad[1],
+                                            sink.ptr, None) != 0
