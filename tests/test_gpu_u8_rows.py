@@ -203,3 +203,22 @@ def test_integer_arithmetic_only_when_every_query_is_byte_valued(case):
             ref, _ = index.search_arrays(q[sel], 10, int(ef), opts=f32_rows())
             assert got[sel].tobytes() == ref.tobytes(), (case, ef, graph)
     del batch
+
+
+@pytest.mark.parametrize("value", [256.0, 0.5, -1.0, 255.25])
+def test_one_non_byte_query_in_a_host_batch_takes_the_float_kernel(value):
+    """The host entry point decides between the integer and the float kernel while it stages
+    the queries: a single component that is not an integer in [0, 255], in one query of a batch
+    of many staging chunks, must send the whole batch to the float kernel (the integer kernel
+    would read it as a different byte), and a batch of bytes (with -0.0 among them) to the
+    integer kernel; either way every query equals the oracle."""
+    print(f"case: host batch with one non-byte component {value}", flush=True)
+    base, queries = datasets.gmm_base_and_queries(datasets.gmm64(), 8000, 1000)
+    index = vdx.HnswIndex(128, vdx.Metric.L2, M=12, ef_construction=80, seed=3)
+    index.add(base, num_threads=8)
+    assert index.gpu_u8_rows()
+    q = np.array(queries, dtype=np.float32)
+    q[3, 5] = np.float32(-0.0)
+    assert_matches_oracle(index, q, 10, 40)
+    q[617, 77] = np.float32(value)
+    assert_matches_oracle(index, q, 10, 40)
