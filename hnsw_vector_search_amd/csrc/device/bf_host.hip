@@ -144,10 +144,12 @@ void bf_mfma(vdx_gpu_bf& b, int prec, const float* d_queries, std::uint64_t nq, 
         // The fraction of the base within tau is about Gamma(rank) / m. The shortlist and its
         // certification are the same for any tau that keeps more than S rows; with fewer, tau
         // itself bounds what was left out, which certifies unless barely k rows survive. rank
-        // >= 6 and two above the rank of 2S survivors make fewer than k survivors a 1e-6 event
-        // or rarer, while every survivor costs filter and shortlist work; the buffer holds 6x the
-        // expected survivors, so overflows (redone exactly) are rarer still.
-        rank = static_cast<std::uint32_t>(std::min<double>(m, std::max(6.0, std::ceil(2.0 * S * m / b.n) + 2.0)));
+        // >= 4 and two above the rank of 2S survivors make fewer than k survivors about a 2e-6
+        // event per query (the fraction within tau ~ Beta(rank, m); P ~ (m k / n)^rank / rank!),
+        // while every survivor costs filter and shortlist work (the filter's staging of
+        // survivors is about a quarter of its time at rank 6); the buffer holds 6x the expected
+        // survivors, so overflows (redone exactly) are rarer still.
+        rank = static_cast<std::uint32_t>(std::min<double>(m, std::max(4.0, std::ceil(2.0 * S * m / b.n) + 2.0)));
         const double expect = static_cast<double>(rank) * b.n / m;
         cap = static_cast<std::uint32_t>(std::clamp<double>(std::ceil(6.0 * expect / 256) * 256, 1024, 16384));
         cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(cap, op.rows));
