@@ -34,6 +34,9 @@ struct BeamArgs {
     std::uint32_t qflag_epoch;
     // the queries (prepared), per-query k and ef
     const float* queries;
+    // the integer kernel's queries as the host staged them when every one is byte-valued: 128
+    // bytes each in element order (queries is then not read), or null
+    const std::uint8_t* qbytes;
     std::uint32_t qld, nq;
     const std::uint32_t* ks;
     const std::uint32_t* efs;

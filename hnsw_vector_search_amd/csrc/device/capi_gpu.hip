@@ -122,7 +122,8 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
                   std::uint32_t* d_count, std::uint32_t* d_queue, unsigned long long* d_stats,
                   const vdx_search_opts* opts, hipStream_t stream,
                   const vdx::dev::BeamArgs* trace = nullptr, const std::uint32_t* d_entries = nullptr,
-                  std::uint32_t layer = 0, DeviceBuffer* bitmap = nullptr, int host_bytes = 0) {
+                  std::uint32_t layer = 0, DeviceBuffer* bitmap = nullptr, int host_bytes = 0,
+                  const std::uint8_t* d_qbytes = nullptr) {
     const int mode = vdx::gpu::beam_mode(opts, trace != nullptr);
     const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(h, max_efp, nq, vdx::gpu::config_from(opts), mode);
     vdx::dev::BeamArgs a = vdx::gpu::beam_args(h, plan, mode, opts, d_queries, nq, d_ks, k, d_efs, ef, d_out,
@@ -152,6 +153,9 @@ void enqueue_beam(vdx_gpu_hnsw& h, const float* d_queries, std::uint64_t nq, con
         if (host_bytes == 1) {
             a.qflag = d_queue + 1;
             a.qflag_epoch = 1;
+            // the integer kernel alone reads them (the LDS-resident beam reads the floats)
+            require(d_qbytes == nullptr || plan.chunks > 0, "byte-staged queries need the register-resident beam");
+            a.qbytes = d_qbytes;
             int_only = true;
         }
     } else if (plan.shape == vdx::dev::BeamShape::D128U8) {
@@ -469,15 +473,47 @@ constexpr std::uint64_t kUnpackChunk = 1024;
 // Checks every query (vecdex's check_query, its messages; the first bad row's error, as a
 // sequential loop reports) and writes the prepared rows (normalized under cosine) to `staging`,
 // in chunks on the context's host threads. `nonbyte`, when given, is set to 1 unless every
-// component staged is an integer in [0, 255] (the 8-bit rows' integer kernel).
+// component staged is an integer in [0, 255] (the 8-bit rows' integer kernel). With `bytes`
+// as well (dimension 128), a batch whose every component is such an integer is staged there
+// instead, one byte per component in order, and `staging` is left unwritten; any other batch
+// is staged as floats in full.
 void prepare_queries(vdx_gpu_ctx& ctx, const char* where, vdx::Metric metric, std::uint64_t dim,
                      std::uint64_t size, const float* queries, std::uint64_t nq, std::uint64_t query_dim,
                      const std::uint32_t* ks, std::uint32_t k, float* staging,
-                     std::atomic<std::uint32_t>* nonbyte = nullptr) {
+                     std::atomic<std::uint32_t>* nonbyte = nullptr, std::uint8_t* bytes = nullptr) {
     const vdx::IndexShape shape{metric, static_cast<std::size_t>(dim), static_cast<std::size_t>(size)};
     const std::string w(where);
     const std::size_t chunks = static_cast<std::size_t>((nq + kHostChunk - 1) / kHostChunk);
+    // chunks staged as bytes (to be restaged as floats if another chunk is not byte-valued)
+    std::unique_ptr<std::uint8_t[]> as_bytes(bytes != nullptr ? new std::uint8_t[chunks]() : nullptr);
     const auto stage = [&](std::uint64_t begin, std::uint64_t end) {
+        if (bytes != nullptr && nonbyte != nullptr && metric != vdx::Metric::Cosine && query_dim == dim &&
+            nonbyte->load(std::memory_order_relaxed) == 0) {
+            // one pass that checks, then the bytes if every value is a byte (else the float
+            // path below, which checks again)
+            std::uint32_t bad = 0, nb = 0;
+            for (std::uint64_t q = begin; q < end; ++q) {
+                const std::uint32_t* src = reinterpret_cast<const std::uint32_t*>(queries + q * dim);
+                const float* f = queries + q * dim;
+                bad |= (ks ? ks[q] : k) == 0 ? 1u : 0u;
+                for (std::uint64_t i = 0; i < dim; ++i) {
+                    bad |= (src[i] & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+                    const float c = (f[i] >= 0.0f && f[i] <= 255.0f) ? f[i] : 0.5f;
+                    nb |= static_cast<float>(static_cast<int>(c)) != c ? 1u : 0u;
+                }
+            }
+            if (bad == 0 && nb == 0) {
+                for (std::uint64_t q = begin; q < end; ++q) {
+                    const float* f = queries + q * dim;
+                    std::uint8_t* dst = bytes + q * dim;
+                    for (std::uint64_t i = 0; i < dim; ++i) {
+                        dst[i] = static_cast<std::uint8_t>(static_cast<int>(f[i]));
+                    }
+                }
+                as_bytes[begin / kHostChunk] = 1;
+                return;
+            }
+        }
         if (metric != vdx::Metric::Cosine && query_dim == dim) {
             // Squared L2 and inner product stage the rows as they are: one pass copies them and
             // looks for a non-finite value (exponent all ones) and a zero k; a chunk with either
@@ -531,6 +567,15 @@ void prepare_queries(vdx_gpu_ctx& ctx, const char* where, vdx::Metric metric, st
         ctx.host_pool->run(chunks, chunk);
     } else {
         for (std::size_t c = 0; c < chunks; ++c) chunk(c);
+    }
+    if (bytes != nullptr && nonbyte != nullptr && nonbyte->load() != 0) {
+        // a mixed batch runs on floats: the chunks staged as bytes (checked) are copied too
+        for (std::size_t c = 0; c < chunks; ++c) {
+            if (as_bytes[c]) {
+                const std::uint64_t b = c * kHostChunk, e = std::min<std::uint64_t>(nq, b + kHostChunk);
+                std::memcpy(staging + b * dim, queries + b * dim, (e - b) * dim * sizeof(float));
+            }
+        }
     }
 }
 
@@ -930,11 +975,28 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         // the one reported, as with one pass. Over the 8-bit rows the same pass tells whether
         // every query is byte-valued, which picks the integer kernel.
         hipStream_t s = ws->stream;
-        const bool want_bytes = h->u8 && h->metric == 0 && h->dim == 128;
+        const bool want_bytes = h->u8 && h->metric == 0 && h->dim == 128 && query_dim == 128;
         std::atomic<std::uint32_t> nonbyte{0};
+        // Byte-valued queries cross PCIe as bytes (a quarter of the floats) when the integer
+        // kernel is the one that will read them: the register-resident beam over the 8-bit rows.
+        // An option error here is left for the launch to report, after the query checks.
+        bool stage_bytes = false;
+        if (want_bytes && efmax <= kMaxBeamEf) {
+            try {
+                const vdx::gpu::BeamPlan plan = vdx::gpu::plan_beam(
+                    *h, efmax, nq, vdx::gpu::config_from(opts), vdx::gpu::beam_mode(opts, false));
+                stage_bytes = plan.shape == vdx::dev::BeamShape::D128U8 && plan.chunks > 0;
+            } catch (const std::exception&) {
+                stage_bytes = false;
+            }
+        }
+        if (stage_bytes) {
+            ws->h_qbytes.reserve(nq * 128);
+        }
         try {
             prepare_queries(*h->ctx, w, metric, h->dim, h->n, queries, nq, query_dim, ks, k,
-                            ws->h_queries.as<float>(), want_bytes ? &nonbyte : nullptr);
+                            ws->h_queries.as<float>(), want_bytes ? &nonbyte : nullptr,
+                            stage_bytes ? ws->h_qbytes.as<std::uint8_t>() : nullptr);
         } catch (...) {
             // nothing in flight may outlive the call (the workspace goes back to the pool)
             (void)hipStreamSynchronize(s);
@@ -986,7 +1048,8 @@ int vdx_gpu_hnsw_search(vdx_gpu_hnsw* h, const float* queries, uint64_t nq, uint
         enqueue_beam(*h, ws->h_queries.dev<float>(), nq, d_ks, k, d_efs, ef, efmax,
                      ws->h_out.dev<vdx::dev::Neighbor>(), kmax, ws->h_count.dev<std::uint32_t>(),
                      ws->counters.as<std::uint32_t>(), d_stats, kopts, s, nullptr, nullptr, 0, &ws->bitmap,
-                     want_bytes ? (nonbyte.load() != 0 ? 2 : 1) : 0);
+                     want_bytes ? (nonbyte.load() != 0 ? 2 : 1) : 0,
+                     stage_bytes && nonbyte.load() == 0 ? ws->h_qbytes.dev<std::uint8_t>() : nullptr);
         // stats after the counts, 8-byte aligned
         unsigned long long* host_stats =
             reinterpret_cast<unsigned long long*>(ws->h_count.as<char>() + (nq * 4 + 7) / 8 * 8);

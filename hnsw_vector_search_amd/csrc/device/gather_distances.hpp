@@ -301,6 +301,19 @@ __device__ __forceinline__ int dpp_xor1_i(int v) { return __builtin_amdgcn_updat
 __device__ __forceinline__ int dpp_xor2_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true); }
 __device__ __forceinline__ int dpp_next4_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, true); }
 
+// The sum of squares of a query held as byte words by the 8 lanes of a group (every lane).
+__device__ __forceinline__ std::uint32_t bytes_sum_sq(const std::uint32_t (&qw)[4]) {
+    std::uint32_t qq = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        qq = __builtin_amdgcn_udot4(qw[i], qw[i], qq, false);
+    }
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 1, 8));
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 2, 8));
+    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 4, 8));
+    return qq;
+}
+
 // The query as byte words in the row copy's lane order (byte j of word i is element
 // 32i + 4m + j of lane m's slice), when every component of it is an integer in [0, 255]:
 // returns false (wave-uniform) otherwise. sum_sq: the query's sum of squares (every lane).
@@ -317,15 +330,7 @@ __device__ __forceinline__ bool query_as_bytes(const float4 (&qb)[4], std::uint3
         }
         qw[i] = w;
     }
-    std::uint32_t qq = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        qq = __builtin_amdgcn_udot4(qw[i], qw[i], qq, false);
-    }
-    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 1, 8));
-    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 2, 8));
-    qq += static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(qq), 4, 8));
-    sum_sq = qq;
+    sum_sq = bytes_sum_sq(qw);
     return __ballot(!ok) == 0;
 }
 

@@ -96,6 +96,7 @@ struct Workspace {
     DeviceBuffer bitmap;     // the HNSW search's visited bitmaps (visited_form 2)
     DeviceBuffer fp64[4];    // the brute force's float64 pass: window, counts, distances, counters
     PinnedBuffer h_queries, h_out, h_count;
+    PinnedBuffer h_qbytes;  // byte-valued queries staged as 128 bytes each (the integer kernel)
     ~Workspace();
 };
 

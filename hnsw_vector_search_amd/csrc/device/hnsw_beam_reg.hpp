@@ -173,16 +173,26 @@ __device__ __forceinline__ void beam_reg_body(const BeamArgs& a) {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             qb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (FIXED || i < nb) {
+            if ((FIXED || i < nb) && (!QI || a.qbytes == nullptr)) {
                 const float* p = qp + 32 * i + 4 * m;
                 qb[i] = make_float4(p[0], p[1], p[2], p[3]);
             }
         }
-        // QI: the query as byte words (the floats are dead from here)
+        // QI: the query as byte words (the floats are dead from here), or as the host staged
+        // it, 128 bytes in element order (lane m's words: elements 32i + 4m .. 32i + 4m + 3)
         std::uint32_t qw[4] = {0u, 0u, 0u, 0u};
         std::uint32_t qq = 0;
         if constexpr (QI) {
-            (void)query_as_bytes(qb, qw, qq);
+            if (a.qbytes != nullptr) {
+                const std::uint32_t* qwp = reinterpret_cast<const std::uint32_t*>(a.qbytes + static_cast<std::size_t>(q) * 128) + m;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    qw[i] = qwp[8 * i];
+                }
+                qq = bytes_sum_sq(qw);
+            } else {
+                (void)query_as_bytes(qb, qw, qq);
+            }
         }
 #pragma unroll
         for (int s = 0; s < N8; ++s) {
