@@ -645,12 +645,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 1 ? 6
 // integers in [0, 255] (*qflag != qflag_epoch after query_bytes_check): exact integer
 // arithmetic with the query held as 4 byte words, so the same distances in a kernel of its own
 // (sharing one kernel with the float path cost both bodies registers and a wave per SIMD).
-// Held to 64 VGPRs (8 waves per SIMD) up to R capacity 256, a few spilled registers cheaper
-// than the wave they buy (measured: ef 36 / 64 0.3875 / 0.5304 -> 0.3848 / 0.5150 ms).
+// Held to 72 VGPRs (7 waves per SIMD) up to R capacity 256: the occupancy its LDS allows (seven
+// 4-wave workgroups per CU), with no spilled register at R capacity 64 (64 VGPRs spilled six:
+// ef 36 0.386 -> 0.377 ms, the expansion-budget launch 0.354 -> 0.344 ms; 80 VGPRs and 6 waves
+// had been slower still).
 // Enqueued together with hnsw_beam_reg_plain_kernel<U8>: whichever the flag does not select
 // returns at once.
 template <int CH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 8 : 4))) void hnsw_beam_reg_u8int_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 7 : 4))) void hnsw_beam_reg_u8int_kernel(
     BeamArgs a) {
     if (a.qflag == nullptr || *a.qflag == a.qflag_epoch) {
         return;
