@@ -629,9 +629,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
 
 // The plain search over the 8-bit rows for a launch with a query that is not byte-valued
 // (*qflag == qflag_epoch after query_bytes_check; the float arithmetic over the converted bytes).
-// Held to 80 VGPRs (6 waves per SIMD, the occupancy its LDS allows) at R capacity 64.
+// Held to 72 VGPRs at R capacity 64: 7 waves per SIMD, the occupancy its LDS allows, for six
+// spilled registers (at 80 VGPRs and 6 waves: fractional queries, ef 64 0.587 -> 0.574 ms and
+// ef 36 0.421 -> 0.418 ms).
 template <int CH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 1 ? 6 : 4))) void hnsw_beam_reg_u8_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CH <= 1 ? 7 : 4))) void hnsw_beam_reg_u8_kernel(
     BeamArgs a) {
     // a launch whose queries are all byte-valued is served by hnsw_beam_reg_u8int_kernel,
     // enqueued with this one
